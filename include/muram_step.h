@@ -1,0 +1,173 @@
+/*
+ * muram_step.h -- C ABI of the MI355X-native MURaM per-timestep MHD update.
+ *
+ * One opaque context per MURaM GridData (one GPU, one HIP stream).  The grid
+ * state lives in HBM as structure-of-arrays (one array per cState component,
+ * dim 0 fastest, the reference's node order); host AoS <-> device SoA
+ * conversion happens only in mrm_set_state / mrm_get_state, which are the
+ * counterparts of ACCH::UpdateGPU / ACCH::UpdateCPU (include/ACCH.h:49-50).
+ *
+ * Every function returns MRM_OK (0) or a negative MRM_ERR_* code; the message
+ * of the last failure is available from mrm_last_error().  The reference aborts
+ * the MPI job on the same conditions (solver.C:497-507, :529-534,
+ * eos_init.C:74-76); the C++ shim maps the codes back onto MPI_Abort.
+ * Calls on one context are not re-entrant; all device work of a context is
+ * ordered on its stream.
+ */
+#ifndef MURAM_STEP_H
+#define MURAM_STEP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRM_OK               0
+#define MRM_ERR_ARG         -1   /* bad argument or unsupported parameter */
+#define MRM_ERR_HIP         -2   /* HIP runtime failure (message has details) */
+#define MRM_ERR_EOS         -3   /* EOS table missing or malformed (eos_init.C:74-76) */
+#define MRM_ERR_INVALID_DT  -4   /* dt NaN, <= 0 or inf (solver.C:497-507) */
+#define MRM_ERR_CFL         -5   /* cfl >= 2 (solver.C:529-534) */
+#define MRM_ERR_STATE       -6   /* call order violated (e.g. no EOS loaded) */
+#define MRM_ERR_UNSUPPORTED -7   /* physics option outside the implemented path */
+
+/* storage precision of the grid state */
+#define MRM_PREC_FP64        0   /* plain method: everything in FP64 */
+#define MRM_PREC_FP32_MIXED  1   /* FP32 state/stencils; FP64 energy, e_int recovery, dt */
+
+/* which state array (GridData::U, U0, Res; grid.H:52-55) */
+#define MRM_U    0
+#define MRM_U0   1
+#define MRM_RES  2
+
+/* scalar cell fields (grid.H:58-68) */
+#define MRM_PRES 0
+#define MRM_TEMP 1
+#define MRM_NE   2
+#define MRM_RHOI 3
+#define MRM_AMB  4
+#define MRM_QTOT 5
+#define MRM_TAU  6
+#define MRM_NFIELD 7
+
+/* host array layouts for mrm_set_state / mrm_get_state */
+#define MRM_LAYOUT_AOS 0   /* cState[bufsize]: 8 doubles per cell (mhd3d.H:24-27) */
+#define MRM_LAYOUT_SOA 1   /* double[8][bufsize] */
+
+/* Geometry: GridData fields set from parameters.dat (init.C:66-93). */
+typedef struct mrm_grid_desc {
+  int    gsize[3];    /* interior cells per dimension; dim 0 is vertical */
+  int    ghosts[3];   /* ghost layers per side (2 in every config) */
+  int    periods[3];  /* 1 = periodic; the bottom/top BC acts on dim 0 */
+  double gxmin[3];
+  double gxmax[3];
+} mrm_grid_desc;
+
+/* PhysicsData (physics.H:80-206), the members the hot path reads, with the
+ * reference's index constants (i_param_*, i_bnd_*, i_tvd_*, i_tchk_*, ...). */
+typedef struct mrm_phys_desc {
+  double params[9];
+  double bnd[5];
+  double tvd_h[4];
+  double tvd_cs[4];
+  double tvd[10];
+  double tvd_h_bnd[2];
+  double tvd_visc_bnd[2];
+  double tvd_eta_bnd[2];
+  double tvd_coeff[4];
+  double tchk[4];
+  double dmp[4];
+  double rt[10];
+  int    rt_ext_cor;
+  int    pad_;
+} mrm_phys_desc;
+
+/* RunData (run.H:6-80), the members the hot path reads or updates. */
+typedef struct mrm_run_desc {
+  double dt;
+  double time;
+  double CFL;
+  double CFL_tvd;
+  double outcad;
+  int    autostep;
+  int    need_diagnostics;   /* must be 0: diagnostics are outside the path */
+  int    globiter;
+  int    iteration;
+} mrm_run_desc;
+
+typedef struct mrm_ctx mrm_ctx;
+
+/* Create a context on the current HIP device (GridData::Init, grid.C:147-348).
+ * precision: MRM_PREC_FP64 or MRM_PREC_FP32_MIXED. */
+int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, int precision,
+               mrm_ctx** out);
+int mrm_destroy(mrm_ctx* ctx);
+const char* mrm_last_error(const mrm_ctx* ctx);
+int mrm_bufsize(const mrm_ctx* ctx, long* bufsize, int dims[3]);
+int mrm_synchronize(mrm_ctx* ctx);
+
+/* eos_init (eos.H:38, eos_init.C:57-212): bytes of a table file in the
+ * reference's binary format (14-float header, 6 forward, 2 inverse tables). */
+int mrm_eos_load(mrm_ctx* ctx, const void* table_bytes, size_t nbytes);
+
+/* ACCH::UpdateGPU / UpdateCPU counterparts for U, U0, Res (ACCH.h:49-50). */
+int mrm_set_state(mrm_ctx* ctx, int which, const double* host, int layout);
+int mrm_get_state(mrm_ctx* ctx, int which, double* host, int layout);
+/* ... and for pres, temp, ne, rhoi, amb, Qtot, Tau (one double per cell). */
+int mrm_set_field(mrm_ctx* ctx, int field, const double* host);
+int mrm_get_field(mrm_ctx* ctx, int field, double* host);
+
+/* Globals the reference keeps outside GridData: inv_va2max, v_lim, e_lim,
+ * va_max_old (Adjust_Valf_Max.C:15,39) and p_bc, s_bc
+ * (boundary_pdmp_1_fftw3.C:32). lim[4] = {inv_va2max, v_lim, e_lim, va_max_old}. */
+int mrm_get_limits(const mrm_ctx* ctx, double lim[4]);
+int mrm_set_limits(mrm_ctx* ctx, const double lim[4]);
+int mrm_get_bc(const mrm_ctx* ctx, double* p_bc, double* s_bc);
+int mrm_set_bc(mrm_ctx* ctx, double p_bc, double s_bc);
+
+/* ConsToPrim (eos.H:39, cons_to_prim.C:39-236): U -> primitive in place over
+ * the whole buffer, pres/temp/ne/rhoi/amb filled. */
+int mrm_cons_to_prim(mrm_ctx* ctx);
+
+/* Adjust_Valf_Max (limit_va.H:7, Adjust_Valf_Max.C:25-150).  If stats is not
+ * NULL it receives {max|v|, max c_s, max eps, n(|v|>=.95 v_lim), n(eps>=.95 e_lim)}. */
+int mrm_adjust_valf_max(mrm_ctx* ctx, const mrm_run_desc* run, double stats[5]);
+
+/* MHD_Residual (mhd_tvd.H:5, mhdres_SR.C:27-854): Res += -div F + Lorentz
+ * force over the interior; returns dt_cfl = dxmin / cmax. */
+int mrm_mhd_residual(mrm_ctx* ctx, const mrm_run_desc* run, double* dt_cfl);
+
+/* Sync_timestep (solver.C:487-548), single rank: updates run->dt if autostep,
+ * writes cfl; MRM_ERR_INVALID_DT / MRM_ERR_CFL on the reference's aborts. */
+int mrm_sync_timestep(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, double dt_mhd,
+                      double* cfl);
+
+/* Source_Integrate_Tcheck (src_int_tck.H:8, Add_Sources_Integrate_SR.C:38-472). */
+int mrm_source_integrate_tcheck(mrm_ctx* ctx, const mrm_run_desc* run, int stage);
+/* SaveCons (src_int_tck.H:10, Add_Sources_Integrate_SR.C:475-531). */
+int mrm_save_cons(mrm_ctx* ctx);
+/* TCheck (src_int_tck.H:7, Add_Sources_Integrate_SR.C:534-625). */
+int mrm_tcheck(mrm_ctx* ctx, const mrm_run_desc* run);
+/* TVDlimit (mhd_tvd.H:6, tvdlimit_SR.C:33-695). */
+int mrm_tvdlimit(mrm_ctx* ctx, const mrm_run_desc* run, double dt_tvd);
+/* exchange_grid_acc (exchange.H:8, exchange.C:532-782) on one rank:
+ * periodic ghost layers of U in dims 1 and 2, filled on the device. */
+int mrm_exchange_grid(mrm_ctx* ctx);
+/* SetBoundaryConditions (icbc.H:10-12, boundary_pdmp_1_fftw3.C:81-493) with
+ * bnd_pot = 0, plus set_p_s_bc (:496-565) on the first call. */
+int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage);
+
+/* nsteps iterations of the ComputeSolution loop body (solver.C:109-365) with
+ * RT, divB cleaning and IO off: SaveCons, 4 x (ConsToPrim, Adjust_Valf_Max at
+ * stage 1, MHD_Residual, Sync_timestep at stage 1, Source_Integrate_Tcheck,
+ * exchange, BC), TVDlimit (split in two halves when cfl > CFL_tvd), TCheck,
+ * exchange, BC, Run.Advance.  dt_rad is the radiative time step the caller
+ * supplies (RT is outside the path). */
+int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MURAM_STEP_H */

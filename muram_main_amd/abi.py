@@ -1,0 +1,104 @@
+"""ctypes mirror of include/muram_step.h and the loader of the HIP library.
+
+The library is the product: ``muram_main_amd/libmuram_mi355x.so`` built from
+``muram_main_amd/csrc`` for gfx950.  ``load()`` fails loudly when it is absent;
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmuram_mi355x.so")
+
+MRM_OK = 0
+ERRORS = {-1: "MRM_ERR_ARG", -2: "MRM_ERR_HIP", -3: "MRM_ERR_EOS", -4: "MRM_ERR_INVALID_DT",
+          -5: "MRM_ERR_CFL", -6: "MRM_ERR_STATE", -7: "MRM_ERR_UNSUPPORTED"}
+PREC_FP64, PREC_FP32_MIXED = 0, 1
+U, U0, RES = 0, 1, 2
+PRES, TEMP, NE, RHOI, AMB, QTOT, TAU = range(7)
+LAYOUT_AOS, LAYOUT_SOA = 0, 1
+
+# every symbol include/muram_step.h declares
+EXPORTS = [
+    "mrm_create", "mrm_destroy", "mrm_last_error", "mrm_bufsize", "mrm_synchronize",
+    "mrm_eos_load", "mrm_set_state", "mrm_get_state", "mrm_set_field", "mrm_get_field",
+    "mrm_get_limits", "mrm_set_limits", "mrm_get_bc", "mrm_set_bc",
+    "mrm_cons_to_prim", "mrm_adjust_valf_max", "mrm_mhd_residual", "mrm_sync_timestep",
+    "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit",
+    "mrm_exchange_grid", "mrm_set_boundary_conditions", "mrm_step",
+]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("gsize", C.c_int * 3), ("ghosts", C.c_int * 3), ("periods", C.c_int * 3),
+                ("gxmin", C.c_double * 3), ("gxmax", C.c_double * 3)]
+
+
+class PhysDesc(C.Structure):
+    _fields_ = [("params", C.c_double * 9), ("bnd", C.c_double * 5), ("tvd_h", C.c_double * 4),
+                ("tvd_cs", C.c_double * 4), ("tvd", C.c_double * 10),
+                ("tvd_h_bnd", C.c_double * 2), ("tvd_visc_bnd", C.c_double * 2),
+                ("tvd_eta_bnd", C.c_double * 2), ("tvd_coeff", C.c_double * 4),
+                ("tchk", C.c_double * 4), ("dmp", C.c_double * 4), ("rt", C.c_double * 10),
+                ("rt_ext_cor", C.c_int), ("pad_", C.c_int)]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [("dt", C.c_double), ("time", C.c_double), ("CFL", C.c_double),
+                ("CFL_tvd", C.c_double), ("outcad", C.c_double), ("autostep", C.c_int),
+                ("need_diagnostics", C.c_int), ("globiter", C.c_int), ("iteration", C.c_int)]
+
+
+class MrmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the gfx950 library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MrmError(f"HIP library {path} is missing: run __graft_entry__.build() first "
+                       "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "mrm_create": ([C.POINTER(GridDesc), C.POINTER(PhysDesc), C.c_int, C.POINTER(P)], C.c_int),
+        "mrm_destroy": ([P], C.c_int),
+        "mrm_last_error": ([P], C.c_char_p),
+        "mrm_bufsize": ([P, C.POINTER(C.c_long), C.POINTER(C.c_int)], C.c_int),
+        "mrm_synchronize": ([P], C.c_int),
+        "mrm_eos_load": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "mrm_set_state": ([P, C.c_int, dp, C.c_int], C.c_int),
+        "mrm_get_state": ([P, C.c_int, dp, C.c_int], C.c_int),
+        "mrm_set_field": ([P, C.c_int, dp], C.c_int),
+        "mrm_get_field": ([P, C.c_int, dp], C.c_int),
+        "mrm_get_limits": ([P, dp], C.c_int),
+        "mrm_set_limits": ([P, dp], C.c_int),
+        "mrm_get_bc": ([P, dp, dp], C.c_int),
+        "mrm_set_bc": ([P, C.c_double, C.c_double], C.c_int),
+        "mrm_cons_to_prim": ([P], C.c_int),
+        "mrm_adjust_valf_max": ([P, C.POINTER(RunDesc), dp], C.c_int),
+        "mrm_mhd_residual": ([P, C.POINTER(RunDesc), dp], C.c_int),
+        "mrm_sync_timestep": ([P, C.POINTER(RunDesc), C.c_double, C.c_double, dp], C.c_int),
+        "mrm_source_integrate_tcheck": ([P, C.POINTER(RunDesc), C.c_int], C.c_int),
+        "mrm_save_cons": ([P], C.c_int),
+        "mrm_tcheck": ([P, C.POINTER(RunDesc)], C.c_int),
+        "mrm_tvdlimit": ([P, C.POINTER(RunDesc), C.c_double], C.c_int),
+        "mrm_exchange_grid": ([P], C.c_int),
+        "mrm_set_boundary_conditions": ([P, C.POINTER(RunDesc), C.c_int], C.c_int),
+        "mrm_step": ([P, C.POINTER(RunDesc), C.c_double, C.c_int], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib

@@ -1,0 +1,61 @@
+/*
+ * muram_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C11, FP64, OpenMP over grid lines) of MURaM's
+ * per-timestep MHD update, used as the parity checker of the HIP path and as
+ * the CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it; the product never does.
+ *
+ * State layout is structure-of-arrays: U[v][node], v in cState order
+ * (rho, M0, M1, M2, e, B0, B1, B2), node = i + n0*(j + n1*k), dim 0 vertical
+ * and fastest -- the reference's node order (grid.C:196-199) with the
+ * components split out of the AoS cState.
+ */
+#ifndef MURAM_ORACLE_H
+#define MURAM_ORACLE_H
+
+#include "../include/muram_step.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx* orc_create(const mrm_grid_desc* g, const mrm_phys_desc* p);
+void orc_destroy(orc_ctx* c);
+long orc_bufsize(const orc_ctx* c);
+int orc_eos_load(orc_ctx* c, const void* bytes, size_t nbytes);
+
+/* direct access to the arrays (SoA, 8*bufsize doubles for U/U0/Res) */
+double* orc_state(orc_ctx* c, int which);
+double* orc_field(orc_ctx* c, int field);
+void orc_get_limits(const orc_ctx* c, double lim[4]);
+void orc_set_limits(orc_ctx* c, const double lim[4]);
+void orc_get_bc(const orc_ctx* c, double* p_bc, double* s_bc);
+void orc_set_bc(orc_ctx* c, double p_bc, double s_bc);
+void orc_set_threads(int n);
+
+void   orc_cons_to_prim(orc_ctx* c);
+void   orc_adjust_valf_max(orc_ctx* c, const mrm_run_desc* run, double stats[5]);
+double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run);
+int    orc_sync_timestep(orc_ctx* c, mrm_run_desc* run, double dt_rad, double dt_mhd, double* cfl);
+void   orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage);
+void   orc_save_cons(orc_ctx* c);
+void   orc_tcheck(orc_ctx* c, const mrm_run_desc* run);
+void   orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd);
+void   orc_exchange_grid(orc_ctx* c);
+void   orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage);
+int    orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps);
+
+/* scalar EOS interpolations of eos_init.C:216-416 */
+double orc_p_interp(const orc_ctx* c, double ee, double dd);
+double orc_s_interp(const orc_ctx* c, double ee, double dd);
+double orc_T_interp(const orc_ctx* c, double ee, double dd);
+double orc_d3_interp(const orc_ctx* c, double pp, double ss);
+double orc_eps3_interp(const orc_ctx* c, double pp, double ss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
