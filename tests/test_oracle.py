@@ -1,0 +1,280 @@
+"""CPU tests of the oracle (oracle/muram_oracle.c).
+
+1. Golden vectors: the compiled reference's own outputs (tests/golden/ref_small.npz,
+   written by tests/golden/make_golden.py) must be reproduced to the last bit
+   on this image's libm (asserted at <= 1e-14 relative per field).
+2. Live reference: when oracle/_ref is built (the container with the
+   reference tree), a larger box over several steps, bit for bit.
+3. Invariants that pin parts of the path independently of the reference:
+   zero residual of a uniform state, the RK stage weights reproducing the
+   4th-order Taylor polynomial for a linear residual, the discrete divergence
+   of the induction residual, TVD mass conservation on periodic lines, and the
+   bilinear EOS lookup being exact on bilinear data.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from muram_main_amd import config, eos_table, initial
+from orc import Oracle, ref_available, run_reference, relerr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+TOL = 1e-14
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(os.path.join(GOLD, "ref_small.npz"))
+
+
+@pytest.fixture(scope="module")
+def table():
+    return open(os.path.join(GOLD, "eos_small.dat"), "rb").read()
+
+
+def small_grid():
+    return config.grid_desc((10, 8, 6), (3.2e6,) * 3)
+
+
+def maxrel(a, b):
+    return max(relerr(a[v], b[v]) for v in range(a.shape[0]))
+
+
+def test_golden_stage_functions(gold, table):
+    g, p = small_grid(), config.stratified_phys()
+    o = Oracle(g, p, table)
+    r = config.run_desc(CFL=0.8)
+    o.set_state(gold["base_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    assert maxrel(o.state(0), gold["base_startup_U"]) <= TOL
+    pb, sb = o.bc()
+    assert pb == pytest.approx(gold["base_resid_sc"][4], rel=TOL)
+    assert sb == pytest.approx(gold["base_resid_sc"][5], rel=TOL)
+    o.save_cons()
+    o.cons_to_prim()
+    assert maxrel(o.state(0), gold["base_c2p_U"]) <= TOL
+    F = np.stack([o.field(f) for f in range(5)])
+    assert maxrel(F, gold["base_c2p_F"]) <= TOL
+    o.adjust_valf_max(r, None)
+    dt_mhd = o.mhd_residual(r)
+    assert dt_mhd == pytest.approx(gold["base_resid_sc"][2], rel=TOL)
+    assert maxrel(o.state(2), gold["base_resid_R"]) <= TOL
+    cfl = C.c_double()
+    assert o.sync_timestep(r, 1e30, dt_mhd, C.byref(cfl)) == 0
+    assert r.dt == pytest.approx(gold["base_srcint_sc"][0], rel=TOL)
+    o.source_integrate_tcheck(r, 1)
+    assert maxrel(o.state(0), gold["base_srcint_U"]) <= TOL
+    assert np.all(o.state(2) == 0.0)
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 1)
+    assert maxrel(o.state(0), gold["base_bc1_U"]) <= TOL
+    o.tvdlimit(r, r.dt)
+    assert maxrel(o.state(0), gold["base_tvd_U"]) <= TOL
+
+
+def test_golden_cons_to_prim_fallbacks(gold, table):
+    o = Oracle(small_grid(), config.stratified_phys(), table)
+    o.set_state(gold["edge_in"])
+    o.cons_to_prim()
+    assert maxrel(o.state(0), gold["edge_c2p_U"]) <= TOL
+    F = np.stack([o.field(f) for f in range(5)])
+    assert maxrel(F, gold["edge_c2p_F"]) <= TOL
+    # the crafted cells really take the fallback branches
+    tab = eos_table.load_tables(table)
+    T = gold["edge_c2p_F"][1]
+    c_temp = (1.65 - 1.0) * 0.62 / 8.314e7
+    eps = gold["edge_c2p_U"][4] / gold["edge_c2p_U"][0] + tab.eps_off
+    assert T[3, 4, 3] == pytest.approx(c_temp * eps[3, 4, 3], rel=1e-15)   # ln rho < xlr[0]
+    assert T[3, 4, 7] == pytest.approx(c_temp * eps[3, 4, 7], rel=1e-15)   # invalid -> ideal gas
+    assert gold["edge_c2p_F"][2][3, 4, 7] == 1.0                           # n_e floor
+
+
+@pytest.mark.parametrize("case", ["base", "split", "vadj"])
+def test_golden_full_steps(gold, table, case):
+    vadj = case == "vadj"
+    p = config.stratified_phys(va_max=3.0e7, va_adjust=1.0) if vadj else config.stratified_phys()
+    o = Oracle(small_grid(), p, table)
+    r = config.run_desc(CFL=float(gold[f"{case}_cfl"]))
+    o.set_state(gold[f"{case}_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    assert o.step(r, 1e30, 1) == 0
+    assert maxrel(o.state(0), gold[f"{case}_step1_U"]) <= TOL
+    assert r.dt == pytest.approx(gold[f"{case}_step1_sc"][0], rel=TOL)
+    assert o.step(r, 1e30, 4) == 0
+    assert maxrel(o.state(0), gold[f"{case}_step5_U"]) <= TOL
+    assert r.time == pytest.approx(gold[f"{case}_step5_sc"][1], rel=TOL)
+    lim = o.limits()
+    assert lim[0] == pytest.approx(gold[f"{case}_step5_sc"][6], rel=TOL)
+    assert lim[1] == pytest.approx(gold[f"{case}_step5_sc"][7], rel=TOL)
+
+
+@pytest.mark.skipif(not ref_available(), reason="compiled reference (oracle/_ref) not built here")
+def test_live_reference_bitwise(table):
+    tab = eos_table.load_tables(table)
+    box = initial.BoxSpec(gsize=(24, 20, 16), dx=(3.2e6,) * 3, seed=3, v_rms=2.0e5)
+    U = initial.make_state(tab, box)
+    g, p = config.grid_desc(box.gsize, box.dx), config.stratified_phys()
+    D = run_reference(g, p, config.run_desc(), table, U, [("STARTUP", 0, False), ("STEP", 3, True)])
+    o = Oracle(g, p, table)
+    r = config.run_desc()
+    o.set_state(U)
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.step(r, 1e30, 3)
+    assert np.array_equal(o.state(0), D[0]["U"])
+
+
+# ---------------------------------------------------------------- invariants
+def uniform_phys():
+    p = config.stratified_phys()
+    p.params[config.I_PARAM_GRAV] = 0.0
+    return p
+
+
+def test_uniform_state_has_zero_residual(table):
+    g = config.grid_desc((12, 10, 8), (3.2e6,) * 3)
+    o = Oracle(g, uniform_phys(), table)
+    U = np.empty((8,) + o.shape)
+    vals = [1e-7, 3e-2, -1e-2, 2e-2, 0.0, 30.0, 5.0, -7.0]
+    rho, v = vals[0], np.array(vals[1:4]) / 1e-7 * 1e-7
+    for i, x in enumerate(vals):
+        U[i] = x
+    U[1:4] = rho * np.array([1e4, -2e4, 5e3])[:, None, None, None]
+    U[4] = rho * 7e11 + 0.5 * np.sum(U[1:4] ** 2, axis=0) / rho
+    o.set_state(U)
+    o.cons_to_prim()
+    o.adjust_valf_max(config.run_desc(), None)
+    dt = o.mhd_residual(config.run_desc())
+    assert np.isfinite(dt) and dt > 0
+    assert np.all(o.state(2) == 0.0)
+
+
+def test_rk_stage_weights_give_taylor_polynomial(table):
+    """U_s = U0 + dt/(5-s) R(U_{s-1}) with R = lam*U gives sum_k z^k/k!, k<=4."""
+    g = config.grid_desc((8, 6, 6), (3.2e6,) * 3)
+    p = uniform_phys()
+    p.tchk[:] = [1e-300, 1e-300, 1e300, 1e300]
+    o = Oracle(g, p, table)
+    rng = np.random.default_rng(0)
+    U0 = np.zeros((8,) + o.shape)
+    U0[0] = 1.0 + rng.random(o.shape)
+    U0[4] = 10.0 + rng.random(o.shape)
+    o.set_state(U0)
+    o.save_cons()
+    r = config.run_desc()
+    r.dt = 0.3
+    lam = -1.7
+    o.set_limits([0.0, 0.0, 0.0, 0.0])
+    sl = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    for stage in range(1, 5):
+        Ucur = o.state(0).copy()
+        R = np.zeros_like(Ucur)
+        R[sl] = lam * Ucur[sl]
+        o.state(2)[...] = R
+        o.source_integrate_tcheck(r, stage)
+    z = lam * r.dt
+    taylor = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    got = o.state(0)[sl][[0, 4]]
+    want = taylor * U0[sl][[0, 4]]
+    assert np.max(np.abs(got - want) / np.abs(want)) < 1e-14
+
+
+def test_induction_residual_is_divergence_free(table):
+    tab = eos_table.load_tables(table)
+    box = initial.BoxSpec(gsize=(20, 16, 16), dx=(3.2e6,) * 3, seed=5)
+    U = initial.make_state(tab, box)
+    g = config.grid_desc(box.gsize, box.dx)
+    o = Oracle(g, config.stratified_phys(), table)
+    o.set_state(U)
+    r = config.run_desc()
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.cons_to_prim()
+    o.adjust_valf_max(r, None)
+    o.mhd_residual(r)
+    R = o.state(2)
+    dx = 3.2e6
+
+    def d4(a, axis):
+        w1, w2 = 8.0 / (12.0 * dx), -1.0 / (12.0 * dx)
+        return w1 * (np.roll(a, -1, axis) - np.roll(a, 1, axis)) + w2 * (np.roll(a, -2, axis) - np.roll(a, 2, axis))
+
+    div = d4(R[5], 2) + d4(R[6], 1) + d4(R[7], 0)
+    scale = np.abs(d4(R[5], 2)) + np.abs(d4(R[6], 1)) + np.abs(d4(R[7], 0))
+    inner = (slice(4, -4), slice(4, -4), slice(4, -4))
+    assert np.max(np.abs(div[inner])) <= 1e-12 * np.max(scale[inner])
+    # and the test is sensitive: a sign error in one induction flux breaks it
+    bad = d4(-R[5], 2) + d4(R[6], 1) + d4(R[7], 0)
+    assert np.max(np.abs(bad[inner])) > 1e-3 * np.max(scale[inner])
+
+
+def test_tvd_conserves_mass_on_periodic_lines(table):
+    g = config.grid_desc((12, 16, 16), (3.2e6,) * 3)
+    o = Oracle(g, config.stratified_phys(), table)
+    rng = np.random.default_rng(11)
+    n2, n1, n0 = o.shape
+    U = np.empty((8, n2, n1, n0))
+    rho = 1e-7 * np.exp(0.3 * rng.standard_normal((n2, n1, 1)))
+    U[0] = rho
+    vel = 1e5 * rng.standard_normal((3, n2, n1, 1))
+    U[1:4] = rho * vel
+    U[4] = rho * 7e11 * (1 + 0.2 * rng.random((n2, n1, 1))) + 0.5 * rho * np.sum(vel ** 2, axis=0)
+    U[5] = 50.0
+    U[6:8] = 5.0 * rng.standard_normal((2, n2, n1, 1))
+    # periodic in the horizontal dims everywhere, ghosts included (the BC makes
+    # the vertical ghost columns periodic images in the real flow)
+    core = U[:, 2:-2, 2:-2, :]
+    U = np.pad(core, ((0, 0), (2, 2), (2, 2), (0, 0)), mode="wrap")
+    U = np.ascontiguousarray(np.broadcast_to(U, (8, n2, n1, n0)))
+    o.set_state(U)
+    r = config.run_desc()
+    o.exchange_grid()
+    o.cons_to_prim()
+    o.adjust_valf_max(r, None)
+    dt = o.mhd_residual(r)
+    r.dt = 0.8 * dt
+    o.state(0)[...] = U
+    o.exchange_grid()
+    inner = (slice(2, -2), slice(2, -2), slice(2, -2))
+    m0 = np.sum(U[0][inner])
+    o.tvdlimit(r, r.dt)
+    m1 = np.sum(o.state(0)[0][inner])
+    changed = np.max(np.abs(o.state(0)[0][inner] - U[0][inner]) / U[0][inner])
+    assert changed > 1e-6            # the limiter did act
+    assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_eos_lookup_exact_on_bilinear_table():
+    spec = eos_table.TableSpec(n_eps=16, n_lr=12, n_lp3=8, n_s3=6)
+    raw = bytearray(eos_table.generate(spec))
+    tab = eos_table.load_tables(bytes(raw))
+    # replace ln p by a bilinear function of (ln(eps+eps_off), ln rho)
+    a, b, cc, dd = 0.3, 1.1, -0.7, 0.02
+    LE, LR = np.meshgrid(tab.xeps, tab.xlr)
+    lnp = (a + b * LE + cc * LR + dd * LE * LR).astype(np.float32)
+    buf = np.frombuffer(raw, dtype=np.float32).copy()
+    n = tab.n_eps * tab.n_lr
+    buf[14:14 + n] = lnp.ravel()
+    g = config.grid_desc((4, 4, 4), (1e6,) * 3)
+    o = Oracle(g, config.stratified_phys(), buf.tobytes())
+    tab2 = eos_table.load_tables(buf.tobytes())
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        le = rng.uniform(tab.xeps[0], tab.xeps[-1])
+        lr = rng.uniform(tab.xlr[0], tab.xlr[-1])
+        ee = np.exp(le) - tab.eps_off
+        got = np.log(o.p_interp(ee, np.exp(lr)))
+        # bilinear interpolation reproduces the bilinear function of the float32 node values
+        i = min(int((le - tab2.xeps[0]) / tab2.del_eps), tab2.n_eps - 2)
+        j = min(int((lr - tab2.xlr[0]) / tab2.del_lr), tab2.n_lr - 2)
+        f = lambda ii, jj: float(lnp[jj, ii])
+        te = (le - tab2.xeps[i]) / tab2.del_eps
+        tr = (lr - tab2.xlr[j]) / tab2.del_lr
+        want = ((1 - te) * (1 - tr) * f(i, j) + te * (1 - tr) * f(i + 1, j)
+                + (1 - te) * tr * f(i, j + 1) + te * tr * f(i + 1, j + 1))
+        assert got == pytest.approx(want, abs=1e-5)
