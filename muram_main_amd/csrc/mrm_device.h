@@ -1,0 +1,526 @@
+// mrm_device.h -- device-side building blocks of the MHD step for gfx950.
+//
+// Grid state is structure-of-arrays in HBM: field v of cell `node` at
+// U[v*bs + node], node = i + n0*(j + n1*k), dim 0 (vertical) fastest -- the
+// reference's node order (grid.C:196-199) with cState's components
+// (rho, M0, M1, M2, e, B0, B1, B2; mhd3d.H:24-27) split into separate arrays so
+// that every wavefront load is a contiguous 512-byte segment.
+//
+// Arithmetic keeps the reference's operation order; the library is built with
+// -ffp-contract=off so no a*b+c is fused.  smax/smin reproduce std::max /
+// std::min (NaN handling included); fmax/fmin are used where the reference
+// calls those.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MRM_NV 8
+
+struct Geom {
+  int n0, n1, n2;          // buffer extents (gsize + 2*ghosts)
+  int g0, g1, g2;          // ghosts
+  int lb0, lb1, lb2;       // lbeg
+  int le0, le1, le2;       // lend
+  long s1, s2;             // strides of dims 1, 2 (stride of dim 0 is 1)
+  long bs;                 // bufsize
+};
+
+struct EosDev {
+  int N_eps, N_lr, N_lp3, N_s3;
+  double inv_del_eps, inv_del_lr, inv_del_lp3, inv_del_s3;
+  double eps_off, ss_off, eps_max;
+  const double* xeps;
+  const double* xlr;
+  const double* xlp3;
+  const double* xs3;
+  const float* tab;    // [6][N_eps][N_lr]: ln p, ln T, s, ln ne, ln rhoi, ln amb (pre-divided)
+  const float* itab;   // [2][N_lp3][N_s3]: ln(eps+eps_off), ln rho (pre-divided)
+};
+
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ int imax_(int a, int b) { return (a < b) ? b : a; }
+__device__ __forceinline__ int imin_(int a, int b) { return (b < a) ? b : a; }
+
+// (int)x as the reference's x86-64 build evaluates it: cvttsd2si returns
+// INT_MIN for NaN and out-of-range values; the GPU conversion saturates.
+__device__ __forceinline__ int cvt_int(double x) {
+  return (x >= -2147483648.0 && x < 2147483648.0) ? (int)x : (int)0x80000000;
+}
+
+// ------------------------------------------------------------------ EOS
+// Bilinear lookups of eos_init.C:216-416 (scalar interpolations used by the
+// boundary condition); term order follows each reference function.
+__device__ __forceinline__ double eos_fwd_raw(const EosDev& E, int t, double ee, double dd) {
+  double ee1 = log(ee + E.eps_off);
+  double logr = log(dd);
+  int i = cvt_int((ee1 - E.xeps[0]) * E.inv_del_eps);
+  if (i < 0) i = 0;
+  if (i > E.N_eps - 2) i = E.N_eps - 2;
+  int j = cvt_int((logr - E.xlr[0]) * E.inv_del_lr);
+  if (j < 0) j = 0;
+  if (j > E.N_lr - 2) j = E.N_lr - 2;
+  const float* T = E.tab + (size_t)t * E.N_eps * E.N_lr;
+  const long L = E.N_lr;
+  double v = (logr - E.xlr[j]) * (ee1 - E.xeps[i]) * (double)T[(i + 1) * L + j + 1]
+           + (logr - E.xlr[j]) * (E.xeps[i + 1] - ee1) * (double)T[i * L + j + 1]
+           + (E.xlr[j + 1] - logr) * (ee1 - E.xeps[i]) * (double)T[(i + 1) * L + j]
+           + (E.xlr[j + 1] - logr) * (E.xeps[i + 1] - ee1) * (double)T[i * L + j];
+  return v;
+}
+__device__ __forceinline__ double p_interp(const EosDev& E, double ee, double dd) {
+  return exp(eos_fwd_raw(E, 0, ee, dd));
+}
+__device__ __forceinline__ double s_interp(const EosDev& E, double ee, double dd) {
+  return eos_fwd_raw(E, 2, ee, dd) - E.ss_off;
+}
+__device__ __forceinline__ double inv_raw(const EosDev& E, int t, double pp, double ss) {
+  double ss1 = ss + E.ss_off;
+  double logp = log(pp);
+  int i = cvt_int((logp - E.xlp3[0]) * E.inv_del_lp3);
+  if (i < 0) i = 0;
+  if (i > E.N_lp3 - 2) i = E.N_lp3 - 2;
+  int j = cvt_int((ss1 - E.xs3[0]) * E.inv_del_s3);
+  if (j < 0) j = 0;
+  if (j > E.N_s3 - 2) j = E.N_s3 - 2;
+  const float* T = E.itab + (size_t)t * E.N_lp3 * E.N_s3;
+  const long L = E.N_s3;
+  return (logp - E.xlp3[i]) * (ss1 - E.xs3[j]) * (double)T[(i + 1) * L + j + 1]
+       + (logp - E.xlp3[i]) * (E.xs3[j + 1] - ss1) * (double)T[(i + 1) * L + j]
+       + (E.xlp3[i + 1] - logp) * (ss1 - E.xs3[j]) * (double)T[i * L + j + 1]
+       + (E.xlp3[i + 1] - logp) * (E.xs3[j + 1] - ss1) * (double)T[i * L + j];
+}
+__device__ __forceinline__ double d3_interp(const EosDev& E, double pp, double ss) {
+  return exp(inv_raw(E, 1, pp, ss));
+}
+__device__ __forceinline__ double eps3_interp(const EosDev& E, double pp, double ss) {
+  double ee = exp(inv_raw(E, 0, pp, ss));
+  return ee - E.eps_off;
+}
+
+// ConsToPrim on one cell (cons_to_prim.C:108-204).  In: rho, M, e.
+// Out: v (3), e_int, and p/T/ne/rhoi/amb in out[0..4].  With NOUT = 2 only
+// p and T are produced (T decides the validity fallback of p); NOUT = 5
+// produces all five.
+template <int NOUT>
+__device__ __forceinline__ void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
+                                                  double& m3, double& e, double out[5]) {
+  const double eos_gamma = 1.65, eos_mu = 0.62, eos_mu_n = 1.3;
+  const double mh = 1.67262158e-24, kb = 1.380622e-16;
+  const double Rgas = 8.314e7;
+  const double c_temp = (eos_gamma - 1.) * eos_mu / Rgas;
+  const double c_pres = (eos_gamma - 1.);
+  double dn = 1.0 / d;
+  m1 = m1 * dn;
+  m2 = m2 * dn;
+  m3 = m3 * dn;
+  double vv = (m1 * m1 + m2 * m2 + m3 * m3);
+  e = e - 0.5 * vv * d;
+  double eps = e * dn + E.eps_off;
+  double lgr = log(d);
+  double ep = log(eps);
+  double lr = lgr;
+  int i_d = cvt_int((lr - E.xlr[0]) * E.inv_del_lr);
+  i_d = imax_(0, i_d);
+  i_d = imin_(E.N_lr - 2, i_d);
+  int i_e = cvt_int((ep - E.xeps[0]) * E.inv_del_eps);
+  i_e = imax_(0, i_e);
+  i_e = imin_(E.N_eps - 2, i_e);
+  double coeff0 = (ep - E.xeps[i_e]) * (lr - E.xlr[i_d]);
+  double coeff1 = (ep - E.xeps[i_e]) * (E.xlr[i_d + 1] - lr);
+  double coeff2 = (E.xeps[i_e + 1] - ep) * (lr - E.xlr[i_d]);
+  double coeff3 = (E.xeps[i_e + 1] - ep) * (E.xlr[i_d + 1] - lr);
+  const long L = E.N_lr;
+  const long NT = (long)E.N_eps * E.N_lr;
+  const long o11 = (i_e + 1) * L + i_d + 1, o10 = (i_e + 1) * L + i_d;
+  const long o01 = i_e * L + i_d + 1, o00 = i_e * L + i_d;
+  const int tabs[5] = {0, 1, 3, 4, 5};
+  double r[5];
+#pragma unroll
+  for (int q = 0; q < NOUT; q++) {
+    const float* T = E.tab + tabs[q] * NT;
+    double v = coeff0 * (double)T[o11];
+    v += coeff1 * (double)T[o10];
+    v += coeff2 * (double)T[o01];
+    v += coeff3 * (double)T[o00];
+    r[q] = exp(v);
+  }
+  double pres = r[0];
+  double temp = r[1];
+  double nel = NOUT > 2 ? r[2] : 0.0;
+  if (lgr < E.xlr[0]) {
+    pres = c_pres * e;
+    temp = c_temp * eps;
+    nel = pres / (temp * kb) - d / (eos_mu_n * mh);
+  }
+  if (eps > 0.8 * E.eps_max) {
+    double xx = smax(0.0, 5.0 * (E.eps_max - eps) / E.eps_max);
+    pres = pres * xx + (1.0 - xx) * c_pres * e;
+    temp = temp * xx + (1.0 - xx) * c_temp * eps;
+    nel = pres / (temp * kb) - d / (eos_mu_n * mh);
+  }
+  int bad = 0;
+  if ((pres != pres) || (pres <= 0.) || (pres == __builtin_inf())) bad = 1;
+  if ((temp != temp) || (temp <= 0.) || (temp == __builtin_inf())) bad = 1;
+  if (bad) {
+    pres = c_pres * e;
+    temp = c_temp * eps;
+  }
+  if (nel == __builtin_inf()) nel = pres / (temp * kb) - d / (eos_mu_n * mh);
+  if ((nel < 1.0) || (nel != nel) || (nel == __builtin_inf())) nel = 1.0;
+  out[0] = pres;
+  out[1] = temp;
+  if (NOUT > 2) {
+    out[2] = nel;
+    out[3] = r[3];
+    out[4] = r[4];
+  }
+}
+
+// ------------------------------------------------------------------ residual
+// Directional flux of one point (mhdres_SR.C:296-360), eta = Spitzer =
+// ambipolar = 0.  D is the sweep direction d1; the del[d1][*] factors of the
+// reference are kept as explicit 0/1 multipliers so NaN/inf propagate alike.
+struct PtFlux {
+  double f[MRM_NV];
+  double ms[3];
+};
+
+template <int D>
+__device__ __forceinline__ void point_flux(const double var[MRM_NV], double pres, PtFlux& o) {
+  const double p1 = D == 0 ? 1.0 : 0.0, p2 = D == 1 ? 1.0 : 0.0, p3 = D == 2 ? 1.0 : 0.0;
+  double var0 = var[0], var1 = var[1], var2 = var[2], var3 = var[3];
+  double var4 = var[4], var5 = var[5], var6 = var[6], var7 = var[7];
+  double vel = var[1 + D];
+  double mag = var[5 + D];
+  double vv = var1 * var1 + var2 * var2 + var3 * var3;
+  double bb = var5 * var5 + var6 * var6 + var7 * var7;
+  double ekin = 0.5 * var0 * vv;
+  double pmag = 0.5 * bb;
+  double mflx = vel * var0;
+  o.f[0] = mflx;
+  o.f[1] = mflx * var1 + p1 * pres;
+  o.f[2] = mflx * var2 + p2 * pres;
+  o.f[3] = mflx * var3 + p3 * pres;
+  o.f[4] = vel * (var4 + ekin + pres);
+  o.f[5] = vel * var5 - var1 * mag;
+  o.f[6] = vel * var6 - var2 * mag;
+  o.f[7] = vel * var7 - var3 * mag;
+  o.ms[0] = mag * var5 - p1 * pmag;
+  o.ms[1] = mag * var6 - p2 * pmag;
+  o.ms[2] = mag * var7 - p3 * pmag;
+}
+
+// lf = (1+x^2)/(1+x^2+x^4), x^2 = va^2 / va_max^2 (mhdres_SR.C:319-323)
+__device__ __forceinline__ double lfac_of(const double var[MRM_NV], double inv_va2max) {
+  double dn = 1.0 / var[0];
+  double bb = var[5] * var[5] + var[6] * var[6] + var[7] * var[7];
+  double va2 = bb * dn;
+  double x2 = va2 * inv_va2max;
+  double x4 = x2 * x2;
+  double s = 1.0 + x2;
+  return s / (s + x4);
+}
+
+// fastest characteristic speed of one point (mhdres_SR.C:310-328)
+__device__ __forceinline__ double point_cmax(const double var[MRM_NV], double pres, double inv_va2max) {
+  double var0 = var[0];
+  double dn = 1.0 / var0;
+  double vv = var[1] * var[1] + var[2] * var[2] + var[3] * var[3];
+  double bb = var[5] * var[5] + var[6] * var[6] + var[7] * var[7];
+  double va2 = bb * dn;
+  double x2 = va2 * inv_va2max;
+  double x4 = x2 * x2;
+  double s = 1.0 + x2;
+  double lf = s / (s + x4);
+  double cs2 = 5.0 / 3.0 * pres * dn;
+  vv = sqrt(vv);
+  return vv + 0.0 + sqrt(smax(cs2, lf * (cs2 + va2)));
+}
+
+// Residual contribution of sweep D at the centre point given the point data of
+// the five points c-2..c+2 along D (mhdres_SR.C:361-412).  res must hold 0.
+template <int D>
+__device__ __forceinline__ void sweep_residual(const double v[5][MRM_NV], const double p[5], double w1,
+                                               double w2, double inv_va2max, double res[MRM_NV]) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  PtFlux fm2, fm1, fp1, fp2;
+  point_flux<D>(v[0], p[0], fm2);
+  point_flux<D>(v[1], p[1], fm1);
+  point_flux<D>(v[3], p[3], fp1);
+  point_flux<D>(v[4], p[4], fp2);
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    res[q] = 0.0;
+    res[q] -= w1 * (fp1.f[q] - fm1.f[q]) + w2 * (fp2.f[q] - fm2.f[q]);
+  }
+  double dB2 = w1 * (v[3][5 + D2] - v[1][5 + D2]) + w2 * (v[4][5 + D2] - v[0][5 + D2]);
+  double dB3 = w1 * (v[3][5 + D3] - v[1][5 + D3]) + w2 * (v[4][5 + D3] - v[0][5 + D3]);
+  double cb[3];
+  cb[D] = -v[2][5 + D2] * dB2 - v[2][5 + D3] * dB3;
+  cb[D2] = v[2][5 + D] * dB2;
+  cb[D3] = v[2][5 + D] * dB3;
+  double lrt0 = w1 * (fp1.ms[0] - fm1.ms[0]) + w2 * (fp2.ms[0] - fm2.ms[0]);
+  double lrt1 = w1 * (fp1.ms[1] - fm1.ms[1]) + w2 * (fp2.ms[1] - fm2.ms[1]);
+  double lrt2 = w1 * (fp1.ms[2] - fm1.ms[2]) + w2 * (fp2.ms[2] - fm2.ms[2]);
+  double lf = lfac_of(v[2], inv_va2max);
+  lrt0 = lf * lrt0 + (1.0 - lf) * cb[0];
+  lrt1 = lf * lrt1 + (1.0 - lf) * cb[1];
+  lrt2 = lf * lrt2 + (1.0 - lf) * cb[2];
+  res[1] += lrt0;
+  res[2] += lrt1;
+  res[3] += lrt2;
+  res[4] += v[2][1] * lrt0 + v[2][2] * lrt1 + v[2][3] * lrt2;
+}
+
+// ------------------------------------------------------------------ sources
+struct SrcParams {
+  double wdt, invRdt, g, inv_va2max;
+  double rho_min, eps_min, eps_max, vmax;
+};
+
+// Tcheck clamp (Add_Sources_Integrate_SR.C:447-465)
+__device__ __forceinline__ void tcheck_vals(double u[MRM_NV], double rho_min, double eps_min, double eps_max,
+                                            double vmax) {
+  double dn = fmax(rho_min, u[0]);
+  double vv = sqrt(u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / dn;
+  double s = vmax / fmax(vmax, vv);
+  double ekin = 0.5 * dn * vv * vv;
+  double ee = u[4];
+  double ee_low = eps_min * dn + ekin * s * s;
+  double ee_up = eps_max * dn + ekin * s * s;
+  u[4] = fmin(fmax(ee, ee_low), ee_up);
+  u[0] = dn;
+  u[1] = u[1] * s;
+  u[2] = u[2] * s;
+  u[3] = u[3] * s;
+}
+
+// Sources, Boris correction, RK update and Tcheck of one cell
+// (Add_Sources_Integrate_SR.C:331-465).  up: primitive U at the cell; R: its
+// residual (modified); u0: U0.  Returns the new conservative state in out.
+__device__ __forceinline__ void source_integrate_cell(const SrcParams& P, const double up[MRM_NV],
+                                                      double R[MRM_NV], const double u0[MRM_NV], double Qrad,
+                                                      double out[MRM_NV]) {
+  double dn = P.invRdt * up[4];
+  double efaci = dn / fmax(dn, fabs(Qrad));
+  dn = P.g * up[0] - 0.0;
+  double e_dmpi = dn * up[1];
+  R[1] += dn;
+  R[4] += e_dmpi + Qrad * efaci;
+  double rh = up[0], rd = R[0];
+  double vx = up[1], vy = up[2], vz = up[3];
+  double bx = up[5], by = up[6], bz = up[7];
+  double fsr_x = R[1], fsr_y = R[2], fsr_z = R[3];
+  double bb = bx * bx + by * by + bz * bz;
+  double va2 = bb / rh;
+  double x2 = va2 * P.inv_va2max;
+  double x4 = x2 * x2;
+  double bf = x4 / (1 + x2 + x4);
+  fsr_x -= vx * rd;
+  fsr_y -= vy * rd;
+  fsr_z -= vz * rd;
+  dn = (fsr_x * bx + fsr_y * by + fsr_z * bz) / fmax(1e-100, bb);
+  double boris0 = -bf * (fsr_x - dn * bx);
+  double boris1 = -bf * (fsr_y - dn * by);
+  double boris2 = -bf * (fsr_z - dn * bz);
+  double boris3 = boris0 * vx + boris1 * vy + boris2 * vz;
+  R[1] += boris0;
+  R[2] += boris1;
+  R[3] += boris2;
+  R[4] += boris3;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) out[q] = u0[q] + P.wdt * R[q];
+  tcheck_vals(out, P.rho_min, P.eps_min, P.eps_max, P.vmax);
+}
+
+// ------------------------------------------------------------------ TVD
+// TVDlimit (tvdlimit_SR.C:33-695), diagnostics and ambipolar off.  One
+// thread streams one grid line with a four-point window: at step t it
+// evaluates point t, the slope at t-1, the interface flux at t-2 and updates
+// point t-2 in place -- the same per-point, per-slope and per-interface
+// arithmetic as the reference's phase loops over whole lines.
+struct TvdParams {
+  double rho_min, eps_min, rho_lev, q_rho_max;
+  double visc_coeff_bot, visc_slope_bot, visc_coeff_top, visc_slope_top;
+  double eta_coeff_bot, eta_slope_bot, eta_coeff_top, eta_slope_top;
+  double B_par_diff, vhyp, tvd_pm_v, tvd_pm_B, vmax_lim, CME_thresh;
+  double tvd_h[4], tvd_cs[4], tc[4];
+  double inv_va2max, vsqr_diff, cmax, idxd1, dt_tvd;
+  int need_tvd_coeff, rho_log, is_gbeg0, is_gend0;
+  const double* hfb;
+  const double* hft;
+  const double* qft;
+  const double* tau;   // NULL: Tau == 0 everywhere
+};
+
+struct TvdPt {
+  double v[MRM_NV];   // primitives; ln rho and ln eps when rho_log
+  double lnr;         // ln rho (density-jump detector)
+  double vsqr, bsqr, cm, hyp_e, hyp_v, hyp_B, BC2, CZ, hd;
+};
+
+// per-point quantities (tvdlimit_SR.C:269-385)
+__device__ __forceinline__ void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, double tau,
+                                          double hfb1, double hft1, bool xsweep, TvdPt& o) {
+  double v0 = smax(P.rho_min, u[0]);
+  double v1 = u[1], v2 = u[2], v3 = u[3], v4 = u[4];
+  o.v[5] = u[5]; o.v[6] = u[6]; o.v[7] = u[7];
+  o.CZ = (double)(tau > 1.0e-5);
+  double dn = 1.0 / v0;
+  v1 = v1 * dn;
+  v2 = v2 * dn;
+  v3 = v3 * dn;
+  double vv = (v1 * v1 + v2 * v2 + v3 * v3);
+  double bb = (o.v[5] * o.v[5] + o.v[6] * o.v[6] + o.v[7] * o.v[7]);
+  v4 = v4 * dn - 0.5 * vv;
+  o.vsqr = vv;
+  o.bsqr = bb;
+  vv = sqrt(vv);
+  double va2 = bb * dn;
+  double x2 = va2 * P.inv_va2max;
+  double x4 = x2 * x2;
+  double s = 1.0 + x2;
+  double lf = s / (s + x4);
+  double cs2 = 5.0 / 3.0 * pres * dn;
+  double cfast = sqrt(va2 + cs2);
+  double CME_mode = (1.0 - o.CZ) * ((double)(vv >= P.CME_thresh * cfast));
+  dn = v0 * v0;
+  double rf = smin(lf, dn / (dn + P.rho_lev));
+  double hh = 1.0 - hfb1 - hft1;
+  double cf = hfb1 * P.tvd_cs[0] + (rf * P.tvd_cs[1] + (1.0 - rf) * P.tvd_cs[2]) * hh + hft1 * P.tvd_cs[3];
+  cf = smax(cf, CME_mode);
+  cs2 *= cf * cf;
+  double vv_amb = 0.0;
+  o.cm = vv_amb + vv + sqrt(smax(cs2, lf * (cs2 + va2)));
+  o.hyp_e = hfb1 * P.tvd_h[0] + (rf * P.tvd_h[1] + (1.0 - rf) * P.tvd_h[2]) * hh + hft1 * P.tvd_h[3];
+  o.hyp_v = hfb1 * P.tvd_h[0] + (rf * P.tvd_h[1] + (1.0 - rf) * P.tvd_h[2] * P.tvd_pm_v) * hh +
+            hft1 * P.tvd_h[3] * P.tvd_pm_v;
+  o.hyp_B = hfb1 * P.tvd_h[0] + (rf * P.tvd_h[1] + (1.0 - rf) * P.tvd_h[2] * P.tvd_pm_B) * hh +
+            hft1 * P.tvd_h[3] * P.tvd_pm_B;
+  o.hyp_e *= (1.0 - CME_mode);
+  o.hyp_v *= (1.0 - CME_mode);
+  o.BC2 = 1.0 - lf;
+  o.hd = 0.0;
+  if (xsweep && (P.vhyp > 0.0)) {
+    double h = P.vhyp * sqrt(o.vsqr);
+    o.hd = 0.5 * P.idxd1 * smax(0.0, smin(h, P.cmax - o.cm));
+  }
+  if (P.rho_log) {
+    v0 = log(v0);
+    v4 = log(v4);
+    o.lnr = v0;
+  } else {
+    o.lnr = log(v0);
+  }
+  o.v[0] = v0; o.v[1] = v1; o.v[2] = v2; o.v[3] = v3; o.v[4] = v4;
+}
+
+// limited slope at point i (tvdlimit_SR.C:388-434)
+__device__ __forceinline__ void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
+                                          double hfb1, double hft1, double S[MRM_NV]) {
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    double sl = b.v[q] - a.v[q];
+    double sr = c.v[q] - b.v[q];
+    double slm = 0.25 * (sl + sr);
+    double lower = smin(smin(sl, sr), slm);
+    double upper = smax(smax(sl, sr), slm);
+    S[q] = smax(0.0, lower) + smin(0.0, upper);
+  }
+  if ((P.visc_slope_bot > 0.0) || (P.visc_slope_top > 0.0)) {
+    double cf = (1.0 - P.visc_slope_bot * hfb1) * (1.0 - P.visc_slope_top * hft1);
+    S[1] *= cf; S[2] *= cf; S[3] *= cf;
+  }
+  if ((P.eta_slope_bot > 0.0) || (P.eta_slope_top > 0.0)) {
+    double cf = (1.0 - P.eta_slope_bot * hfb1) * (1.0 - P.eta_slope_top * hft1);
+    S[5] *= cf; S[6] *= cf; S[7] *= cf;
+  }
+  double qrho = fabs(b.lnr - c.lnr);
+  if (qrho > P.q_rho_max) S[0] = 0.0;
+  if (P.vmax_lim > 0.0) {
+    if (b.vsqr > P.vsqr_diff) { S[1] = 0.0; S[2] = 0.0; S[3] = 0.0; }
+  }
+}
+
+// diffusive flux at interface i (between points i = b and i+1 = c), with
+// a = i-1 and d = i+2 (tvdlimit_SR.C:436-563); returns the resistive-heating
+// interface term (:609-614)
+template <int D>
+__device__ __forceinline__ double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
+                                            const TvdPt& d, const double Sb[MRM_NV], const double Sc[MRM_NV],
+                                            double hfb1, double hft1, double qft1, bool hyper_on,
+                                            bool zero_bflux, double F[MRM_NV]) {
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    double sl = c.v[q] - b.v[q];
+    double sl_lim = c.v[q] - b.v[q] - (Sb[q] + Sc[q]);
+    double dn = (q >= 1 && q <= 3) ? b.hyp_v : (q >= 5 ? b.hyp_B : b.hyp_e);
+    double rf = fabs(sl_lim) / smax(1e-100, fabs(sl));
+    double cf = smax(0.0, smin(sl_lim, sl)) + smin(0.0, smax(sl_lim, sl));
+    F[q] = smax(0.0, 1.0 + dn * (rf - 1.0)) * cf;
+  }
+  double tvd_faci = 0.5 * P.idxd1 * smin(P.cmax, smax(b.cm, c.cm));
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) F[q] *= tvd_faci;
+  F[5 + D] *= P.B_par_diff;
+  if (P.need_tvd_coeff) {
+    F[0] *= P.tc[0];
+    double cf = smax(P.tc[1], P.visc_coeff_bot * hfb1 + P.visc_coeff_top * hft1);
+    F[1] *= cf; F[2] *= cf; F[3] *= cf;
+    F[4] *= P.tc[2];
+    cf = smax(P.tc[3], P.eta_coeff_bot * hfb1 + P.eta_coeff_top * hft1);
+    F[5] *= cf; F[6] *= cf; F[7] *= cf;
+  }
+  if (D == 0 && hyper_on) {
+    double dn = smax(b.hd, c.hd);
+    F[0] -= dn * (0.125 * (d.v[0] - a.v[0]) - 0.375 * (c.v[0] - b.v[0]));
+    F[1] -= dn * (0.125 * (d.v[1] - a.v[1]) - 0.375 * (c.v[1] - b.v[1]));
+    F[4] -= dn * (0.125 * (d.v[4] - a.v[4]) - 0.375 * (c.v[4] - b.v[4]));
+  }
+  double flx0 = F[0], flx1 = F[1], flx2 = F[2], flx3 = F[3], flx4 = F[4];
+  double uif0 = 0.5 * (b.v[0] + Sb[0] + c.v[0] - Sc[0]);
+  double uif1 = 0.5 * (b.v[1] + Sb[1] + c.v[1] - Sc[1]);
+  double uif2 = 0.5 * (b.v[2] + Sb[2] + c.v[2] - Sc[2]);
+  double uif3 = 0.5 * (b.v[3] + Sb[3] + c.v[3] - Sc[3]);
+  double uif4 = 0.5 * (b.v[4] + Sb[4] + c.v[4] - Sc[4]);
+  if (P.rho_log) {
+    uif0 = exp(uif0);
+    flx0 *= uif0;
+    uif4 = exp(uif4);
+    flx4 *= uif4;
+  }
+  double min_cz = flx0 * smin(b.CZ, c.CZ);
+  flx1 *= uif0; flx1 += uif1 * min_cz;
+  flx2 *= uif0; flx2 += uif2 * min_cz;
+  flx3 *= uif0; flx3 += uif3 * min_cz;
+  flx4 *= uif0;
+  flx4 += (uif4 * min_cz) + qft1 * (uif1 * flx1 + uif2 * flx2 + uif3 * flx3);
+  F[0] = flx0; F[1] = flx1; F[2] = flx2; F[3] = flx3; F[4] = flx4;
+  if (zero_bflux) F[5] = 0.0;
+  return (c.v[5] - b.v[5]) * F[5] + (c.v[6] - b.v[6]) * F[6] + (c.v[7] - b.v[7]) * F[7];
+}
+
+// in-place update of point i from the fluxes at i-1 and i (tvdlimit_SR.C:565-657)
+__device__ __forceinline__ void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_NV],
+                                           const double F[MRM_NV], double qm, double q, double qft1,
+                                           double u[MRM_NV]) {
+  double r[MRM_NV];
+#pragma unroll
+  for (int k = 0; k < MRM_NV; k++) r[k] = P.dt_tvd * (F[k] - Fm[k]);
+  double v1 = b.v[1], v2 = b.v[2], v3 = b.v[3], b5 = b.v[5], b6 = b.v[6], b7 = b.v[7];
+  r[4] += (1.0 - qft1) * (r[1] * v1 + r[2] * v2 + r[3] * v3);
+  double dn = (r[1] * b5 + r[2] * b6 + r[3] * b7) / smax(1e-100, b.bsqr);
+  double boris0 = -b.BC2 * (r[1] - dn * b5);
+  double boris1 = -b.BC2 * (r[2] - dn * b6);
+  double boris2 = -b.BC2 * (r[3] - dn * b7);
+  double boris3 = boris0 * v1 + boris1 * v2 + boris2 * v3;
+  r[1] += boris0; r[2] += boris1; r[3] += boris2; r[4] += boris3;
+  double qres = 0.5 * (q + qm) * qft1;
+  r[4] += P.dt_tvd * qres;
+#pragma unroll
+  for (int k = 0; k < MRM_NV; k++) u[k] += r[k];
+  u[0] = smax(P.rho_min, u[0]);
+  u[4] = smax(u[4], P.eps_min * u[0] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]);
+}

@@ -1,0 +1,494 @@
+// mrm_kernels.h -- HIP kernels of the MHD step (FP64 path), one per
+// reference operation.  Launch geometry is chosen by the host code in
+// mrm_step.hip; every kernel checks its own index range.
+#pragma once
+#include "mrm_device.h"
+
+#define RED_CMAX 0      // uint64 bits of the max characteristic speed
+#define RED_VMAX 1      // uint64 bits of max |v|
+#define RED_CS2MAX 2    // uint64 bits of max 5/3 p/rho
+#define RED_EMAX 3      // uint64 bits of max eps
+#define RED_CNT_V 4     // uint64 count |v| >= 0.95 v_lim
+#define RED_CNT_E 5     // uint64 count eps >= 0.95 e_lim
+#define RED_SUM0 8      // double sums written by k_sum_partials
+#define RED_NSLOT 16
+
+#define MRM_BLOCK 256
+
+// ---------------------------------------------------------------- helpers
+// non-negative doubles order like their bit patterns: max via integer atomics
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* slot, double v) {
+  if (v > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+struct OpAdd { static __device__ __forceinline__ double f(double a, double b) { return a + b; } };
+struct OpMax { static __device__ __forceinline__ double f(double a, double b) { return fmax(a, b); } };
+
+// block-wide reduction of N values; the result is valid in thread 0.
+// lds must hold N * blockDim.x / 64 doubles.
+template <class OP, int N>
+__device__ __forceinline__ void block_reduce(double (&v)[N], double* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; q++) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) x = OP::f(x, __shfl_xor(x, o, 64));
+    if (lane == 0) lds[q * nw + wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      double x = lds[q * nw];
+      for (int w = 1; w < nw; w++) x = OP::f(x, lds[q * nw + w]);
+      v[q] = x;
+    }
+  }
+}
+
+// interior index -> (i, j, k)
+__device__ __forceinline__ bool interior_ijk(const Geom& G, long t, int& i, int& j, int& k) {
+  const int nx = G.le0 - G.lb0 + 1, ny = G.le1 - G.lb1 + 1, nz = G.le2 - G.lb2 + 1;
+  if (t >= (long)nx * ny * nz) return false;
+  i = (int)(t % nx);
+  long r = t / nx;
+  j = (int)(r % ny);
+  k = (int)(r / ny);
+  i += G.lb0; j += G.lb1; k += G.lb2;
+  return true;
+}
+
+// ---------------------------------------------------------------- ConsToPrim
+template <int NOUT>
+__global__ void __launch_bounds__(MRM_BLOCK) k_cons_to_prim(Geom G, EosDev E, double* __restrict__ U,
+                                                            double* __restrict__ F) {
+  const long bs = G.bs;
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    double d = U[n], m1 = U[bs + n], m2 = U[2 * bs + n], m3 = U[3 * bs + n], e = U[4 * bs + n];
+    double out[5];
+    cons_to_prim_cell<NOUT>(E, d, m1, m2, m3, e, out);
+    U[bs + n] = m1; U[2 * bs + n] = m2; U[3 * bs + n] = m3; U[4 * bs + n] = e;
+    F[n] = out[0];
+    if (NOUT > 2) {
+#pragma unroll
+      for (int q = 1; q < 5; q++) F[q * bs + n] = out[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- Adjust_Valf_Max
+__global__ void __launch_bounds__(MRM_BLOCK) k_valf_reduce(Geom G, const double* __restrict__ U,
+                                                           const double* __restrict__ pres, double vlim95,
+                                                           double elim95, unsigned long long* red) {
+  __shared__ double lds[3 * MRM_BLOCK / 64];
+  __shared__ unsigned long long cnt[2];
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const long bs = G.bs;
+  double m[3] = {0.0, 0.0, 0.0};
+  unsigned long long c0 = 0, c1 = 0;
+  const long nint = (long)(G.le0 - G.lb0 + 1) * (G.le1 - G.lb1 + 1) * (G.le2 - G.lb2 + 1);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nint; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    interior_ijk(G, t, i, j, k);
+    long node = i + j * G.s1 + k * G.s2;
+    double vx = U[bs + node], vy = U[2 * bs + node], vz = U[3 * bs + node];
+    double vv = sqrt(vx * vx + vy * vy + vz * vz);
+    double ee = U[4 * bs + node] / U[node];
+    m[0] = fmax(m[0], vv);
+    m[1] = fmax(m[1], 5 / 3. * pres[node] / U[node]);
+    m[2] = fmax(m[2], ee);
+    if (vv >= vlim95) c0++;
+    if (ee >= elim95) c1++;
+  }
+  atomicAdd(&cnt[0], c0);
+  atomicAdd(&cnt[1], c1);
+  block_reduce<OpMax, 3>(m, lds);
+  if (threadIdx.x == 0) {
+    atomic_max_nonneg(&red[RED_VMAX], m[0]);
+    atomic_max_nonneg(&red[RED_CS2MAX], m[1]);
+    atomic_max_nonneg(&red[RED_EMAX], m[2]);
+    atomicAdd(&red[RED_CNT_V], cnt[0]);
+    atomicAdd(&red[RED_CNT_E], cnt[1]);
+  }
+}
+
+// ---------------------------------------------------------------- MHD_Residual
+struct ResidParams {
+  double w1[3], w2[3];
+  double inv_va2max;
+};
+
+__device__ __forceinline__ void load_pt(const double* __restrict__ U, const double* __restrict__ P, long bs,
+                                        long node, double v[MRM_NV], double& p) {
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) v[q] = U[q * bs + node];
+  p = P[node];
+}
+
+template <int D>
+__device__ __forceinline__ void resid_dir(const double* __restrict__ U, const double* __restrict__ Pr, long bs,
+                                          long node, long str, const double vc[MRM_NV], double pc,
+                                          const ResidParams& RP, double acc[MRM_NV]) {
+  double v[5][MRM_NV], p[5];
+  load_pt(U, Pr, bs, node - 2 * str, v[0], p[0]);
+  load_pt(U, Pr, bs, node - str, v[1], p[1]);
+  load_pt(U, Pr, bs, node + str, v[3], p[3]);
+  load_pt(U, Pr, bs, node + 2 * str, v[4], p[4]);
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) v[2][q] = vc[q];
+  p[2] = pc;
+  double res[MRM_NV];
+  sweep_residual<D>(v, p, RP.w1[D], RP.w2[D], RP.inv_va2max, res);
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+}
+
+// R += residual over the interior; cmax of the interior points into red
+__global__ void __launch_bounds__(MRM_BLOCK) k_residual(Geom G, ResidParams RP, const double* __restrict__ U,
+                                                        const double* __restrict__ Pr, double* __restrict__ R,
+                                                        unsigned long long* red) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const long bs = G.bs;
+  double cm[1] = {0.0};
+  int i, j, k;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (interior_ijk(G, t, i, j, k)) {
+    long node = i + j * G.s1 + k * G.s2;
+    double vc[MRM_NV], pc;
+    load_pt(U, Pr, bs, node, vc, pc);
+    double acc[MRM_NV];
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) acc[q] = R[q * bs + node];
+    resid_dir<0>(U, Pr, bs, node, 1, vc, pc, RP, acc);
+    resid_dir<1>(U, Pr, bs, node, G.s1, vc, pc, RP, acc);
+    resid_dir<2>(U, Pr, bs, node, G.s2, vc, pc, RP, acc);
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) R[q * bs + node] = acc[q];
+    cm[0] = smax(0.0, point_cmax(vc, pc, RP.inv_va2max));
+  }
+  block_reduce<OpMax, 1>(cm, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(&red[RED_CMAX], cm[0]);
+}
+
+// cmax over the ghost points that the sweeps of MHD_Residual visit: the two
+// ghost layers at each end of every interior line (mhdres_SR.C:237-328).
+__global__ void __launch_bounds__(MRM_BLOCK) k_cmax_faces(Geom G, double inv_va2max, const double* __restrict__ U,
+                                                          const double* __restrict__ Pr, unsigned long long* red) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const int nx = G.le0 - G.lb0 + 1, ny = G.le1 - G.lb1 + 1, nz = G.le2 - G.lb2 + 1;
+  const long f0 = (long)2 * G.g0 * ny * nz, f1 = (long)2 * G.g1 * nx * nz, f2 = (long)2 * G.g2 * nx * ny;
+  double cm[1] = {0.0};
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < f0 + f1 + f2; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    if (t < f0) {                          // dim-0 ghosts at interior (j,k)
+      long r = t;
+      int a = (int)(r % (2 * G.g0)); r /= 2 * G.g0;
+      j = G.lb1 + (int)(r % ny); k = G.lb2 + (int)(r / ny);
+      i = a < G.g0 ? a : G.le0 + 1 + (a - G.g0);
+    } else if (t < f0 + f1) {              // dim-1 ghosts at interior (i,k)
+      long r = t - f0;
+      i = G.lb0 + (int)(r % nx); r /= nx;
+      int a = (int)(r % (2 * G.g1)); k = G.lb2 + (int)(r / (2 * G.g1));
+      j = a < G.g1 ? a : G.le1 + 1 + (a - G.g1);
+    } else {                               // dim-2 ghosts at interior (i,j)
+      long r = t - f0 - f1;
+      i = G.lb0 + (int)(r % nx); r /= nx;
+      j = G.lb1 + (int)(r % ny); int a = (int)(r / ny);
+      k = a < G.g2 ? a : G.le2 + 1 + (a - G.g2);
+    }
+    long node = i + j * G.s1 + k * G.s2;
+    double v[MRM_NV], p;
+    load_pt(U, Pr, G.bs, node, v, p);
+    cm[0] = smax(cm[0], point_cmax(v, p, inv_va2max));
+  }
+  block_reduce<OpMax, 1>(cm, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(&red[RED_CMAX], cm[0]);
+}
+
+// ---------------------------------------------------------------- sources / integrate
+__global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParams SP, double* __restrict__ U,
+                                                                const double* __restrict__ U0, double* __restrict__ R,
+                                                                const double* __restrict__ Qtot) {
+  const long bs = G.bs;
+  int i, j, k;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (!interior_ijk(G, t, i, j, k)) return;
+  long node = i + j * G.s1 + k * G.s2;
+  double up[MRM_NV], r[MRM_NV], u0[MRM_NV], out[MRM_NV];
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    up[q] = U[q * bs + node];
+    r[q] = R[q * bs + node];
+    u0[q] = U0[q * bs + node];
+  }
+  double Qrad = Qtot ? Qtot[node] : 0.0;
+  source_integrate_cell(SP, up, r, u0, Qrad, out);
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    U[q * bs + node] = out[q];
+    R[q * bs + node] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(MRM_BLOCK) k_save_cons(Geom G, const double* __restrict__ U, double* __restrict__ U0) {
+  int i, j, k;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (!interior_ijk(G, t, i, j, k)) return;
+  long node = i + j * G.s1 + k * G.s2;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) U0[q * G.bs + node] = U[q * G.bs + node];
+}
+
+__global__ void __launch_bounds__(MRM_BLOCK) k_tcheck(Geom G, double rho_min, double eps_min, double eps_max,
+                                                      double vmax, double* __restrict__ U) {
+  int i, j, k;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (!interior_ijk(G, t, i, j, k)) return;
+  long node = i + j * G.s1 + k * G.s2;
+  double u[MRM_NV];
+#pragma unroll
+  for (int q = 0; q < 5; q++) u[q] = U[q * G.bs + node];
+  tcheck_vals(u, rho_min, eps_min, eps_max, vmax);
+#pragma unroll
+  for (int q = 0; q < 5; q++) U[q * G.bs + node] = u[q];
+}
+
+// ---------------------------------------------------------------- exchange (periodic, one rank)
+// dim 1 ghosts at interior (i,k); dim 2 ghosts at interior i and all j
+template <int DIM>
+__global__ void __launch_bounds__(MRM_BLOCK) k_exchange(Geom G, double* __restrict__ U) {
+  const int nx = G.le0 - G.lb0 + 1;
+  const int gh = DIM == 1 ? G.g1 : G.g2;
+  const int lb = DIM == 1 ? G.lb1 : G.lb2, le = DIM == 1 ? G.le1 : G.le2;
+  const int nb = DIM == 1 ? (G.le2 - G.lb2 + 1) : G.n1;     // extent of the other transverse dim
+  const int ob = DIM == 1 ? G.lb2 : 0;
+  const long sd = DIM == 1 ? G.s1 : G.s2, so = DIM == 1 ? G.s2 : G.s1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nx * nb * gh) return;
+  int i = G.lb0 + (int)(t % nx);
+  long r = t / nx;
+  int b = ob + (int)(r % nb);
+  int igh = (int)(r / nb);
+  long base = i + b * so;
+  long src_r = base + (long)(lb + igh) * sd, dst_r = base + (long)(le + 1 + igh) * sd;
+  long src_l = base + (long)(le - gh + 1 + igh) * sd, dst_l = base + (long)(lb - gh + igh) * sd;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    U[q * G.bs + dst_r] = U[q * G.bs + src_r];
+    U[q * G.bs + dst_l] = U[q * G.bs + src_l];
+  }
+}
+
+// ---------------------------------------------------------------- boundary conditions
+// per-block partial sums over the interior (j,k) of the two bottom layers:
+// mode 0: p_interp of layers lbeg, lbeg+1 (boundary_pdmp_1_fftw3.C:172-202)
+// mode 1: set_p_s_bc's five sums (boundary_pdmp_1_fftw3.C:516-536)
+template <int MODE>
+__global__ void __launch_bounds__(MRM_BLOCK) k_bc_sums(Geom G, EosDev E, const double* __restrict__ U,
+                                                       double* __restrict__ part) {
+  constexpr int NS = MODE == 0 ? 2 : 5;
+  __shared__ double lds[NS * MRM_BLOCK / 64];
+  const long bs = G.bs;
+  const int ny = G.le1 - G.lb1 + 1, nz = G.le2 - G.lb2 + 1;
+  double s[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) s[q] = 0.0;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < (long)ny * nz) {
+    int j = G.lb1 + (int)(t % ny), k = G.lb2 + (int)(t / ny);
+    long n1 = G.lb0 + j * G.s1 + k * G.s2, n2 = n1 + 1;
+    double d1 = U[n1], d2 = U[n2];
+    double m1x = U[bs + n1], m1y = U[2 * bs + n1], m1z = U[3 * bs + n1];
+    double m2x = U[bs + n2], m2y = U[2 * bs + n2], m2z = U[3 * bs + n2];
+    if (MODE == 0) {
+      double e1 = U[4 * bs + n1] - 0.5 * ((m1x * m1x + m1y * m1y + m1z * m1z) / d1);
+      double e2 = U[4 * bs + n2] - 0.5 * ((m2x * m2x + m2y * m2y + m2z * m2z) / d2);
+      s[0] = p_interp(E, e1 / d1, d1);
+      s[1] = p_interp(E, e2 / d2, d2);
+    } else {
+      double e1 = U[4 * bs + n1] - 0.5 * (m1x * m1x + m1y * m1y + m1z * m1z) / d1;
+      double e2 = U[4 * bs + n2] - 0.5 * (m2x * m2x + m2y * m2y + m2z * m2z) / d2;
+      double eps1 = e1 / d1, eps2 = e2 / d2;
+      s[0] = p_interp(E, eps1, d1);
+      s[1] = p_interp(E, eps2, d2);
+      double si = s_interp(E, eps1, d1);
+      if (m1x > 0.0) { s[2] = 1.0; s[3] = si; }
+      s[NS - 1] = si;
+    }
+  }
+  block_reduce<OpAdd, NS>(s, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; q++) part[q * gridDim.x + blockIdx.x] = s[q];
+  }
+}
+
+// fixed-order sum of per-block partials: out[q] = sum_b part[q*nb + b]
+__global__ void __launch_bounds__(MRM_BLOCK) k_sum_partials(const double* __restrict__ part, int nb, int nq,
+                                                            double* __restrict__ out) {
+  __shared__ double lds[MRM_BLOCK];
+  for (int q = 0; q < nq; q++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[q * nb + b];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double x = 0.0;
+      for (int w = 0; w < (int)blockDim.x; w++) x += lds[w];
+      out[q] = x;
+    }
+    __syncthreads();
+  }
+}
+
+struct BcParams {
+  double p_bc, s_bc, B_crit, eps_top, cc;
+  int open_top;
+};
+
+// bottom ghost layers (boundary_pdmp_1_fftw3.C:240-318), all (j,k) incl. ghosts;
+// psum[0..1] are the layer sums of p written by k_sum_partials
+__global__ void __launch_bounds__(MRM_BLOCK) k_bc_bottom(Geom G, EosDev E, BcParams B, const double* __restrict__ psum,
+                                                         double* __restrict__ U) {
+  const long bs = G.bs;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)G.n1 * G.n2) return;
+  const double p1_dmp = 0.8;
+  const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
+  int j = (int)(t % G.n1), k = (int)(t / G.n1);
+  long base = j * G.s1 + k * G.s2;
+  long n1 = G.lb0 + base, n2 = n1 + 1;
+  double u1[MRM_NV], u2[MRM_NV];
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+  u1[4] -= 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
+  u2[4] -= 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+  double eps1 = u1[4] / u1[0];
+  double eps2 = u2[4] / u2[0];
+  double pmag = 0.5 * u1[5] * u1[5];
+  double pmin1 = p_interp(E, eps1, u1[0]) - pmean1;
+  double smin1 = s_interp(E, eps1, u1[0]);
+  double smin2 = s_interp(E, eps2, u2[0]);
+  double s1, s2;
+  if (u1[1] >= 0.e0) { s1 = B.s_bc; s2 = B.s_bc; }
+  else { s1 = smin1; s2 = smin2; }
+  double c = B.p_bc / sqrt(pmean1 * pmean2);
+  double p1 = pmean1 * c;
+  double p2 = pmean1 * c * c;
+  p1 += pmin1 * p1_dmp;
+  p2 += pmin1 * p1_dmp * p1_dmp;
+  p1 -= pmag * (1.0 - p1_dmp);
+  p2 -= pmag * (1.0 - p1_dmp * p1_dmp);
+  c = u1[5] / B.B_crit;
+  c = pow(c, 4.0);
+  c = (1.0 - c) / (1.0 + c);
+  u1[1] *= c; u1[2] *= c; u1[3] *= c;
+  u2[1] *= c; u2[2] *= c; u2[3] *= c;
+  double rho1 = d3_interp(E, p1, s1);
+  double rho2 = d3_interp(E, p2, s2);
+  eps1 = eps3_interp(E, p1, s1);
+  eps2 = eps3_interp(E, p2, s2);
+  u1[0] = rho1;
+  u2[0] = rho2;
+  u1[4] = eps1 * rho1 + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
+  u2[4] = eps2 * rho2 + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+  long g1 = G.lb0 - 1 + base, g2 = G.lb0 - 2 + base;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
+}
+
+// top ghost layers, vertical field (boundary_pdmp_1_fftw3.C:402-474, bnd_pot = 0)
+__global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double* __restrict__ U) {
+  const long bs = G.bs;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)G.n1 * G.n2) return;
+  int j = (int)(t % G.n1), k = (int)(t / G.n1);
+  long base = j * G.s1 + k * G.s2;
+  long n1 = G.le0 + base, n2 = n1 - 1;
+  double u1[MRM_NV], u2[MRM_NV];
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+  if (B.open_top == 1) {
+    if (u1[1] < 0.0) { u1[1] *= -1; u2[1] *= -1; }
+  } else {
+    u1[1] *= -1; u2[1] *= -1;
+  }
+  if (B.eps_top > 0.0) {
+    u1[4] = B.eps_top * u1[0] + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
+    u2[4] = B.eps_top * u2[0] + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+  }
+  u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+  long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
+}
+
+// ---------------------------------------------------------------- TVD sweeps
+// Line sets (tvdlimit_SR.C:199-205, 671-672): sweep 0 covers every (j,k)
+// incl. ghosts; sweep 1 interior i and all k; sweep 2 interior i and j.
+template <int D, bool HAS_TAU>
+__global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U,
+                                                         const double* __restrict__ Pr) {
+  const long bs = G.bs;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long base, str;
+  int sz, gh, lineX = 0;
+  if (D == 0) {
+    if (t >= (long)G.n1 * G.n2) return;
+    int j = (int)(t % G.n1), k = (int)(t / G.n1);
+    base = j * G.s1 + k * G.s2; str = 1; sz = G.n0; gh = G.g0;
+  } else {
+    const int nx = G.le0 - G.lb0 + 1;
+    const int nb = D == 1 ? G.n2 : (G.le1 - G.lb1 + 1);
+    if (t >= (long)nx * nb) return;
+    int i = G.lb0 + (int)(t % nx);
+    int b = (int)(t / nx);
+    lineX = i;
+    if (D == 1) { base = i + b * G.s2; str = G.s1; sz = G.n1; gh = G.g1; }
+    else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
+  }
+  const double lhfb = D == 0 ? 0.0 : P.hfb[lineX];
+  const double lhft = D == 0 ? 0.0 : P.hft[lineX];
+  const double lqft = D == 0 ? 0.0 : P.qft[lineX];
+  const int istart = 1 + (P.is_gbeg0 ? gh : 0);
+
+  TvdPt W0, W1, W2, W3;      // points t-3, t-2, t-1, t
+  double S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
+  double qm = 0.0;
+  for (int s = 0; s < sz; s++) {
+    long node = base + s * str;
+    double u[MRM_NV];
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + node];
+    double tau = HAS_TAU ? P.tau[node] : 0.0;
+    double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
+    tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3);
+    if (s >= 2) {
+      const int ip = s - 1;   // slope at point s-1
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) S0[q] = S1[q];
+      tvd_slope(P, W1, W2, W3, D == 0 ? P.hfb[ip] : lhfb, D == 0 ? P.hft[ip] : lhft, S1);
+    }
+    if (s >= 3) {
+      const int ii = s - 2;   // interface between ii and ii+1; update of point ii
+      const double ih = D == 0 ? P.hfb[ii] : lhfb, it = D == 0 ? P.hft[ii] : lhft;
+      const double iq = D == 0 ? P.qft[ii] : lqft;
+      bool zb = (D == 0) && ((P.is_gend0 && ii == sz - gh - 1) || (P.is_gbeg0 && ii == gh - 1));
+      double q = tvd_iface<D>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F);
+      if (ii >= gh && ii <= sz - gh - 1) {
+        long nd = base + ii * str;
+        double uu[MRM_NV];
+#pragma unroll
+        for (int k = 0; k < MRM_NV; k++) uu[k] = U[k * bs + nd];
+        tvd_update(P, W1, Fm, F, qm, q, iq, uu);
+#pragma unroll
+        for (int k = 0; k < MRM_NV; k++) U[k * bs + nd] = uu[k];
+      }
+#pragma unroll
+      for (int k = 0; k < MRM_NV; k++) Fm[k] = F[k];
+      qm = q;
+    }
+    W0 = W1; W1 = W2; W2 = W3;
+  }
+}

@@ -1,0 +1,705 @@
+// mrm_step.hip -- C ABI (include/muram_step.h) of the MI355X-native MURaM
+// per-timestep MHD update: context, device memory, launches, and the host-side
+// scalar logic of Adjust_Valf_Max, Sync_timestep and set_p_s_bc.
+//
+// One context per GridData; all device work on the context's stream; the
+// state stays in HBM between calls.  Host round trips happen only where the
+// reference returns a scalar (dt_cfl, the limiter statistics, the first
+// p_bc/s_bc) and in mrm_set_state/mrm_get_state.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/muram_step.h"
+#include "mrm_kernels.h"
+
+struct mrm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mrm_grid_desc grid;
+  mrm_phys_desc phys;
+  int precision = MRM_PREC_FP64;
+  Geom G;
+  int lsize[3];
+  int vsize = 1;
+  double dx[3];
+  double dxmin = 0.0;
+
+  double* dU = nullptr;
+  double* dU0 = nullptr;
+  double* dR = nullptr;
+  double* dF = nullptr;   // MRM_NFIELD arrays of bs doubles
+  bool qtot_set = false, tau_set = false;
+
+  bool eos_ok = false;
+  EosDev E;
+  void* d_eos = nullptr;  // one allocation for axes and tables
+
+  double inv_va2max = 0.0, v_lim = 0.0, e_lim = 0.0, va_max_old = 0.0;
+  int valf_ini = 1;
+  double p_bc = -1.0, s_bc = -1.0;
+  int bnd_ini = 1;
+
+  double* d_prof = nullptr;  // hfb, hft, qft (3 * n0)
+  int tvd_ini = 1;
+
+  unsigned long long* d_red = nullptr;   // RED_NSLOT slots
+  double* d_part = nullptr;
+  long npart = 0;
+  unsigned long long* h_red = nullptr;   // pinned mirror of d_red
+
+  std::string err;
+};
+
+static int fail(mrm_ctx* c, int code, const char* what, hipError_t e = hipSuccess) {
+  if (c) {
+    char buf[512];
+    if (e != hipSuccess)
+      snprintf(buf, sizeof buf, "%s: %s (%s)", what, hipGetErrorString(e), hipGetErrorName(e));
+    else
+      snprintf(buf, sizeof buf, "%s", what);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(c, x)                                  \
+  do {                                                \
+    hipError_t e_ = (x);                              \
+    if (e_ != hipSuccess) return fail(c, MRM_ERR_HIP, #x, e_); \
+  } while (0)
+
+#define LAUNCHCHK(c, name)                            \
+  do {                                                \
+    hipError_t e_ = hipGetLastError();                \
+    if (e_ != hipSuccess) return fail(c, MRM_ERR_HIP, name, e_); \
+  } while (0)
+
+static inline unsigned nblk(long n) { return (unsigned)((n + MRM_BLOCK - 1) / MRM_BLOCK); }
+static inline long ninterior(const mrm_ctx* c) { return (long)c->lsize[0] * c->lsize[1] * c->lsize[2]; }
+static inline double smaxh(double a, double b) { return (a < b) ? b : a; }
+static inline double sminh(double a, double b) { return (b < a) ? b : a; }
+static inline double bits_to_double(unsigned long long b) {
+  double d;
+  memcpy(&d, &b, 8);
+  return d;
+}
+
+// ------------------------------------------------------------------ lifecycle
+extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, int precision, mrm_ctx** out) {
+  if (!grid || !phys || !out) return MRM_ERR_ARG;
+  *out = nullptr;
+  if (precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
+  // physics outside the implemented path
+  if (phys->params[3] > 0.0 || phys->params[4] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
+  if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
+  for (int d = 0; d < 3; d++) {
+    if (grid->gsize[d] < 1 || grid->ghosts[d] != 2) return MRM_ERR_ARG;
+    if (!(grid->gxmax[d] > grid->gxmin[d])) return MRM_ERR_ARG;
+  }
+  if (grid->periods[0] || !grid->periods[1] || !grid->periods[2]) return MRM_ERR_UNSUPPORTED;
+  if (grid->gsize[0] < 3) return MRM_ERR_ARG;
+  mrm_ctx* c = new mrm_ctx();
+  c->grid = *grid;
+  c->phys = *phys;
+  c->precision = precision;
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MRM_ERR_HIP; }
+  Geom& G = c->G;
+  int n[3];
+  for (int d = 0; d < 3; d++) {
+    c->lsize[d] = grid->gsize[d];
+    n[d] = grid->gsize[d] + 2 * grid->ghosts[d];
+    c->vsize = n[d] > c->vsize ? n[d] : c->vsize;
+    c->dx[d] = (grid->gxmax[d] - grid->gxmin[d]) / grid->gsize[d];
+  }
+  c->dxmin = c->dx[0];
+  for (int d = 1; d < 3; d++) c->dxmin = sminh(c->dxmin, c->dx[d]);
+  G.n0 = n[0]; G.n1 = n[1]; G.n2 = n[2];
+  G.g0 = grid->ghosts[0]; G.g1 = grid->ghosts[1]; G.g2 = grid->ghosts[2];
+  G.lb0 = G.g0; G.lb1 = G.g1; G.lb2 = G.g2;
+  G.le0 = grid->gsize[0] + G.g0 - 1; G.le1 = grid->gsize[1] + G.g1 - 1; G.le2 = grid->gsize[2] + G.g2 - 1;
+  G.s1 = n[0];
+  G.s2 = (long)n[0] * n[1];
+  G.bs = (long)n[0] * n[1] * n[2];
+  const size_t bs = (size_t)G.bs;
+  int rc = MRM_OK;
+  hipError_t e;
+  if ((e = hipStreamCreate(&c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipStreamCreate", e);
+  if (!rc && (e = hipMalloc(&c->dU, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc U", e);
+  if (!rc && (e = hipMalloc(&c->dU0, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc U0", e);
+  if (!rc && (e = hipMalloc(&c->dR, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc Res", e);
+  if (!rc && (e = hipMalloc(&c->dF, MRM_NFIELD * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc fields", e);
+  c->npart = 8 * (long)nblk((long)n[1] * n[2]) + 64;
+  if (!rc && (e = hipMalloc(&c->d_part, c->npart * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc part", e);
+  if (!rc && (e = hipMalloc(&c->d_red, RED_NSLOT * sizeof(unsigned long long))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc red", e);
+  if (!rc && (e = hipHostMalloc(&c->h_red, RED_NSLOT * sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipHostMalloc", e);
+  if (!rc && (e = hipMalloc(&c->d_prof, 3 * n[0] * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc prof", e);
+  // Res starts at zero (the reference relies on it, grid.C:279); all arrays defined
+  if (!rc && (e = hipMemsetAsync(c->dU, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && (e = hipMemsetAsync(c->dU0, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && (e = hipMemsetAsync(c->dR, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && (e = hipMemsetAsync(c->dF, 0, MRM_NFIELD * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && (e = hipStreamSynchronize(c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "sync", e);
+  if (rc) {
+    fprintf(stderr, "mrm_create: %s\n", c->err.c_str());
+    mrm_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MRM_OK;
+}
+
+extern "C" int mrm_destroy(mrm_ctx* c) {
+  if (!c) return MRM_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  hipFree(c->dU); hipFree(c->dU0); hipFree(c->dR); hipFree(c->dF);
+  hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
+  if (c->h_red) hipHostFree(c->h_red);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MRM_OK;
+}
+
+extern "C" const char* mrm_last_error(const mrm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int mrm_bufsize(const mrm_ctx* c, long* bufsize, int dims[3]) {
+  if (!c) return MRM_ERR_ARG;
+  if (bufsize) *bufsize = c->G.bs;
+  if (dims) { dims[0] = c->G.n0; dims[1] = c->G.n1; dims[2] = c->G.n2; }
+  return MRM_OK;
+}
+
+extern "C" int mrm_synchronize(mrm_ctx* c) {
+  if (!c) return MRM_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ EOS
+// eos_init (eos_init.C:57-212): axes in double from the float header; the
+// forward tables transposed to [i_eps][i_rho] and divided by del_eps*del_lr in
+// double before rounding to float (:170-175); inverse tables likewise (:195-196).
+extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
+  if (!c || !bytes) return MRM_ERR_ARG;
+  const float* buf = (const float*)bytes;
+  if (nbytes < 14 * sizeof(float)) return fail(c, MRM_ERR_EOS, "EOS table: short header");
+  int N_eps = (int)buf[0], N_lr = (int)buf[1], N_lp3 = (int)buf[2], N_s3 = (int)buf[3];
+  if (N_eps < 2 || N_lr < 2 || N_lp3 < 2 || N_s3 < 2) return fail(c, MRM_ERR_EOS, "EOS table: bad sizes");
+  size_t nt = (size_t)N_eps * N_lr, ni = (size_t)N_lp3 * N_s3;
+  if (nbytes < (14 + 6 * nt + 2 * ni) * sizeof(float)) return fail(c, MRM_ERR_EOS, "EOS table: truncated");
+  double eps0 = buf[4], eps1 = buf[5], lr0 = buf[6], lr1 = buf[7];
+  double lp0 = buf[8], lp1 = buf[9], s0 = buf[10], s1 = buf[11];
+  double del_eps = (eps1 - eps0) / ((double)(N_eps - 1));
+  double del_lr = (lr1 - lr0) / ((double)(N_lr - 1));
+  double del_lp3 = (lp1 - lp0) / ((double)(N_lp3 - 1));
+  double del_s3 = (s1 - s0) / ((double)(N_s3 - 1));
+  std::vector<double> ax(N_eps + N_lr + N_lp3 + N_s3);
+  double* xeps = ax.data();
+  double* xlr = xeps + N_eps;
+  double* xlp3 = xlr + N_lr;
+  double* xs3 = xlp3 + N_lp3;
+  for (int i = 0; i < N_eps; i++) xeps[i] = eps0 + del_eps * ((double)i);
+  for (int i = 0; i < N_lr; i++) xlr[i] = lr0 + del_lr * ((double)i);
+  for (int i = 0; i < N_lp3; i++) xlp3[i] = lp0 + del_lp3 * ((double)i);
+  for (int i = 0; i < N_s3; i++) xs3[i] = s0 + del_s3 * ((double)i);
+  std::vector<float> tab(6 * nt), itab(2 * ni);
+  const float* fw = buf + 14;
+  double sc = del_eps * del_lr;
+  for (int t = 0; t < 6; t++)
+    for (int i = 0; i < N_eps; i++)
+      for (int j = 0; j < N_lr; j++)
+        tab[t * nt + (size_t)i * N_lr + j] = (float)(((double)fw[t * nt + (size_t)j * N_eps + i]) / sc);
+  const float* iv = fw + 6 * nt;
+  double sci = del_lp3 * del_s3;
+  for (int t = 0; t < 2; t++)
+    for (int i = 0; i < N_lp3; i++)
+      for (int j = 0; j < N_s3; j++)
+        itab[t * ni + (size_t)i * N_s3 + j] = (float)(((double)iv[t * ni + (size_t)j * N_lp3 + i]) / sci);
+  size_t axb = ax.size() * sizeof(double);
+  size_t total = axb + (tab.size() + itab.size()) * sizeof(float);
+  if (c->d_eos) { hipFree(c->d_eos); c->d_eos = nullptr; }
+  HIPCHK(c, hipMalloc(&c->d_eos, total));
+  char* base = (char*)c->d_eos;
+  HIPCHK(c, hipMemcpy(base, ax.data(), axb, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(base + axb, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(base + axb + tab.size() * sizeof(float), itab.data(), itab.size() * sizeof(float),
+                      hipMemcpyHostToDevice));
+  EosDev& E = c->E;
+  E.N_eps = N_eps; E.N_lr = N_lr; E.N_lp3 = N_lp3; E.N_s3 = N_s3;
+  E.inv_del_eps = 1.0 / del_eps; E.inv_del_lr = 1.0 / del_lr;
+  E.inv_del_lp3 = 1.0 / del_lp3; E.inv_del_s3 = 1.0 / del_s3;
+  E.eps_off = buf[12];
+  E.ss_off = buf[13];
+  E.eps_max = exp(xeps[N_eps - 1]) - E.eps_off;   // cons_to_prim.C:52, host libm
+  double* dax = (double*)base;
+  E.xeps = dax; E.xlr = dax + N_eps; E.xlp3 = dax + N_eps + N_lr; E.xs3 = dax + N_eps + N_lr + N_lp3;
+  E.tab = (const float*)(base + axb);
+  E.itab = (const float*)(base + axb + tab.size() * sizeof(float));
+  c->eos_ok = true;
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ host <-> device
+static double* state_ptr(mrm_ctx* c, int which) {
+  return which == MRM_U ? c->dU : which == MRM_U0 ? c->dU0 : which == MRM_RES ? c->dR : nullptr;
+}
+
+extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layout) {
+  if (!c || !host) return MRM_ERR_ARG;
+  double* d = state_ptr(c, which);
+  if (!d) return fail(c, MRM_ERR_ARG, "mrm_set_state: bad array id");
+  const size_t bs = c->G.bs;
+  if (layout == MRM_LAYOUT_SOA) {
+    HIPCHK(c, hipMemcpyAsync(d, host, 8 * bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else if (layout == MRM_LAYOUT_AOS) {
+    std::vector<double> soa(8 * bs);
+    for (size_t n = 0; n < bs; n++)
+      for (int v = 0; v < 8; v++) soa[v * bs + n] = host[8 * n + v];
+    HIPCHK(c, hipMemcpyAsync(d, soa.data(), 8 * bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    return fail(c, MRM_ERR_ARG, "mrm_set_state: bad layout");
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRM_OK;
+}
+
+extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
+  if (!c || !host) return MRM_ERR_ARG;
+  double* d = state_ptr(c, which);
+  if (!d) return fail(c, MRM_ERR_ARG, "mrm_get_state: bad array id");
+  const size_t bs = c->G.bs;
+  if (layout == MRM_LAYOUT_SOA) {
+    HIPCHK(c, hipMemcpyAsync(host, d, 8 * bs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else if (layout == MRM_LAYOUT_AOS) {
+    std::vector<double> soa(8 * bs);
+    HIPCHK(c, hipMemcpyAsync(soa.data(), d, 8 * bs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t n = 0; n < bs; n++)
+      for (int v = 0; v < 8; v++) host[8 * n + v] = soa[v * bs + n];
+  } else {
+    return fail(c, MRM_ERR_ARG, "mrm_get_state: bad layout");
+  }
+  return MRM_OK;
+}
+
+extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
+  if (!c || !host || f < 0 || f >= MRM_NFIELD) return MRM_ERR_ARG;
+  const size_t bs = c->G.bs;
+  HIPCHK(c, hipMemcpyAsync(c->dF + f * bs, host, bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (f == MRM_QTOT) c->qtot_set = true;
+  if (f == MRM_TAU) c->tau_set = true;
+  return MRM_OK;
+}
+
+extern "C" int mrm_get_field(mrm_ctx* c, int f, double* host) {
+  if (!c || !host || f < 0 || f >= MRM_NFIELD) return MRM_ERR_ARG;
+  const size_t bs = c->G.bs;
+  HIPCHK(c, hipMemcpyAsync(host, c->dF + f * bs, bs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRM_OK;
+}
+
+extern "C" int mrm_get_limits(const mrm_ctx* c, double lim[4]) {
+  if (!c || !lim) return MRM_ERR_ARG;
+  lim[0] = c->inv_va2max; lim[1] = c->v_lim; lim[2] = c->e_lim; lim[3] = c->va_max_old;
+  return MRM_OK;
+}
+extern "C" int mrm_set_limits(mrm_ctx* c, const double lim[4]) {
+  if (!c || !lim) return MRM_ERR_ARG;
+  c->inv_va2max = lim[0]; c->v_lim = lim[1]; c->e_lim = lim[2]; c->va_max_old = lim[3];
+  return MRM_OK;
+}
+extern "C" int mrm_get_bc(const mrm_ctx* c, double* p_bc, double* s_bc) {
+  if (!c) return MRM_ERR_ARG;
+  if (p_bc) *p_bc = c->p_bc;
+  if (s_bc) *s_bc = c->s_bc;
+  return MRM_OK;
+}
+extern "C" int mrm_set_bc(mrm_ctx* c, double p_bc, double s_bc) {
+  if (!c) return MRM_ERR_ARG;
+  c->p_bc = p_bc; c->s_bc = s_bc;
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ ConsToPrim
+extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
+  if (!c) return MRM_ERR_ARG;
+  if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_cons_to_prim: no EOS table loaded");
+  unsigned nb = nblk(c->G.bs);
+  if (nb > 65536u * 4) nb = 65536u * 4;
+  k_cons_to_prim<5><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF);
+  LAUNCHCHK(c, "k_cons_to_prim");
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ Adjust_Valf_Max
+// device reductions + the host-side update of Adjust_Valf_Max.C:97-144
+extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
+  (void)run;
+  if (!c) return MRM_ERR_ARG;
+  HIPCHK(c, hipMemsetAsync(c->d_red, 0, RED_NSLOT * sizeof(unsigned long long), c->stream));
+  long n = ninterior(c);
+  unsigned nb = nblk(n);
+  if (nb > 4096) nb = 4096;
+  k_valf_reduce<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dF + MRM_PRES * c->G.bs, c->v_lim * 0.95,
+                                                 c->e_lim * 0.95, c->d_red);
+  LAUNCHCHK(c, "k_valf_reduce");
+  HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, RED_NSLOT * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double g_max[3] = {bits_to_double(c->h_red[RED_VMAX]), sqrt(bits_to_double(c->h_red[RED_CS2MAX])),
+                     bits_to_double(c->h_red[RED_EMAX])};
+  double g_sum[2] = {(double)c->h_red[RED_CNT_V], (double)c->h_red[RED_CNT_E]};
+  const double* prm = c->phys.params;
+  const double max_fill = prm[5], cs_fac = 2.0, vv_fac = 3.0, eps_decay = 0.99;
+  double va_max;
+  if (prm[2] > 0.0) {
+    if (c->valf_ini) {
+      c->v_lim = fmin(g_max[0], c->phys.tchk[3]);
+      c->e_lim = fmin(g_max[2], c->phys.tchk[2]);
+      c->va_max_old = 0.0;
+      c->valf_ini = 0;
+    } else {
+      if (g_sum[0] >= 4 * max_fill) c->v_lim *= 1.04;
+      else if (g_sum[0] >= max_fill) c->v_lim *= 1.01;
+      else if (g_sum[0] < 0.5 * max_fill) c->v_lim *= 0.99;
+      if (g_sum[1] >= 4 * max_fill) c->e_lim *= 1.04;
+      else if (g_sum[1] >= max_fill) c->e_lim *= 1.01;
+      else if (g_sum[1] < 0.5 * max_fill) c->e_lim *= 0.99;
+    }
+    va_max = fmax(prm[1], prm[2] * vv_fac * g_max[0]);
+    va_max = fmax(va_max, prm[2] * cs_fac * g_max[1]);
+    va_max = fmax(va_max, c->va_max_old * eps_decay);
+    c->v_lim = fmax(c->v_lim, prm[1] / vv_fac);
+    c->v_lim = fmin(c->v_lim, c->phys.tchk[3]);
+    c->e_lim = fmax(c->e_lim, pow(prm[1] / cs_fac, 2));
+    c->e_lim = fmin(c->e_lim, c->phys.tchk[2]);
+  } else {
+    va_max = prm[1];
+    c->v_lim = c->phys.tchk[3];
+    c->e_lim = c->phys.tchk[2];
+  }
+  c->inv_va2max = 1.0 / pow(va_max, 2);
+  c->va_max_old = va_max;
+  if (stats) {
+    stats[0] = g_max[0]; stats[1] = g_max[1]; stats[2] = g_max[2];
+    stats[3] = g_sum[0]; stats[4] = g_sum[1];
+  }
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ MHD_Residual
+static int launch_residual(mrm_ctx* c) {
+  ResidParams RP;
+  for (int d = 0; d < 3; d++) {
+    RP.w1[d] = 8. / (12. * c->dx[d]);
+    RP.w2[d] = -1. / (12. * c->dx[d]);
+  }
+  RP.inv_va2max = c->inv_va2max;
+  HIPCHK(c, hipMemsetAsync(c->d_red + RED_CMAX, 0, sizeof(unsigned long long), c->stream));
+  const double* P = c->dF + MRM_PRES * c->G.bs;
+  k_residual<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red);
+  LAUNCHCHK(c, "k_residual");
+  long nf = 2L * c->G.g0 * c->lsize[1] * c->lsize[2] + 2L * c->G.g1 * c->lsize[0] * c->lsize[2] +
+            2L * c->G.g2 * c->lsize[0] * c->lsize[1];
+  unsigned nb = nblk(nf);
+  if (nb > 2048) nb = 2048;
+  k_cmax_faces<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->inv_va2max, c->dU, P, c->d_red);
+  LAUNCHCHK(c, "k_cmax_faces");
+  return MRM_OK;
+}
+
+static int read_cmax(mrm_ctx* c, double* cmax) {
+  HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *cmax = bits_to_double(c->h_red[RED_CMAX]);
+  return MRM_OK;
+}
+
+extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_cfl) {
+  if (!c) return MRM_ERR_ARG;
+  if (run && run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
+  int rc = launch_residual(c);
+  if (rc) return rc;
+  double cmax;
+  rc = read_cmax(c, &cmax);
+  if (rc) return rc;
+  if (dt_cfl) *dt_cfl = c->dxmin / cmax;   // mhdres_SR.C:795
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ Sync_timestep
+static int invalid_dt(double a) { return (a != a) || (a <= 0.) || (a == INFINITY); }
+
+extern "C" int mrm_sync_timestep(mrm_ctx* c, mrm_run_desc* run, double dt_rad, double dt_mhd, double* cfl) {
+  if (!c || !run) return MRM_ERR_ARG;
+  double r0 = dt_rad, r1 = dt_mhd;
+  if (invalid_dt(r0)) return fail(c, MRM_ERR_INVALID_DT, "Invalid time step: dt_rad");
+  if (invalid_dt(r1)) return fail(c, MRM_ERR_INVALID_DT, "Invalid time step: dt_mhd");
+  double dt_min_rt = c->phys.rt[3] * run->CFL * r1;
+  if (r0 < dt_min_rt) r0 = smaxh(r0, dt_min_rt);
+  double dt_min = sminh(r0, r1);
+  double dt = sminh(r0, run->CFL * r1);
+  if (run->outcad > 0) {
+    double num = run->time / run->outcad;
+    int ceil_ = (int)num;
+    double rem = (num == (double)ceil_) ? 0.0 : (ceil_ + 1) * run->outcad - run->time;
+    if (rem > 0.0) dt = sminh(dt, rem);
+  }
+  if (run->autostep) run->dt = dt;
+  double cf = run->dt / dt_min;
+  if (cfl) *cfl = cf;
+  if (cf >= 2.) return fail(c, MRM_ERR_CFL, "Invalid time step: CFL >= 2");
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ sources / integrate
+static SrcParams src_params(const mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  SrcParams SP;
+  SP.wdt = run->dt / double(4 + 1 - stage);
+  SP.invRdt = c->phys.rt[4] / run->dt;
+  SP.g = -c->phys.params[0];
+  SP.inv_va2max = c->inv_va2max;
+  SP.rho_min = c->phys.tchk[1];
+  SP.eps_min = c->phys.tchk[0];
+  SP.vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
+  SP.eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  return SP;
+}
+
+extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
+  SrcParams SP = src_params(c, run, stage);
+  const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
+  k_source_integrate<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q);
+  LAUNCHCHK(c, "k_source_integrate");
+  return MRM_OK;
+}
+
+extern "C" int mrm_save_cons(mrm_ctx* c) {
+  if (!c) return MRM_ERR_ARG;
+  k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
+  LAUNCHCHK(c, "k_save_cons");
+  return MRM_OK;
+}
+
+extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
+  (void)run;
+  if (!c) return MRM_ERR_ARG;
+  double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
+  double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  k_tcheck<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax,
+                                                          c->dU);
+  LAUNCHCHK(c, "k_tcheck");
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ exchange
+extern "C" int mrm_exchange_grid(mrm_ctx* c) {
+  if (!c) return MRM_ERR_ARG;
+  const Geom& G = c->G;
+  long n1 = (long)c->lsize[0] * c->lsize[2] * G.g1;
+  k_exchange<1><<<nblk(n1), MRM_BLOCK, 0, c->stream>>>(G, c->dU);
+  LAUNCHCHK(c, "k_exchange<1>");
+  long n2 = (long)c->lsize[0] * G.n1 * G.g2;
+  k_exchange<2><<<nblk(n2), MRM_BLOCK, 0, c->stream>>>(G, c->dU);
+  LAUNCHCHK(c, "k_exchange<2>");
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ boundary conditions
+static int bottom_sums(mrm_ctx* c, int mode, double* dst_dev) {
+  const long nyz = (long)c->lsize[1] * c->lsize[2];
+  unsigned nb = nblk(nyz);
+  if (mode == 0) {
+    k_bc_sums<0><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->d_part);
+    LAUNCHCHK(c, "k_bc_sums<0>");
+  } else {
+    k_bc_sums<1><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->d_part);
+    LAUNCHCHK(c, "k_bc_sums<1>");
+  }
+  k_sum_partials<<<1, MRM_BLOCK, 0, c->stream>>>(c->d_part, (int)nb, mode == 0 ? 2 : 5, dst_dev);
+  LAUNCHCHK(c, "k_sum_partials");
+  return MRM_OK;
+}
+
+// set_p_s_bc (boundary_pdmp_1_fftw3.C:496-565), one rank
+static int set_p_s_bc(mrm_ctx* c) {
+  double* dsum = (double*)(c->d_red + RED_SUM0);
+  int rc = bottom_sums(c, 1, dsum);
+  if (rc) return rc;
+  double h[5];
+  HIPCHK(c, hipMemcpyAsync(h, dsum, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double nh = (double)(c->grid.gsize[1] * c->grid.gsize[2]);
+  h[0] /= nh;
+  h[1] /= nh;
+  if (c->p_bc < 0.0) c->p_bc = h[0] * sqrt(h[0] / h[1]);
+  if (c->s_bc < 0.0) {
+    if (h[2] > 0.0) c->s_bc = h[3] / h[2];
+    else c->s_bc = h[4] / nh;
+  }
+  return MRM_OK;
+}
+
+extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  (void)run; (void)stage;
+  if (!c) return MRM_ERR_ARG;
+  if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
+  int rc;
+  if (c->bnd_ini) {
+    if ((c->p_bc < 0.0) || (c->s_bc < 0.0)) {
+      rc = set_p_s_bc(c);
+      if (rc) return rc;
+    }
+    c->bnd_ini = 0;
+  }
+  double* dsum = (double*)(c->d_red + RED_SUM0);
+  rc = bottom_sums(c, 0, dsum);
+  if (rc) return rc;
+  BcParams B;
+  B.p_bc = c->p_bc;
+  B.s_bc = c->s_bc;
+  B.B_crit = c->phys.bnd[2] / sqrt(8.0 * asin(1.0));
+  B.eps_top = c->phys.bnd[3];
+  B.cc = 1.0 / ((double)(c->grid.gsize[1] * c->grid.gsize[2]));
+  B.open_top = (int)c->phys.bnd[0];
+  const long nyz = (long)c->G.n1 * c->G.n2;
+  k_bc_bottom<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, c->dU);
+  LAUNCHCHK(c, "k_bc_bottom");
+  k_bc_top<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, B, c->dU);
+  LAUNCHCHK(c, "k_bc_top");
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ TVD
+static int tvd_profiles(mrm_ctx* c, double h_bnd_bot, double h_bnd_top, double Qdiff_bnd) {
+  // tvdlimit_SR.C:137-149, evaluated once
+  const int n0 = c->G.n0;
+  std::vector<double> prof(3 * n0);
+  const double gxmax0 = c->grid.gxmax[0], lxmin0 = c->grid.gxmin[0], dx0 = c->dx[0];
+  for (int i = 0; i < n0; i++) {
+    double coord = lxmin0 + (0.5 + (i - c->G.lb0)) * dx0;
+    double hh = coord / gxmax0;
+    double vv = smaxh(0.0, 1.0 - 0.5 * (1.0 - hh) / h_bnd_top);
+    prof[n0 + i] = sminh(1.0, vv * vv);                 // hft
+    vv = smaxh(0.0, 1.0 - 0.5 * hh / h_bnd_bot);
+    prof[i] = sminh(1.0, vv * vv);                      // hfb
+    prof[2 * n0 + i] = (hh < 1 - 2 * h_bnd_top) ? 1.0 : Qdiff_bnd;   // qft
+  }
+  HIPCHK(c, hipMemcpy(c->d_prof, prof.data(), prof.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->tvd_ini = 0;
+  return MRM_OK;
+}
+
+extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
+  if (!c || !run) return MRM_ERR_ARG;
+  const mrm_phys_desc& Ph = c->phys;
+  TvdParams P;
+  P.rho_min = Ph.tchk[1];
+  P.eps_min = Ph.tchk[0];
+  P.rho_lev = pow(Ph.tvd[0], 2);
+  P.rho_log = (int)Ph.tvd[1];
+  P.q_rho_max = log(Ph.tvd[2]);
+  P.visc_coeff_bot = sminh(1.0, Ph.tvd_visc_bnd[0]);
+  P.visc_slope_bot = smaxh(0.0, Ph.tvd_visc_bnd[0] - 1.0);
+  P.visc_coeff_top = sminh(1.0, Ph.tvd_visc_bnd[1]);
+  P.visc_slope_top = smaxh(0.0, Ph.tvd_visc_bnd[1] - 1.0);
+  P.eta_coeff_bot = sminh(1.0, Ph.tvd_eta_bnd[0]);
+  P.eta_slope_bot = smaxh(0.0, Ph.tvd_eta_bnd[0] - 1.0);
+  P.eta_coeff_top = sminh(1.0, Ph.tvd_eta_bnd[1]);
+  P.eta_slope_top = smaxh(0.0, Ph.tvd_eta_bnd[1] - 1.0);
+  P.B_par_diff = Ph.tvd[3];
+  P.vhyp = Ph.tvd[4];
+  P.tvd_pm_v = Ph.tvd[6];
+  P.tvd_pm_B = Ph.tvd[7];
+  P.vmax_lim = Ph.tvd[8];
+  P.CME_thresh = Ph.tvd[9];
+  for (int q = 0; q < 4; q++) { P.tvd_h[q] = Ph.tvd_h[q]; P.tvd_cs[q] = Ph.tvd_cs[q]; P.tc[q] = Ph.tvd_coeff[q]; }
+  P.need_tvd_coeff = (P.tc[0] * P.tc[1] * P.tc[2] * P.tc[3] != 1.0);
+  P.inv_va2max = c->inv_va2max;
+  P.vsqr_diff = (P.vmax_lim < 1.0) ? pow(P.vmax_lim * c->v_lim, 2.0) : pow(P.vmax_lim, 2.0);
+  P.dt_tvd = dt_tvd;
+  P.is_gbeg0 = 1;
+  P.is_gend0 = 1;
+  if (c->tvd_ini) {
+    double hb = (Ph.tvd_h_bnd[0] < 1) ? Ph.tvd_h_bnd[0] : Ph.tvd_h_bnd[0] / double(c->grid.gsize[0]);
+    double ht = (Ph.tvd_h_bnd[1] < 1) ? Ph.tvd_h_bnd[1] : Ph.tvd_h_bnd[1] / double(c->grid.gsize[0]);
+    int rc = tvd_profiles(c, hb, ht, Ph.tvd[5]);
+    if (rc) return rc;
+  }
+  const int n0 = c->G.n0;
+  P.hfb = c->d_prof;
+  P.hft = c->d_prof + n0;
+  P.qft = c->d_prof + 2 * n0;
+  P.tau = c->tau_set ? c->dF + MRM_TAU * c->G.bs : nullptr;
+  const double* Pr = c->dF + MRM_PRES * c->G.bs;
+  const Geom& G = c->G;
+  const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
+  for (int d = 0; d < 3; d++) {
+    P.cmax = 0.975 * c->dx[d] / dt_tvd;
+    P.idxd1 = idx[d];
+    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
+    unsigned nb = nblk(nl);
+    const bool tau = P.tau != nullptr;
+    if (d == 0) {
+      if (tau) k_tvd_sweep<0, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+      else k_tvd_sweep<0, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+    } else if (d == 1) {
+      if (tau) k_tvd_sweep<1, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+      else k_tvd_sweep<1, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+    } else {
+      if (tau) k_tvd_sweep<2, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+      else k_tvd_sweep<2, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+    }
+    LAUNCHCHK(c, "k_tvd_sweep");
+  }
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ full step
+extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
+  if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
+  if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
+  int rc;
+#define STEPCHK(x) do { rc = (x); if (rc) return rc; } while (0)
+  for (int it = 0; it < nsteps; it++) {
+    double cfl = 0.0;
+    STEPCHK(mrm_save_cons(c));
+    for (int stage = 1; stage <= 4; stage++) {
+      STEPCHK(mrm_cons_to_prim(c));
+      if (stage == 1) STEPCHK(mrm_adjust_valf_max(c, run, nullptr));
+      if (stage == 1) {
+        double dt_mhd;
+        STEPCHK(mrm_mhd_residual(c, run, &dt_mhd));
+        STEPCHK(mrm_sync_timestep(c, run, dt_rad, dt_mhd, &cfl));
+      } else {
+        STEPCHK(launch_residual(c));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
+      }
+      STEPCHK(mrm_source_integrate_tcheck(c, run, stage));
+      STEPCHK(mrm_exchange_grid(c));
+      STEPCHK(mrm_set_boundary_conditions(c, run, stage));
+    }
+    if (cfl <= run->CFL_tvd) {
+      STEPCHK(mrm_tvdlimit(c, run, run->dt));
+    } else {
+      STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
+      STEPCHK(mrm_exchange_grid(c));
+      STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
+    }
+    STEPCHK(mrm_tcheck(c, run));
+    STEPCHK(mrm_exchange_grid(c));
+    STEPCHK(mrm_set_boundary_conditions(c, run, 5));
+    run->iteration++;
+    run->globiter++;
+    run->time += run->dt;
+  }
+#undef STEPCHK
+  return MRM_OK;
+}
