@@ -166,6 +166,17 @@ int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage
  * supplies (RT is outside the path). */
 int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
 
+/* Per-kernel device timers: the counterpart of the reference's per-routine
+ * timers printed at verbose > 1 (solver.C:41-43, :428-471).  When enabled,
+ * every launch is bracketed by HIP events on the context's stream.
+ * mrm_timers(ctx, 1) clears and starts, mrm_timers(ctx, 0) stops;
+ * mrm_get_timers synchronizes and returns the accumulated milliseconds and
+ * launch counts per timer id (MRM_NTIMER entries; names from mrm_timer_name). */
+#define MRM_NTIMER 24
+int mrm_timers(mrm_ctx* ctx, int enable);
+int mrm_get_timers(mrm_ctx* ctx, double ms[MRM_NTIMER], long launches[MRM_NTIMER]);
+const char* mrm_timer_name(int id);
+
 #ifdef __cplusplus
 }
 #endif

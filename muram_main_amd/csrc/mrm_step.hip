@@ -52,7 +52,58 @@ struct mrm_ctx {
   long npart = 0;
   unsigned long long* h_red = nullptr;   // pinned mirror of d_red
 
+  // per-kernel timers (solver.C:428-471 counterpart)
+  bool timing = false;
+  struct Rec { int id; hipEvent_t a, b; };
+  std::vector<hipEvent_t> evpool;
+  std::vector<Rec> pending;
+  double tms[MRM_NTIMER] = {0};
+  long tcnt[MRM_NTIMER] = {0};
+  hipEvent_t ev_get() {
+    if (!evpool.empty()) { hipEvent_t e = evpool.back(); evpool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+  }
+  void harvest() {
+    hipStreamSynchronize(stream);
+    for (auto& r : pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) { tms[r.id] += ms; tcnt[r.id]++; }
+      evpool.push_back(r.a);
+      evpool.push_back(r.b);
+    }
+    pending.clear();
+  }
+
   std::string err;
+};
+
+enum {
+  T_C2P = 0, T_VALF, T_RESID, T_CMAXF, T_SRCINT, T_SAVECONS, T_TCHECK, T_EXCH, T_BCSUM, T_BCBOT, T_BCTOP,
+  T_TVD0, T_TVD1, T_TVD2, T_NAMED
+};
+static const char* const timer_names[MRM_NTIMER] = {
+  "cons_to_prim", "valf_reduce", "residual", "cmax_faces", "source_integrate", "save_cons", "tcheck",
+  "exchange", "bc_sums", "bc_bottom", "bc_top", "tvd_sweep_x", "tvd_sweep_y", "tvd_sweep_z",
+  "t14", "t15", "t16", "t17", "t18", "t19", "t20", "t21", "t22", "t23"};
+
+// brackets the launches of one timer id with events on the context's stream
+struct TScope {
+  mrm_ctx* c;
+  int id;
+  hipEvent_t a = nullptr;
+  TScope(mrm_ctx* c_, int id_) : c(c_), id(id_) {
+    if (c->timing) { a = c->ev_get(); hipEventRecord(a, c->stream); }
+  }
+  ~TScope() {
+    if (a) {
+      hipEvent_t b = c->ev_get();
+      hipEventRecord(b, c->stream);
+      c->pending.push_back({id, a, b});
+      if (c->pending.size() > 8192) c->harvest();
+    }
+  }
 };
 
 static int fail(mrm_ctx* c, int code, const char* what, hipError_t e = hipSuccess) {
@@ -156,6 +207,8 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
 extern "C" int mrm_destroy(mrm_ctx* c) {
   if (!c) return MRM_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  c->harvest();
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   hipFree(c->dU); hipFree(c->dU0); hipFree(c->dR); hipFree(c->dF);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
   if (c->h_red) hipHostFree(c->h_red);
@@ -334,6 +387,7 @@ extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_cons_to_prim: no EOS table loaded");
   unsigned nb = nblk(c->G.bs);
   if (nb > 65536u * 4) nb = 65536u * 4;
+  TScope ts(c, T_C2P);
   k_cons_to_prim<5><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF);
   LAUNCHCHK(c, "k_cons_to_prim");
   return MRM_OK;
@@ -348,8 +402,10 @@ extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double s
   long n = ninterior(c);
   unsigned nb = nblk(n);
   if (nb > 4096) nb = 4096;
+  { TScope ts(c, T_VALF);
   k_valf_reduce<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dF + MRM_PRES * c->G.bs, c->v_lim * 0.95,
                                                  c->e_lim * 0.95, c->d_red);
+  }
   LAUNCHCHK(c, "k_valf_reduce");
   HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, RED_NSLOT * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -404,13 +460,17 @@ static int launch_residual(mrm_ctx* c) {
   RP.inv_va2max = c->inv_va2max;
   HIPCHK(c, hipMemsetAsync(c->d_red + RED_CMAX, 0, sizeof(unsigned long long), c->stream));
   const double* P = c->dF + MRM_PRES * c->G.bs;
+  { TScope ts(c, T_RESID);
   k_residual<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red);
+  }
   LAUNCHCHK(c, "k_residual");
   long nf = 2L * c->G.g0 * c->lsize[1] * c->lsize[2] + 2L * c->G.g1 * c->lsize[0] * c->lsize[2] +
             2L * c->G.g2 * c->lsize[0] * c->lsize[1];
   unsigned nb = nblk(nf);
   if (nb > 2048) nb = 2048;
+  { TScope ts(c, T_CMAXF);
   k_cmax_faces<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->inv_va2max, c->dU, P, c->d_red);
+  }
   LAUNCHCHK(c, "k_cmax_faces");
   return MRM_OK;
 }
@@ -477,6 +537,7 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
   if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
   SrcParams SP = src_params(c, run, stage);
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
+  TScope ts(c, T_SRCINT);
   k_source_integrate<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q);
   LAUNCHCHK(c, "k_source_integrate");
   return MRM_OK;
@@ -484,6 +545,7 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
 
 extern "C" int mrm_save_cons(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  TScope ts(c, T_SAVECONS);
   k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
   LAUNCHCHK(c, "k_save_cons");
   return MRM_OK;
@@ -494,6 +556,7 @@ extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
   if (!c) return MRM_ERR_ARG;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  TScope ts(c, T_TCHECK);
   k_tcheck<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax,
                                                           c->dU);
   LAUNCHCHK(c, "k_tcheck");
@@ -504,6 +567,7 @@ extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
 extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   const Geom& G = c->G;
+  TScope ts(c, T_EXCH);
   long n1 = (long)c->lsize[0] * c->lsize[2] * G.g1;
   k_exchange<1><<<nblk(n1), MRM_BLOCK, 0, c->stream>>>(G, c->dU);
   LAUNCHCHK(c, "k_exchange<1>");
@@ -517,6 +581,7 @@ extern "C" int mrm_exchange_grid(mrm_ctx* c) {
 static int bottom_sums(mrm_ctx* c, int mode, double* dst_dev) {
   const long nyz = (long)c->lsize[1] * c->lsize[2];
   unsigned nb = nblk(nyz);
+  TScope ts(c, T_BCSUM);
   if (mode == 0) {
     k_bc_sums<0><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->d_part);
     LAUNCHCHK(c, "k_bc_sums<0>");
@@ -571,9 +636,13 @@ extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, 
   B.cc = 1.0 / ((double)(c->grid.gsize[1] * c->grid.gsize[2]));
   B.open_top = (int)c->phys.bnd[0];
   const long nyz = (long)c->G.n1 * c->G.n2;
+  { TScope ts(c, T_BCBOT);
   k_bc_bottom<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, c->dU);
+  }
   LAUNCHCHK(c, "k_bc_bottom");
+  { TScope ts(c, T_BCTOP);
   k_bc_top<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, B, c->dU);
+  }
   LAUNCHCHK(c, "k_bc_top");
   return MRM_OK;
 }
@@ -648,6 +717,7 @@ extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) 
     long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
     unsigned nb = nblk(nl);
     const bool tau = P.tau != nullptr;
+    TScope ts(c, T_TVD0 + d);
     if (d == 0) {
       if (tau) k_tvd_sweep<0, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
       else k_tvd_sweep<0, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
@@ -702,4 +772,29 @@ extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps
   }
 #undef STEPCHK
   return MRM_OK;
+}
+
+// ------------------------------------------------------------------ timers
+extern "C" int mrm_timers(mrm_ctx* c, int enable) {
+  if (!c) return MRM_ERR_ARG;
+  c->harvest();
+  if (enable) {
+    for (int i = 0; i < MRM_NTIMER; i++) { c->tms[i] = 0.0; c->tcnt[i] = 0; }
+  }
+  c->timing = enable != 0;
+  return MRM_OK;
+}
+
+extern "C" int mrm_get_timers(mrm_ctx* c, double ms[MRM_NTIMER], long launches[MRM_NTIMER]) {
+  if (!c) return MRM_ERR_ARG;
+  c->harvest();
+  for (int i = 0; i < MRM_NTIMER; i++) {
+    if (ms) ms[i] = c->tms[i];
+    if (launches) launches[i] = c->tcnt[i];
+  }
+  return MRM_OK;
+}
+
+extern "C" const char* mrm_timer_name(int id) {
+  return (id >= 0 && id < MRM_NTIMER) ? timer_names[id] : "";
 }
