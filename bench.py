@@ -37,6 +37,9 @@ KERNEL_BYTES = {
     "tvd_sweep_x": (136.0, "interior"),   # read U 64 + p 8; write U 64
     "tvd_sweep_y": (136.0, "interior"),
     "tvd_sweep_z": (136.0, "interior"),
+    # fused path: stage kernel reads U_in 64 + p 8 + U0 64, writes U_out 64 + p 8
+    "stage_fused": (208.0, "interior"),
+    "eos_p": (72.0, "buffer"),
 }
 
 
@@ -80,7 +83,12 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-n", type=int, default=512)
+    ap.add_argument("--path", default="fused", choices=["fused", "sequence"])
+    ap.add_argument("--stage-zwin", default=None, choices=["0", "1"])
     args = ap.parse_args()
+    os.environ["MRM_STEP_PATH"] = args.path
+    if args.stage_zwin is not None:
+        os.environ["MRM_STAGE_ZWIN"] = args.stage_zwin
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -157,7 +165,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded stratified box, generated EOS table)",
         "config": {"workload": wl.name, "gsize": list(box.gsize), "dx_cm": box.dx[0],
-                   "precision": args.precision, "steps_timed": args.steps, "dt_s": r.dt},
+                   "precision": args.precision, "steps_timed": args.steps, "dt_s": r.dt,
+                   "path": args.path, "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "0")},
         "hbm_frac_model": round(value * 1e6 * MODEL_BYTES[args.precision] / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": roof,
         "kernels": {k: {"mean_ms": round(v["mean_ms"], 4), "launches": v["launches"],

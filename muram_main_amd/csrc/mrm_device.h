@@ -238,6 +238,21 @@ __device__ __forceinline__ double point_cmax(const double var[MRM_NV], double pr
   return vv + 0.0 + sqrt(smax(cs2, lf * (cs2 + va2)));
 }
 
+// cmax of a conservative point, primitives formed as ConsToPrim forms them
+__device__ __forceinline__ double point_cmax_cons(const double u[MRM_NV], double pres, double inv_va2max) {
+  double w[MRM_NV];
+  double dn = 1.0 / u[0];
+  w[0] = u[0];
+  w[1] = u[1] * dn;
+  w[2] = u[2] * dn;
+  w[3] = u[3] * dn;
+  w[4] = 0.0;
+  w[5] = u[5];
+  w[6] = u[6];
+  w[7] = u[7];
+  return point_cmax(w, pres, inv_va2max);
+}
+
 // Residual contribution of sweep D at the centre point given the point data of
 // the five points c-2..c+2 along D (mhdres_SR.C:361-412).  res must hold 0.
 template <int D>
@@ -272,6 +287,69 @@ __device__ __forceinline__ void sweep_residual(const double v[5][MRM_NV], const 
   res[2] += lrt1;
   res[3] += lrt2;
   res[4] += v[2][1] * lrt0 + v[2][2] * lrt1 + v[2][3] * lrt2;
+}
+
+// The same residual as sweep_residual with a lower register peak: the four
+// neighbour points are fetched through get(s, w[9]) (s = 0,1,3,4 for offsets
+// -2,-1,+1,+2) one pair at a time, and only the differences
+// F(+1)-F(-1) and F(+2)-F(-2) stay live.  Every arithmetic operation is the
+// one sweep_residual (and mhdres_SR.C:361-412) performs.
+template <int D, class GET>
+__device__ __forceinline__ void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
+                                                  double inv_va2max, double res[MRM_NV]) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  double d1[13], d2[13];   // 8 fluxes, 3 stresses, B_D2, B_D3
+  {
+    double pa[9], pb[9];
+    PtFlux fa, fb;
+    get(1, pa);
+    point_flux<D>(pa, pa[8], fa);
+    get(3, pb);
+    point_flux<D>(pb, pb[8], fb);
+#pragma unroll
+    for (int q = 0; q < 8; q++) d1[q] = fb.f[q] - fa.f[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) d1[8 + q] = fb.ms[q] - fa.ms[q];
+    d1[11] = pb[5 + D2] - pa[5 + D2];
+    d1[12] = pb[5 + D3] - pa[5 + D3];
+  }
+  {
+    double pa[9], pb[9];
+    PtFlux fa, fb;
+    get(0, pa);
+    point_flux<D>(pa, pa[8], fa);
+    get(4, pb);
+    point_flux<D>(pb, pb[8], fb);
+#pragma unroll
+    for (int q = 0; q < 8; q++) d2[q] = fb.f[q] - fa.f[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) d2[8 + q] = fb.ms[q] - fa.ms[q];
+    d2[11] = pb[5 + D2] - pa[5 + D2];
+    d2[12] = pb[5 + D3] - pa[5 + D3];
+  }
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    res[q] = 0.0;
+    res[q] -= w1 * d1[q] + w2 * d2[q];
+  }
+  double dB2 = w1 * d1[11] + w2 * d2[11];
+  double dB3 = w1 * d1[12] + w2 * d2[12];
+  double cb[3];
+  cb[D] = -c[5 + D2] * dB2 - c[5 + D3] * dB3;
+  cb[D2] = c[5 + D] * dB2;
+  cb[D3] = c[5 + D] * dB3;
+  double lrt0 = w1 * d1[8] + w2 * d2[8];
+  double lrt1 = w1 * d1[9] + w2 * d2[9];
+  double lrt2 = w1 * d1[10] + w2 * d2[10];
+  double lf = lfac_of(c, inv_va2max);
+  lrt0 = lf * lrt0 + (1.0 - lf) * cb[0];
+  lrt1 = lf * lrt1 + (1.0 - lf) * cb[1];
+  lrt2 = lf * lrt2 + (1.0 - lf) * cb[2];
+  res[1] += lrt0;
+  res[2] += lrt1;
+  res[3] += lrt2;
+  res[4] += c[1] * lrt0 + c[2] * lrt1 + c[3] * lrt2;
 }
 
 // ------------------------------------------------------------------ sources

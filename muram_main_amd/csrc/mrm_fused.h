@@ -1,0 +1,439 @@
+// mrm_fused.h -- kernels of the device-resident fused step.
+//
+// One RK stage is one kernel: the MHD residual of all three sweeps, the
+// sources, the Boris correction, U = U0 + wdt*R, the Tcheck clamp and the EOS
+// pressure of the new state, written to a second state buffer (the stage's
+// input stays intact, so U0 needs no copy and Res never touches HBM).  The
+// arithmetic per cell is the drop-in path's, term for term: the residual is
+// accumulated in the reference's order ((0 + R_x) + R_y) + R_z
+// (mhdres_SR.C:176, :456-466), primitives are formed with ConsToPrim's
+// expressions (cons_to_prim.C:110-118).
+//
+// Tiling: a 16x16 (dim0 x dim1) column tile per block, marched along dim 2.
+// The plane being updated sits in LDS with a two-cell ring in dims 0 and 1;
+// dim 2 neighbours come from a five-plane register window per thread.
+#pragma once
+#include "mrm_kernels.h"
+
+#define ST_SX 16
+#define ST_SY 16
+#define ST_KC 32
+#define ST_LX (ST_SX + 4)
+#define ST_LY (ST_SY + 4)
+#define ST_NF 9            // rho, v0, v1, v2, e_int, B0, B1, B2, p
+
+struct StageArgs {
+  Geom G;
+  EosDev E;
+  ResidParams RP;
+  SrcParams SP;
+  int ntx, nty, nkc;       // tiles in dim 0, dim 1, chunks in dim 2
+  int write_p;             // compute the EOS pressure of the new state
+};
+
+// conservative point -> primitive (ConsToPrim's arithmetic, cons_to_prim.C:110-118)
+__device__ __forceinline__ void load_prim(const double* __restrict__ U, const double* __restrict__ P, long bs,
+                                          long node, double w[ST_NF]) {
+  double d = U[node];
+  double dn = 1.0 / d;
+  double m1 = U[bs + node] * dn, m2 = U[2 * bs + node] * dn, m3 = U[3 * bs + node] * dn;
+  double vv = (m1 * m1 + m2 * m2 + m3 * m3);
+  w[0] = d;
+  w[1] = m1;
+  w[2] = m2;
+  w[3] = m3;
+  w[4] = U[4 * bs + node] - 0.5 * vv * d;
+  w[5] = U[5 * bs + node];
+  w[6] = U[6 * bs + node];
+  w[7] = U[7 * bs + node];
+  w[8] = P[node];
+}
+
+// map block index to a tile so that consecutive tiles share an XCD (blocks are
+// dealt to the 8 XCDs round robin)
+__device__ __forceinline__ long xcd_remap(long b, long nb) {
+  long q = nb / 8;
+  if (b < 8 * q) return (b % 8) * q + b / 8;
+  return b;
+}
+
+// ZWIN = true: dim-2 neighbours from a register window (one wave per SIMD);
+// ZWIN = false: dim-2 neighbours re-read through the caches each plane.
+template <bool ZWIN>
+__global__ void __launch_bounds__(ST_SX* ST_SY, ZWIN ? 1 : 2)
+k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ Pin, const double* __restrict__ U0,
+        double* __restrict__ Uout, double* __restrict__ Pout, const double* __restrict__ Qtot) {
+  __shared__ double lds[ST_NF][ST_LY][ST_LX];
+  const Geom& G = A.G;
+  const long bs = G.bs;
+  const long nblocks = (long)A.ntx * A.nty * A.nkc;
+  long lb = xcd_remap(blockIdx.x, nblocks);
+  const int tx_t = (int)(lb % A.ntx);
+  lb /= A.ntx;
+  const int ty_t = (int)(lb % A.nty);
+  const int kc = (int)(lb / A.nty);
+  const int tx = threadIdx.x % ST_SX, ty = threadIdx.x / ST_SX;
+  const int i0 = G.lb0 + tx_t * ST_SX, j0 = G.lb1 + ty_t * ST_SY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = (i <= G.le0) && (j <= G.le1);
+  const int kbeg = G.lb2 + kc * ST_KC;
+  const int kend = min(kbeg + ST_KC, G.le2 + 1);     // exclusive
+  // halo assignment: tx<2 / tx>=SX-2 load the dim-0 ring of their row,
+  // ty<4 load one point of the dim-1 ring rows
+  int hx_c = -1, hx_i = 0;
+  if (tx < 2) { hx_c = tx; hx_i = i0 - 2 + tx; }
+  else if (tx >= ST_SX - 2) { hx_c = tx + 4; hx_i = i0 + tx + 2; }
+  int hy_r = -1, hy_j = 0;
+  if (ty < 4) {
+    hy_r = ty < 2 ? ty : ST_SY + ty;          // rows 0,1 and SY+2, SY+3
+    hy_j = ty < 2 ? j0 - 2 + ty : j0 + ST_SY + (ty - 2);
+  }
+  const bool hx_ok = hx_c >= 0 && hx_i < G.n0 && j < G.n1;
+  const bool hy_ok = hy_r >= 0 && hy_j < G.n1 && i < G.n0;
+  const bool own_ok = i < G.n0 && j < G.n1;
+  const long col = (long)i + (long)j * G.s1;
+
+  double w[5][ST_NF];   // planes k-2 .. k+2 of this thread's column (w[2] is the centre)
+  if (ZWIN) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int kk = kbeg - 2 + q;
+      if (own_ok) load_prim(Uin, Pin, bs, col + kk * G.s2, w[q + 1]);
+      else {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0;
+      }
+    }
+  }
+  for (int k = kbeg; k < kend; k++) {
+    if (ZWIN) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
+      if (own_ok) load_prim(Uin, Pin, bs, col + (long)(k + 2) * G.s2, w[4]);
+    } else {
+      if (own_ok) load_prim(Uin, Pin, bs, col + (long)k * G.s2, w[2]);
+      else {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) w[2][f] = 1.0;
+      }
+    }
+    // plane k into LDS
+#pragma unroll
+    for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][tx + 2] = w[2][f];
+    if (hx_ok) {
+      double h[ST_NF];
+      load_prim(Uin, Pin, bs, (long)hx_i + (long)j * G.s1 + (long)k * G.s2, h);
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][hx_c] = h[f];
+    }
+    if (hy_ok) {
+      double h[ST_NF];
+      load_prim(Uin, Pin, bs, (long)i + (long)hy_j * G.s1 + (long)k * G.s2, h);
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) lds[f][hy_r][tx + 2] = h[f];
+    }
+    __syncthreads();
+    if (valid) {
+      double acc[MRM_NV], res[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] = 0.0;
+      auto getx = [&](int s, double o[9]) {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + 2][tx + s];
+      };
+      sweep_residual_lr<0>(getx, w[2], A.RP.w1[0], A.RP.w2[0], A.RP.inv_va2max, res);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      auto gety = [&](int s, double o[9]) {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + s][tx + 2];
+      };
+      sweep_residual_lr<1>(gety, w[2], A.RP.w1[1], A.RP.w2[1], A.RP.inv_va2max, res);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      auto getz = [&](int s, double o[9]) {
+        if (ZWIN) {
+#pragma unroll
+          for (int f = 0; f < ST_NF; f++) o[f] = w[s][f];
+        } else {
+          load_prim(Uin, Pin, bs, col + (long)(k + s - 2) * G.s2, o);
+        }
+      };
+      sweep_residual_lr<2>(getz, w[2], A.RP.w1[2], A.RP.w2[2], A.RP.inv_va2max, res);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      // sources, Boris, RK update, Tcheck
+      const long node = col + (long)k * G.s2;
+      double u0[MRM_NV], out[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) u0[q] = U0[q * bs + node];
+      double Qrad = Qtot ? Qtot[node] : 0.0;
+      source_integrate_cell(A.SP, w[2], acc, u0, Qrad, out);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) Uout[q * bs + node] = out[q];
+      if (A.write_p) {
+        double m1 = out[1], m2 = out[2], m3 = out[3], e = out[4], pr[5];
+        cons_to_prim_cell<2>(A.E, out[0], m1, m2, m3, e, pr);
+        Pout[node] = pr[0];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ ghost fill of the fused path
+// periodic copies of U and p (exchange.C:532-782 on one rank): dim 1 at
+// interior (i,k), dim 2 at interior i and all j
+template <int DIM>
+__global__ void __launch_bounds__(MRM_BLOCK) k_exchange_p(Geom G, double* __restrict__ U, double* __restrict__ P) {
+  const int nx = G.le0 - G.lb0 + 1;
+  const int gh = DIM == 1 ? G.g1 : G.g2;
+  const int lb = DIM == 1 ? G.lb1 : G.lb2, le = DIM == 1 ? G.le1 : G.le2;
+  const int nb = DIM == 1 ? (G.le2 - G.lb2 + 1) : G.n1;
+  const int ob = DIM == 1 ? G.lb2 : 0;
+  const long sd = DIM == 1 ? G.s1 : G.s2, so = DIM == 1 ? G.s2 : G.s1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nx * nb * gh) return;
+  int i = G.lb0 + (int)(t % nx);
+  long r = t / nx;
+  int b = ob + (int)(r % nb);
+  int igh = (int)(r / nb);
+  long base = i + b * so;
+  long src_r = base + (long)(lb + igh) * sd, dst_r = base + (long)(le + 1 + igh) * sd;
+  long src_l = base + (long)(le - gh + 1 + igh) * sd, dst_l = base + (long)(lb - gh + igh) * sd;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    U[q * G.bs + dst_r] = U[q * G.bs + src_r];
+    U[q * G.bs + dst_l] = U[q * G.bs + src_l];
+  }
+  if (P) {
+    P[dst_r] = P[src_r];
+    P[dst_l] = P[src_l];
+  }
+}
+
+// pressure of a freshly written ghost cell (ConsToPrim's p, cons_to_prim.C:108-198)
+__device__ __forceinline__ double ghost_pres(const EosDev& E, const double u[MRM_NV]) {
+  double m1 = u[1], m2 = u[2], m3 = u[3], e = u[4], pr[5];
+  cons_to_prim_cell<2>(E, u[0], m1, m2, m3, e, pr);
+  return pr[0];
+}
+
+// bottom and top ghost layers with their pressures; when cmax_out is set, the
+// characteristic speed of the dim-0 ghost cells on interior (j,k) lines is
+// reduced into it (mhdres_SR.C:237-328 visits them)
+__global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcParams B, const double* __restrict__ psum,
+                                                        double* __restrict__ U, double* __restrict__ P,
+                                                        double inv_va2max, unsigned long long* cmax_out) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const long bs = G.bs;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double cm[1] = {0.0};
+  if (t < (long)G.n1 * G.n2) {
+    const double p1_dmp = 0.8;
+    const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
+    int j = (int)(t % G.n1), k = (int)(t / G.n1);
+    const bool inner = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
+    long base = j * G.s1 + k * G.s2;
+    // ---- bottom (boundary_pdmp_1_fftw3.C:240-318)
+    {
+      long n1 = G.lb0 + base, n2 = n1 + 1;
+      double u1[MRM_NV], u2[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+      u1[4] -= 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
+      u2[4] -= 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+      double eps1 = u1[4] / u1[0];
+      double eps2 = u2[4] / u2[0];
+      double pmag = 0.5 * u1[5] * u1[5];
+      double pmin1 = p_interp(E, eps1, u1[0]) - pmean1;
+      double smin1 = s_interp(E, eps1, u1[0]);
+      double smin2 = s_interp(E, eps2, u2[0]);
+      double s1, s2;
+      if (u1[1] >= 0.e0) { s1 = B.s_bc; s2 = B.s_bc; }
+      else { s1 = smin1; s2 = smin2; }
+      double c = B.p_bc / sqrt(pmean1 * pmean2);
+      double p1 = pmean1 * c;
+      double p2 = pmean1 * c * c;
+      p1 += pmin1 * p1_dmp;
+      p2 += pmin1 * p1_dmp * p1_dmp;
+      p1 -= pmag * (1.0 - p1_dmp);
+      p2 -= pmag * (1.0 - p1_dmp * p1_dmp);
+      c = u1[5] / B.B_crit;
+      c = pow(c, 4.0);
+      c = (1.0 - c) / (1.0 + c);
+      u1[1] *= c; u1[2] *= c; u1[3] *= c;
+      u2[1] *= c; u2[2] *= c; u2[3] *= c;
+      double rho1 = d3_interp(E, p1, s1);
+      double rho2 = d3_interp(E, p2, s2);
+      eps1 = eps3_interp(E, p1, s1);
+      eps2 = eps3_interp(E, p2, s2);
+      u1[0] = rho1;
+      u2[0] = rho2;
+      u1[4] = eps1 * rho1 + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
+      u2[4] = eps2 * rho2 + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+      long g1 = G.lb0 - 1 + base, g2 = G.lb0 - 2 + base;
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
+      double pg1 = ghost_pres(E, u1), pg2 = ghost_pres(E, u2);
+      P[g1] = pg1;
+      P[g2] = pg2;
+      if (cmax_out && inner) {
+        cm[0] = smax(cm[0], point_cmax_cons(u1, pg1, inv_va2max));
+        cm[0] = smax(cm[0], point_cmax_cons(u2, pg2, inv_va2max));
+      }
+    }
+    // ---- top (boundary_pdmp_1_fftw3.C:402-474), vertical field
+    {
+      long n1 = G.le0 + base, n2 = n1 - 1;
+      double u1[MRM_NV], u2[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+      if (B.open_top == 1) {
+        if (u1[1] < 0.0) { u1[1] *= -1; u2[1] *= -1; }
+      } else {
+        u1[1] *= -1; u2[1] *= -1;
+      }
+      if (B.eps_top > 0.0) {
+        u1[4] = B.eps_top * u1[0] + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
+        u2[4] = B.eps_top * u2[0] + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+      }
+      u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+      long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
+      double pg1 = ghost_pres(E, u1), pg2 = ghost_pres(E, u2);
+      P[g1] = pg1;
+      P[g2] = pg2;
+      if (cmax_out && inner) {
+        cm[0] = smax(cm[0], point_cmax_cons(u1, pg1, inv_va2max));
+        cm[0] = smax(cm[0], point_cmax_cons(u2, pg2, inv_va2max));
+      }
+    }
+  }
+  if (cmax_out) {
+    block_reduce<OpMax, 1>(cm, lds);
+    if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
+  }
+}
+
+// pressure (and optionally cmax over the interior and the dim-0 ghost faces)
+// of a conservative state over the whole buffer, U left untouched
+__global__ void __launch_bounds__(MRM_BLOCK) k_eos_p(Geom G, EosDev E, const double* __restrict__ U,
+                                                     double* __restrict__ P, double inv_va2max,
+                                                     unsigned long long* cmax_out) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const long bs = G.bs;
+  double cm[1] = {0.0};
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    double u[MRM_NV];
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + n];
+    double p = ghost_pres(E, u);
+    P[n] = p;
+    if (cmax_out) {
+      int i = (int)(n % G.n0);
+      long r = n / G.n0;
+      int j = (int)(r % G.n1), k = (int)(r / G.n1);
+      bool ii = i >= G.lb0 && i <= G.le0, ji = j >= G.lb1 && j <= G.le1, ki = k >= G.lb2 && k <= G.le2;
+      // interior, or a ghost cell on an interior line of its own direction
+      bool vis = (ii && ji && ki) || (!ii && ji && ki) || (ii && !ji && ki) || (ii && ji && !ki);
+      if (vis) cm[0] = smax(cm[0], point_cmax_cons(u, p, inv_va2max));
+    }
+  }
+  if (cmax_out) {
+    block_reduce<OpMax, 1>(cm, lds);
+    if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
+  }
+}
+
+// ------------------------------------------------------------------ TVD, dim-0 sweep, one block per line
+// thread t owns point t of the line; the per-point, per-slope, per-interface
+// and per-update arithmetic is tvd_point / tvd_slope / tvd_iface / tvd_update
+#define TX_MAXN 1024
+struct TvdLineArgs {
+  Geom G;
+  TvdParams P;
+};
+
+__global__ void __launch_bounds__(TX_MAXN) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
+                                                        const double* __restrict__ Pr) {
+  extern __shared__ double sm[];
+  const Geom& G = A.G;
+  const TvdParams& P = A.P;
+  const long bs = G.bs;
+  const int sz = G.n0, gh = G.g0;
+  const long nlines = (long)G.n1 * G.n2;
+  const long line = xcd_remap(blockIdx.x, nlines);
+  const int j = (int)(line % G.n1), k = (int)(line / G.n1);
+  const long base = j * G.s1 + k * G.s2;
+  double* sv = sm;                     // [8][sz] primitives (log rho/eps)
+  double* sS = sv + MRM_NV * sz;       // [8][sz] slopes
+  double* sF = sS + MRM_NV * sz;       // [8][sz] interface fluxes
+  double* scm = sF + MRM_NV * sz;      // cm
+  double* shd = scm + sz;              // hyperdiffusion coefficient
+  double* scz = shd + sz;              // CZ
+  double* slr = scz + sz;              // ln rho
+  double* sq = slr + sz;               // resistive interface term
+  const int t = threadIdx.x;
+  TvdPt me;
+  double u[MRM_NV];
+  if (t < sz) {
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + base + t];
+    double tau = P.tau ? P.tau[base + t] : 0.0;
+    tvd_point(P, u, Pr[base + t], tau, P.hfb[t], P.hft[t], true, me);
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) sv[q * sz + t] = me.v[q];
+    scm[t] = me.cm;
+    shd[t] = me.hd;
+    scz[t] = me.CZ;
+    slr[t] = me.lnr;
+  }
+  __syncthreads();
+  // slopes need v at t-1, t+1 and ln rho at t+1: rebuild the neighbour
+  // points' fields that tvd_slope reads
+  double S[MRM_NV];
+  if (t >= 1 && t <= sz - 2) {
+    TvdPt a, c;
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) { a.v[q] = sv[q * sz + t - 1]; c.v[q] = sv[q * sz + t + 1]; }
+    c.lnr = slr[t + 1];
+    tvd_slope(P, a, me, c, P.hfb[t], P.hft[t], S);
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) sS[q * sz + t] = S[q];
+  }
+  __syncthreads();
+  double F[MRM_NV];
+  const int istart = 1 + (P.is_gbeg0 ? gh : 0);
+  if (t >= 1 && t <= sz - 3) {
+    TvdPt a, c, d;
+    double Sc[MRM_NV];
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) {
+      a.v[q] = sv[q * sz + t - 1];
+      c.v[q] = sv[q * sz + t + 1];
+      d.v[q] = sv[q * sz + t + 2];
+      Sc[q] = sS[q * sz + t + 1];
+    }
+    c.cm = scm[t + 1];
+    c.hd = shd[t + 1];
+    c.CZ = scz[t + 1];
+    bool zb = (P.is_gend0 && t == sz - gh - 1) || (P.is_gbeg0 && t == gh - 1);
+    double q = tvd_iface<0>(P, a, me, c, d, S, Sc, P.hfb[t], P.hft[t], P.qft[t], t >= istart, zb, F);
+#pragma unroll
+    for (int qq = 0; qq < MRM_NV; qq++) sF[qq * sz + t] = F[qq];
+    sq[t] = q;
+  }
+  __syncthreads();
+  if (t >= gh && t <= sz - gh - 1) {
+    double Fm[MRM_NV];
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) Fm[q] = sF[q * sz + t - 1];
+    tvd_update(P, me, Fm, F, sq[t - 1], sq[t], P.qft[t], u);
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) U[q * bs + base + t] = u[q];
+  }
+}

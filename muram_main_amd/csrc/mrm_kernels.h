@@ -427,68 +427,99 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double
 // ---------------------------------------------------------------- TVD sweeps
 // Line sets (tvdlimit_SR.C:199-205, 671-672): sweep 0 covers every (j,k)
 // incl. ghosts; sweep 1 interior i and all k; sweep 2 interior i and j.
-template <int D, bool HAS_TAU>
+// With EPI, the last sweep also applies TCheck to each finished cell
+// (Add_Sources_Integrate_SR.C:591-624; every interior cell is finished by the
+// dim-2 sweep), writes its EOS pressure and reduces its characteristic speed,
+// which is what the next step's ConsToPrim and MHD_Residual compute from it.
+struct TvdEpilogue {
+  EosDev E;
+  double rho_min, eps_min, eps_max, vmax, inv_va2max;
+  double* Pnew;
+  unsigned long long* cmax;
+};
+
+__device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
+  double m1 = u[1], m2 = u[2], m3 = u[3], e = u[4], pr[5];
+  cons_to_prim_cell<2>(E, u[0], m1, m2, m3, e, pr);
+  return pr[0];
+}
+
+template <int D, bool HAS_TAU, bool EPI>
 __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U,
-                                                         const double* __restrict__ Pr) {
+                                                         const double* __restrict__ Pr, TvdEpilogue X) {
+  __shared__ double lds[MRM_BLOCK / 64];
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  long base, str;
-  int sz, gh, lineX = 0;
+  long base = 0, str = 1;
+  int sz = 0, gh = 0, lineX = 0;
+  bool active;
   if (D == 0) {
-    if (t >= (long)G.n1 * G.n2) return;
+    active = t < (long)G.n1 * G.n2;
     int j = (int)(t % G.n1), k = (int)(t / G.n1);
     base = j * G.s1 + k * G.s2; str = 1; sz = G.n0; gh = G.g0;
   } else {
     const int nx = G.le0 - G.lb0 + 1;
     const int nb = D == 1 ? G.n2 : (G.le1 - G.lb1 + 1);
-    if (t >= (long)nx * nb) return;
+    active = t < (long)nx * nb;
     int i = G.lb0 + (int)(t % nx);
     int b = (int)(t / nx);
     lineX = i;
     if (D == 1) { base = i + b * G.s2; str = G.s1; sz = G.n1; gh = G.g1; }
     else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
   }
-  const double lhfb = D == 0 ? 0.0 : P.hfb[lineX];
-  const double lhft = D == 0 ? 0.0 : P.hft[lineX];
-  const double lqft = D == 0 ? 0.0 : P.qft[lineX];
-  const int istart = 1 + (P.is_gbeg0 ? gh : 0);
-
-  TvdPt W0, W1, W2, W3;      // points t-3, t-2, t-1, t
-  double S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
-  double qm = 0.0;
-  for (int s = 0; s < sz; s++) {
-    long node = base + s * str;
-    double u[MRM_NV];
+  double cm[1] = {0.0};
+  if (active) {
+    const double lhfb = D == 0 ? 0.0 : P.hfb[lineX];
+    const double lhft = D == 0 ? 0.0 : P.hft[lineX];
+    const double lqft = D == 0 ? 0.0 : P.qft[lineX];
+    const int istart = 1 + (P.is_gbeg0 ? gh : 0);
+    TvdPt W0, W1, W2, W3;      // points t-3, t-2, t-1, t
+    double S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
+    double qm = 0.0;
+    for (int s = 0; s < sz; s++) {
+      long node = base + s * str;
+      double u[MRM_NV];
 #pragma unroll
-    for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + node];
-    double tau = HAS_TAU ? P.tau[node] : 0.0;
-    double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
-    tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3);
-    if (s >= 2) {
-      const int ip = s - 1;   // slope at point s-1
+      for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + node];
+      double tau = HAS_TAU ? P.tau[node] : 0.0;
+      double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
+      tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3);
+      if (s >= 2) {
+        const int ip = s - 1;   // slope at point s-1
 #pragma unroll
-      for (int q = 0; q < MRM_NV; q++) S0[q] = S1[q];
-      tvd_slope(P, W1, W2, W3, D == 0 ? P.hfb[ip] : lhfb, D == 0 ? P.hft[ip] : lhft, S1);
-    }
-    if (s >= 3) {
-      const int ii = s - 2;   // interface between ii and ii+1; update of point ii
-      const double ih = D == 0 ? P.hfb[ii] : lhfb, it = D == 0 ? P.hft[ii] : lhft;
-      const double iq = D == 0 ? P.qft[ii] : lqft;
-      bool zb = (D == 0) && ((P.is_gend0 && ii == sz - gh - 1) || (P.is_gbeg0 && ii == gh - 1));
-      double q = tvd_iface<D>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F);
-      if (ii >= gh && ii <= sz - gh - 1) {
-        long nd = base + ii * str;
-        double uu[MRM_NV];
-#pragma unroll
-        for (int k = 0; k < MRM_NV; k++) uu[k] = U[k * bs + nd];
-        tvd_update(P, W1, Fm, F, qm, q, iq, uu);
-#pragma unroll
-        for (int k = 0; k < MRM_NV; k++) U[k * bs + nd] = uu[k];
+        for (int q = 0; q < MRM_NV; q++) S0[q] = S1[q];
+        tvd_slope(P, W1, W2, W3, D == 0 ? P.hfb[ip] : lhfb, D == 0 ? P.hft[ip] : lhft, S1);
       }
+      if (s >= 3) {
+        const int ii = s - 2;   // interface between ii and ii+1; update of point ii
+        const double ih = D == 0 ? P.hfb[ii] : lhfb, it = D == 0 ? P.hft[ii] : lhft;
+        const double iq = D == 0 ? P.qft[ii] : lqft;
+        bool zb = (D == 0) && ((P.is_gend0 && ii == sz - gh - 1) || (P.is_gbeg0 && ii == gh - 1));
+        double q = tvd_iface<D>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F);
+        if (ii >= gh && ii <= sz - gh - 1) {
+          long nd = base + ii * str;
+          double uu[MRM_NV];
 #pragma unroll
-      for (int k = 0; k < MRM_NV; k++) Fm[k] = F[k];
-      qm = q;
+          for (int k = 0; k < MRM_NV; k++) uu[k] = U[k * bs + nd];
+          tvd_update(P, W1, Fm, F, qm, q, iq, uu);
+          if (EPI) {
+            tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
+            double pn = ghost_pres_(X.E, uu);
+            X.Pnew[nd] = pn;
+            cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
+          }
+#pragma unroll
+          for (int k = 0; k < MRM_NV; k++) U[k * bs + nd] = uu[k];
+        }
+#pragma unroll
+        for (int k = 0; k < MRM_NV; k++) Fm[k] = F[k];
+        qm = q;
+      }
+      W0 = W1; W1 = W2; W2 = W3;
     }
-    W0 = W1; W1 = W2; W2 = W3;
+  }
+  if (EPI) {
+    block_reduce<OpMax, 1>(cm, lds);
+    if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
   }
 }

@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/muram_step.h"
-#include "mrm_kernels.h"
+#include "mrm_fused.h"
 
 struct mrm_ctx {
   int device = 0;
@@ -29,9 +29,15 @@ struct mrm_ctx {
   double dx[3];
   double dxmin = 0.0;
 
-  double* dU = nullptr;
-  double* dU0 = nullptr;
+  double* S[3] = {nullptr, nullptr, nullptr};   // state buffers (fused step rotates them)
+  int cur = 0;                                   // S[cur] is the current state U
+  double* dU = nullptr;                          // = S[cur]
+  double* dU0 = nullptr;                         // = S[(cur+1)%3]
   double* dR = nullptr;
+  double* dP2 = nullptr;                         // second pressure buffer of the fused step
+  bool fast_valid = false;                       // pres and cmax describe S[cur]
+  bool force_sequence = false;
+  bool stage_zwin = false;                       // stage kernel variant
   double* dF = nullptr;   // MRM_NFIELD arrays of bs doubles
   bool qtot_set = false, tau_set = false;
 
@@ -86,7 +92,7 @@ enum {
 static const char* const timer_names[MRM_NTIMER] = {
   "cons_to_prim", "valf_reduce", "residual", "cmax_faces", "source_integrate", "save_cons", "tcheck",
   "exchange", "bc_sums", "bc_bottom", "bc_top", "tvd_sweep_x", "tvd_sweep_y", "tvd_sweep_z",
-  "t14", "t15", "t16", "t17", "t18", "t19", "t20", "t21", "t22", "t23"};
+  "stage_fused", "eos_p", "t16", "t17", "t18", "t19", "t20", "t21", "t22", "t23"};
 
 // brackets the launches of one timer id with events on the context's stream
 struct TScope {
@@ -180,8 +186,11 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   int rc = MRM_OK;
   hipError_t e;
   if ((e = hipStreamCreate(&c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipStreamCreate", e);
-  if (!rc && (e = hipMalloc(&c->dU, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc U", e);
-  if (!rc && (e = hipMalloc(&c->dU0, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc U0", e);
+  for (int q = 0; q < 3 && !rc; q++)
+    if ((e = hipMalloc(&c->S[q], 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc state", e);
+  c->dU = c->S[0];
+  c->dU0 = c->S[1];
+  if (!rc && (e = hipMalloc(&c->dP2, bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc P2", e);
   if (!rc && (e = hipMalloc(&c->dR, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc Res", e);
   if (!rc && (e = hipMalloc(&c->dF, MRM_NFIELD * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc fields", e);
   c->npart = 8 * (long)nblk((long)n[1] * n[2]) + 64;
@@ -190,8 +199,13 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (!rc && (e = hipHostMalloc(&c->h_red, RED_NSLOT * sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipHostMalloc", e);
   if (!rc && (e = hipMalloc(&c->d_prof, 3 * n[0] * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc prof", e);
   // Res starts at zero (the reference relies on it, grid.C:279); all arrays defined
-  if (!rc && (e = hipMemsetAsync(c->dU, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
-  if (!rc && (e = hipMemsetAsync(c->dU0, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  for (int q = 0; q < 3 && !rc; q++)
+    if ((e = hipMemsetAsync(c->S[q], 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && (e = hipMemsetAsync(c->dP2, 0, bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  const char* path = getenv("MRM_STEP_PATH");
+  c->force_sequence = path && strcmp(path, "sequence") == 0;
+  const char* zw = getenv("MRM_STAGE_ZWIN");
+  c->stage_zwin = zw && zw[0] == '1';
   if (!rc && (e = hipMemsetAsync(c->dR, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipMemsetAsync(c->dF, 0, MRM_NFIELD * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipStreamSynchronize(c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "sync", e);
@@ -209,7 +223,8 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   c->harvest();
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
-  hipFree(c->dU); hipFree(c->dU0); hipFree(c->dR); hipFree(c->dF);
+  for (int q = 0; q < 3; q++) hipFree(c->S[q]);
+  hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
   if (c->h_red) hipHostFree(c->h_red);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -239,6 +254,7 @@ extern "C" int mrm_synchronize(mrm_ctx* c) {
 extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   if (!c || !bytes) return MRM_ERR_ARG;
   const float* buf = (const float*)bytes;
+  c->fast_valid = false;
   if (nbytes < 14 * sizeof(float)) return fail(c, MRM_ERR_EOS, "EOS table: short header");
   int N_eps = (int)buf[0], N_lr = (int)buf[1], N_lp3 = (int)buf[2], N_s3 = (int)buf[3];
   if (N_eps < 2 || N_lr < 2 || N_lp3 < 2 || N_s3 < 2) return fail(c, MRM_ERR_EOS, "EOS table: bad sizes");
@@ -305,6 +321,7 @@ extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layo
   if (!c || !host) return MRM_ERR_ARG;
   double* d = state_ptr(c, which);
   if (!d) return fail(c, MRM_ERR_ARG, "mrm_set_state: bad array id");
+  c->fast_valid = false;
   const size_t bs = c->G.bs;
   if (layout == MRM_LAYOUT_SOA) {
     HIPCHK(c, hipMemcpyAsync(d, host, 8 * bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -343,6 +360,7 @@ extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
 
 extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
   if (!c || !host || f < 0 || f >= MRM_NFIELD) return MRM_ERR_ARG;
+  c->fast_valid = false;
   const size_t bs = c->G.bs;
   HIPCHK(c, hipMemcpyAsync(c->dF + f * bs, host, bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -367,6 +385,7 @@ extern "C" int mrm_get_limits(const mrm_ctx* c, double lim[4]) {
 extern "C" int mrm_set_limits(mrm_ctx* c, const double lim[4]) {
   if (!c || !lim) return MRM_ERR_ARG;
   c->inv_va2max = lim[0]; c->v_lim = lim[1]; c->e_lim = lim[2]; c->va_max_old = lim[3];
+  c->fast_valid = false;
   return MRM_OK;
 }
 extern "C" int mrm_get_bc(const mrm_ctx* c, double* p_bc, double* s_bc) {
@@ -384,6 +403,7 @@ extern "C" int mrm_set_bc(mrm_ctx* c, double p_bc, double s_bc) {
 // ------------------------------------------------------------------ ConsToPrim
 extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  c->fast_valid = false;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_cons_to_prim: no EOS table loaded");
   unsigned nb = nblk(c->G.bs);
   if (nb > 65536u * 4) nb = 65536u * 4;
@@ -398,6 +418,7 @@ extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
 extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
   (void)run;
   if (!c) return MRM_ERR_ARG;
+  c->fast_valid = false;
   HIPCHK(c, hipMemsetAsync(c->d_red, 0, RED_NSLOT * sizeof(unsigned long long), c->stream));
   long n = ninterior(c);
   unsigned nb = nblk(n);
@@ -535,6 +556,7 @@ static SrcParams src_params(const mrm_ctx* c, const mrm_run_desc* run, int stage
 
 extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
+  c->fast_valid = false;
   SrcParams SP = src_params(c, run, stage);
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
   TScope ts(c, T_SRCINT);
@@ -545,6 +567,7 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
 
 extern "C" int mrm_save_cons(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  c->fast_valid = false;
   TScope ts(c, T_SAVECONS);
   k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
   LAUNCHCHK(c, "k_save_cons");
@@ -554,6 +577,7 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
 extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
   (void)run;
   if (!c) return MRM_ERR_ARG;
+  c->fast_valid = false;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
   TScope ts(c, T_TCHECK);
@@ -564,29 +588,35 @@ extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
 }
 
 // ------------------------------------------------------------------ exchange
-extern "C" int mrm_exchange_grid(mrm_ctx* c) {
-  if (!c) return MRM_ERR_ARG;
+// periodic ghosts of U (and of the pressure P when given) in dims 1 and 2
+static int exchange_buf(mrm_ctx* c, double* U, double* P) {
   const Geom& G = c->G;
   TScope ts(c, T_EXCH);
   long n1 = (long)c->lsize[0] * c->lsize[2] * G.g1;
-  k_exchange<1><<<nblk(n1), MRM_BLOCK, 0, c->stream>>>(G, c->dU);
-  LAUNCHCHK(c, "k_exchange<1>");
+  k_exchange_p<1><<<nblk(n1), MRM_BLOCK, 0, c->stream>>>(G, U, P);
+  LAUNCHCHK(c, "k_exchange_p<1>");
   long n2 = (long)c->lsize[0] * G.n1 * G.g2;
-  k_exchange<2><<<nblk(n2), MRM_BLOCK, 0, c->stream>>>(G, c->dU);
-  LAUNCHCHK(c, "k_exchange<2>");
+  k_exchange_p<2><<<nblk(n2), MRM_BLOCK, 0, c->stream>>>(G, U, P);
+  LAUNCHCHK(c, "k_exchange_p<2>");
   return MRM_OK;
 }
 
+extern "C" int mrm_exchange_grid(mrm_ctx* c) {
+  if (!c) return MRM_ERR_ARG;
+  c->fast_valid = false;
+  return exchange_buf(c, c->dU, nullptr);
+}
+
 // ------------------------------------------------------------------ boundary conditions
-static int bottom_sums(mrm_ctx* c, int mode, double* dst_dev) {
+static int bottom_sums(mrm_ctx* c, int mode, const double* U, double* dst_dev) {
   const long nyz = (long)c->lsize[1] * c->lsize[2];
   unsigned nb = nblk(nyz);
   TScope ts(c, T_BCSUM);
   if (mode == 0) {
-    k_bc_sums<0><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->d_part);
+    k_bc_sums<0><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, U, c->d_part);
     LAUNCHCHK(c, "k_bc_sums<0>");
   } else {
-    k_bc_sums<1><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->d_part);
+    k_bc_sums<1><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, U, c->d_part);
     LAUNCHCHK(c, "k_bc_sums<1>");
   }
   k_sum_partials<<<1, MRM_BLOCK, 0, c->stream>>>(c->d_part, (int)nb, mode == 0 ? 2 : 5, dst_dev);
@@ -595,9 +625,9 @@ static int bottom_sums(mrm_ctx* c, int mode, double* dst_dev) {
 }
 
 // set_p_s_bc (boundary_pdmp_1_fftw3.C:496-565), one rank
-static int set_p_s_bc(mrm_ctx* c) {
+static int set_p_s_bc(mrm_ctx* c, const double* U) {
   double* dsum = (double*)(c->d_red + RED_SUM0);
-  int rc = bottom_sums(c, 1, dsum);
+  int rc = bottom_sums(c, 1, U, dsum);
   if (rc) return rc;
   double h[5];
   HIPCHK(c, hipMemcpyAsync(h, dsum, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -613,21 +643,7 @@ static int set_p_s_bc(mrm_ctx* c) {
   return MRM_OK;
 }
 
-extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, int stage) {
-  (void)run; (void)stage;
-  if (!c) return MRM_ERR_ARG;
-  if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
-  int rc;
-  if (c->bnd_ini) {
-    if ((c->p_bc < 0.0) || (c->s_bc < 0.0)) {
-      rc = set_p_s_bc(c);
-      if (rc) return rc;
-    }
-    c->bnd_ini = 0;
-  }
-  double* dsum = (double*)(c->d_red + RED_SUM0);
-  rc = bottom_sums(c, 0, dsum);
-  if (rc) return rc;
+static BcParams bc_params(const mrm_ctx* c) {
   BcParams B;
   B.p_bc = c->p_bc;
   B.s_bc = c->s_bc;
@@ -635,16 +651,50 @@ extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, 
   B.eps_top = c->phys.bnd[3];
   B.cc = 1.0 / ((double)(c->grid.gsize[1] * c->grid.gsize[2]));
   B.open_top = (int)c->phys.bnd[0];
+  return B;
+}
+
+// SetBoundaryConditions on buffer U.  With P the ghost pressures are written
+// too (fused step); with cmax the dim-0 ghost faces enter the cmax reduction.
+static int bc_buf(mrm_ctx* c, double* U, double* P, unsigned long long* cmax) {
+  int rc;
+  if (c->bnd_ini) {
+    if ((c->p_bc < 0.0) || (c->s_bc < 0.0)) {
+      rc = set_p_s_bc(c, U);
+      if (rc) return rc;
+    }
+    c->bnd_ini = 0;
+  }
+  double* dsum = (double*)(c->d_red + RED_SUM0);
+  rc = bottom_sums(c, 0, U, dsum);
+  if (rc) return rc;
+  BcParams B = bc_params(c);
   const long nyz = (long)c->G.n1 * c->G.n2;
-  { TScope ts(c, T_BCBOT);
-  k_bc_bottom<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, c->dU);
+  if (P) {
+    TScope ts(c, T_BCBOT);
+    k_bc_fused<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, U, P, c->inv_va2max, cmax);
+    LAUNCHCHK(c, "k_bc_fused");
+  } else {
+    {
+      TScope ts(c, T_BCBOT);
+      k_bc_bottom<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, U);
+    }
+    LAUNCHCHK(c, "k_bc_bottom");
+    {
+      TScope ts(c, T_BCTOP);
+      k_bc_top<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, B, U);
+    }
+    LAUNCHCHK(c, "k_bc_top");
   }
-  LAUNCHCHK(c, "k_bc_bottom");
-  { TScope ts(c, T_BCTOP);
-  k_bc_top<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, B, c->dU);
-  }
-  LAUNCHCHK(c, "k_bc_top");
   return MRM_OK;
+}
+
+extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  (void)run; (void)stage;
+  if (!c) return MRM_ERR_ARG;
+  if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
+  c->fast_valid = false;
+  return bc_buf(c, c->dU, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------------ TVD
@@ -667,10 +717,8 @@ static int tvd_profiles(mrm_ctx* c, double h_bnd_bot, double h_bnd_top, double Q
   return MRM_OK;
 }
 
-extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
-  if (!c || !run) return MRM_ERR_ARG;
+static int tvd_params(mrm_ctx* c, double dt_tvd, TvdParams& P) {
   const mrm_phys_desc& Ph = c->phys;
-  TvdParams P;
   P.rho_min = Ph.tchk[1];
   P.eps_min = Ph.tchk[0];
   P.rho_lev = pow(Ph.tvd[0], 2);
@@ -708,69 +756,203 @@ extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) 
   P.hft = c->d_prof + n0;
   P.qft = c->d_prof + 2 * n0;
   P.tau = c->tau_set ? c->dF + MRM_TAU * c->G.bs : nullptr;
-  const double* Pr = c->dF + MRM_PRES * c->G.bs;
+  return MRM_OK;
+}
+
+static size_t tvd_x_lds(const mrm_ctx* c) { return (size_t)(3 * MRM_NV + 5) * c->G.n0 * sizeof(double); }
+
+// the three sweeps of TVDlimit on buffer U with pressure Pr; with epi the
+// dim-2 sweep finishes each cell (TCheck, new pressure, cmax)
+static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi) {
+  TvdParams P;
+  int rc = tvd_params(c, dt_tvd, P);
+  if (rc) return rc;
   const Geom& G = c->G;
   const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
+  const bool tau = P.tau != nullptr;
+  TvdEpilogue X{};
   for (int d = 0; d < 3; d++) {
     P.cmax = 0.975 * c->dx[d] / dt_tvd;
     P.idxd1 = idx[d];
     long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
     unsigned nb = nblk(nl);
-    const bool tau = P.tau != nullptr;
     TScope ts(c, T_TVD0 + d);
     if (d == 0) {
-      if (tau) k_tvd_sweep<0, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
-      else k_tvd_sweep<0, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+      size_t lds = tvd_x_lds(c);
+      if (G.n0 <= TX_MAXN && lds <= 64 * 1024) {
+        TvdLineArgs A{G, P};
+        unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
+        k_tvd_x_lines<<<(unsigned)nl, bsz, lds, c->stream>>>(A, U, Pr);
+      } else if (tau) {
+        k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
+      } else {
+        k_tvd_sweep<0, false, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
+      }
     } else if (d == 1) {
-      if (tau) k_tvd_sweep<1, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
-      else k_tvd_sweep<1, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+      if (tau) k_tvd_sweep<1, true, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
+      else k_tvd_sweep<1, false, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
+    } else if (epi) {
+      if (tau) k_tvd_sweep<2, true, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, *epi);
+      else k_tvd_sweep<2, false, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, *epi);
     } else {
-      if (tau) k_tvd_sweep<2, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
-      else k_tvd_sweep<2, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, c->dU, Pr);
+      if (tau) k_tvd_sweep<2, true, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
+      else k_tvd_sweep<2, false, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
     }
-    LAUNCHCHK(c, "k_tvd_sweep");
+    LAUNCHCHK(c, "tvd sweep");
   }
   return MRM_OK;
 }
 
+extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
+  if (!c || !run) return MRM_ERR_ARG;
+  c->fast_valid = false;
+  return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, dt_tvd, nullptr);
+}
+
 // ------------------------------------------------------------------ full step
+// the reference's call sequence, one drop-in operation after the other
+static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
+  int rc;
+#define STEPCHK(x) do { rc = (x); if (rc) return rc; } while (0)
+  double cfl = 0.0;
+  STEPCHK(mrm_save_cons(c));
+  for (int stage = 1; stage <= 4; stage++) {
+    STEPCHK(mrm_cons_to_prim(c));
+    if (stage == 1) STEPCHK(mrm_adjust_valf_max(c, run, nullptr));
+    if (stage == 1) {
+      double dt_mhd;
+      STEPCHK(mrm_mhd_residual(c, run, &dt_mhd));
+      STEPCHK(mrm_sync_timestep(c, run, dt_rad, dt_mhd, &cfl));
+    } else {
+      STEPCHK(launch_residual(c));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
+    }
+    STEPCHK(mrm_source_integrate_tcheck(c, run, stage));
+    STEPCHK(mrm_exchange_grid(c));
+    STEPCHK(mrm_set_boundary_conditions(c, run, stage));
+  }
+  if (cfl <= run->CFL_tvd) {
+    STEPCHK(mrm_tvdlimit(c, run, run->dt));
+  } else {
+    STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
+    STEPCHK(mrm_exchange_grid(c));
+    STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
+  }
+  STEPCHK(mrm_tcheck(c, run));
+  STEPCHK(mrm_exchange_grid(c));
+  STEPCHK(mrm_set_boundary_conditions(c, run, 5));
+#undef STEPCHK
+  return MRM_OK;
+}
+
+// Adjust_Valf_Max with va_adjust = 0 (Adjust_Valf_Max.C:136-144): its
+// reductions only feed the va_adjust branch, the limits are constants
+static void valf_constants(mrm_ctx* c) {
+  double va_max = c->phys.params[1];
+  c->v_lim = c->phys.tchk[3];
+  c->e_lim = c->phys.tchk[2];
+  c->inv_va2max = 1.0 / pow(va_max, 2);
+  c->va_max_old = va_max;
+}
+
+static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const double* Uin, const double* Pin,
+                        const double* U0, double* Uout, double* Pout) {
+  StageArgs A;
+  A.G = c->G;
+  A.E = c->E;
+  for (int d = 0; d < 3; d++) {
+    A.RP.w1[d] = 8. / (12. * c->dx[d]);
+    A.RP.w2[d] = -1. / (12. * c->dx[d]);
+  }
+  A.RP.inv_va2max = c->inv_va2max;
+  A.SP = src_params(c, run, stage);
+  A.ntx = (c->lsize[0] + ST_SX - 1) / ST_SX;
+  A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
+  A.nkc = (c->lsize[2] + ST_KC - 1) / ST_KC;
+  A.write_p = Pout != nullptr;
+  const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
+  unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
+  TScope ts(c, T_NAMED);
+  if (c->stage_zwin) k_stage<true><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  else k_stage<false><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  LAUNCHCHK(c, "k_stage");
+  return MRM_OK;
+}
+
+// One time step as device-resident fused kernels: four stage kernels with
+// ghost fills between them, three TVD sweeps (the last one finishing each
+// cell), and the final ghost fill.  Pressures travel with the state, so
+// ConsToPrim never runs as a separate pass; the only host round trip is the
+// dt reduction (solver.C:487-548 needs it on the host).
+static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
+  const Geom& G = c->G;
+  double* Pc = c->dF + MRM_PRES * G.bs;
+  double* Pa = c->dP2;
+  unsigned long long* cmax = c->d_red + RED_CMAX;
+  int rc;
+  valf_constants(c);
+  if (!c->fast_valid) {
+    HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+    unsigned nb = nblk(G.bs);
+    if (nb > 65536u) nb = 65536u;
+    TScope ts(c, T_NAMED + 1);
+    k_eos_p<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->E, c->S[c->cur], Pc, c->inv_va2max, cmax);
+    LAUNCHCHK(c, "k_eos_p");
+  }
+  double cm;
+  if ((rc = read_cmax(c, &cm))) return rc;
+  double cfl = 0.0;
+  if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
+  const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
+  double* U0 = c->S[i0];
+  struct { double *in, *pin, *out, *pout; } st[4] = {
+      {c->S[i0], Pc, c->S[i1], Pa}, {c->S[i1], Pa, c->S[i2], Pc},
+      {c->S[i2], Pc, c->S[i1], Pa}, {c->S[i1], Pa, c->S[i2], nullptr}};
+  for (int s = 0; s < 4; s++) {
+    if ((rc = stage_launch(c, run, s + 1, st[s].in, st[s].pin, U0, st[s].out, st[s].pout))) return rc;
+    if ((rc = exchange_buf(c, st[s].out, st[s].pout))) return rc;
+    if ((rc = bc_buf(c, st[s].out, st[s].pout, nullptr))) return rc;
+  }
+  // TVD on U4 with the pressure of the stage-4 input (tvdlimit_SR.C:332 reads
+  // Grid.pres of the last ConsToPrim)
+  double* Uf = c->S[i2];
+  HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+  TvdEpilogue X;
+  X.E = c->E;
+  X.rho_min = c->phys.tchk[1];
+  X.eps_min = c->phys.tchk[0];
+  X.vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
+  X.eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  X.inv_va2max = c->inv_va2max;
+  X.Pnew = Pc;
+  X.cmax = cmax;
+  if (cfl <= run->CFL_tvd) {
+    if ((rc = tvd_buf(c, Uf, Pa, run->dt, &X))) return rc;
+  } else {
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, nullptr))) return rc;
+    if ((rc = exchange_buf(c, Uf, nullptr))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, &X))) return rc;
+  }
+  if ((rc = exchange_buf(c, Uf, Pc))) return rc;
+  if ((rc = bc_buf(c, Uf, Pc, cmax))) return rc;
+  c->cur = i2;
+  c->dU = c->S[i2];
+  c->dU0 = c->S[(i2 + 1) % 3];
+  c->fast_valid = true;
+  return MRM_OK;
+}
+
 extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
   if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
-  int rc;
-#define STEPCHK(x) do { rc = (x); if (rc) return rc; } while (0)
+  if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
+  const bool fused = !c->force_sequence && !(c->phys.params[2] > 0.0);
   for (int it = 0; it < nsteps; it++) {
-    double cfl = 0.0;
-    STEPCHK(mrm_save_cons(c));
-    for (int stage = 1; stage <= 4; stage++) {
-      STEPCHK(mrm_cons_to_prim(c));
-      if (stage == 1) STEPCHK(mrm_adjust_valf_max(c, run, nullptr));
-      if (stage == 1) {
-        double dt_mhd;
-        STEPCHK(mrm_mhd_residual(c, run, &dt_mhd));
-        STEPCHK(mrm_sync_timestep(c, run, dt_rad, dt_mhd, &cfl));
-      } else {
-        STEPCHK(launch_residual(c));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
-      }
-      STEPCHK(mrm_source_integrate_tcheck(c, run, stage));
-      STEPCHK(mrm_exchange_grid(c));
-      STEPCHK(mrm_set_boundary_conditions(c, run, stage));
-    }
-    if (cfl <= run->CFL_tvd) {
-      STEPCHK(mrm_tvdlimit(c, run, run->dt));
-    } else {
-      STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
-      STEPCHK(mrm_exchange_grid(c));
-      STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
-    }
-    STEPCHK(mrm_tcheck(c, run));
-    STEPCHK(mrm_exchange_grid(c));
-    STEPCHK(mrm_set_boundary_conditions(c, run, 5));
+    int rc = fused ? step_fused(c, run, dt_rad) : step_sequence(c, run, dt_rad);
+    if (rc) return rc;
     run->iteration++;
     run->globiter++;
     run->time += run->dt;
   }
-#undef STEPCHK
   return MRM_OK;
 }
 

@@ -37,8 +37,17 @@ def maxrel(a, b):
     return max(relerr(a[v], b[v]) for v in range(a.shape[0]))
 
 
-def make(grid, phys, table):
-    m = MhdGrid(grid, phys)
+def make(grid, phys, table, path=None):
+    old = os.environ.get("MRM_STEP_PATH")
+    if path:
+        os.environ["MRM_STEP_PATH"] = path
+    try:
+        m = MhdGrid(grid, phys)
+    finally:
+        if old is None:
+            os.environ.pop("MRM_STEP_PATH", None)
+        else:
+            os.environ["MRM_STEP_PATH"] = old
     m.eos_init(table)
     return m
 
@@ -90,12 +99,13 @@ def test_cons_to_prim_fallbacks(gold, table):
         assert ok.all(), (f, np.argwhere(~ok)[:5])
 
 
+@pytest.mark.parametrize("path", ["fused", "sequence"])
 @pytest.mark.parametrize("case", ["base", "split", "vadj"])
-def test_full_steps_match_reference(gold, table, case):
-    print(f"case: full steps {case}", flush=True)
+def test_full_steps_match_reference(gold, table, case, path):
+    print(f"case: full steps {case} {path}", flush=True)
     vadj = case == "vadj"
     p = config.stratified_phys(va_max=3.0e7, va_adjust=1.0) if vadj else config.stratified_phys()
-    m = make(small_grid(), p, table)
+    m = make(small_grid(), p, table, path)
     r = config.run_desc(CFL=float(gold[f"{case}_cfl"]))
     m.set_state(gold[f"{case}_in"])
     m.exchange_grid()
