@@ -85,10 +85,15 @@ def main():
     ap.add_argument("--table-n", type=int, default=512)
     ap.add_argument("--path", default="fused", choices=["fused", "sequence"])
     ap.add_argument("--stage-zwin", default=None, choices=["0", "1"])
+    ap.add_argument("--fma", default=None, choices=["0", "1"])
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run a 512 MiB device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     os.environ["MRM_STEP_PATH"] = args.path
     if args.stage_zwin is not None:
         os.environ["MRM_STAGE_ZWIN"] = args.stage_zwin
+    if args.fma is not None:
+        os.environ["MRM_FMA"] = args.fma
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -100,6 +105,11 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
+    if args.calibrate:
+        x = torch.ones(1 << 26, dtype=torch.float64, device="cuda")
+        y = x.clone()
+        torch.cuda.synchronize()
+        del x, y
     wl = config.workloads()[args.workload]
     box = wl.box
     t = time.perf_counter()
@@ -166,7 +176,8 @@ def main():
         "data": "synthetic (seeded stratified box, generated EOS table)",
         "config": {"workload": wl.name, "gsize": list(box.gsize), "dx_cm": box.dx[0],
                    "precision": args.precision, "steps_timed": args.steps, "dt_s": r.dt,
-                   "path": args.path, "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "0")},
+                   "path": args.path, "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "1"),
+                   "fma": os.environ.get("MRM_FMA", "1")},
         "hbm_frac_model": round(value * 1e6 * MODEL_BYTES[args.precision] / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": roof,
         "kernels": {k: {"mean_ms": round(v["mean_ms"], 4), "launches": v["launches"],

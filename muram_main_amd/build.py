@@ -11,26 +11,39 @@ OUT = os.path.join(HERE, "libmuram_mi355x.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # plain IEEE arithmetic in the reference's operation order: no FMA contraction
-    "-ffp-contract=off", "-fno-fast-math",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fno-fast-math",
     # packed FP32 instructions stay off (see the design notes)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
 
 
-def build(verbose: bool = False) -> str:
-    src = os.path.join(CSRC, "mrm_step.hip")
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-I", CSRC, "-o", tmp, src]
+# translation units and their contraction mode: the drop-in operations keep
+# plain IEEE arithmetic in the reference's operation order (no FMA); the
+# fused-step kernels are also built with contraction in mrm_fast.hip
+UNITS = [("mrm_step.hip", "-ffp-contract=off"), ("mrm_fast.hip", "-ffp-contract=fast")]
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed for libmuram_mi355x.so")
+
+
+def build(verbose: bool = False) -> str:
+    objs = []
+    for src, contract in UNITS:
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        _run([HIPCC] + FLAGS + [contract, "-I", CSRC, "-c", "-o", obj, os.path.join(CSRC, src)], verbose)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, OUT)
+    for o in objs:
+        os.remove(o)
     return OUT
 
 

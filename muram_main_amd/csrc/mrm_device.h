@@ -13,29 +13,16 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "mrm_types.h"
 
-#define MRM_NV 8
+#ifndef MRM_NS
+#define MRM_NS exact
+#endif
 
-struct Geom {
-  int n0, n1, n2;          // buffer extents (gsize + 2*ghosts)
-  int g0, g1, g2;          // ghosts
-  int lb0, lb1, lb2;       // lbeg
-  int le0, le1, le2;       // lend
-  long s1, s2;             // strides of dims 1, 2 (stride of dim 0 is 1)
-  long bs;                 // bufsize
-};
+namespace MRM_NS {
 
-struct EosDev {
-  int N_eps, N_lr, N_lp3, N_s3;
-  double inv_del_eps, inv_del_lr, inv_del_lp3, inv_del_s3;
-  double eps_off, ss_off, eps_max;
-  const double* xeps;
-  const double* xlr;
-  const double* xlp3;
-  const double* xs3;
-  const float* tab;    // [6][N_eps][N_lr]: ln p, ln T, s, ln ne, ln rhoi, ln amb (pre-divided)
-  const float* itab;   // [2][N_lp3][N_s3]: ln(eps+eps_off), ln rho (pre-divided)
-};
+
+
 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
@@ -136,6 +123,7 @@ __device__ __forceinline__ void cons_to_prim_cell(const EosDev& E, double d, dou
   const long o01 = i_e * L + i_d + 1, o00 = i_e * L + i_d;
   const int tabs[5] = {0, 1, 3, 4, 5};
   double r[5];
+  double tt = 0.0;
 #pragma unroll
   for (int q = 0; q < NOUT; q++) {
     const float* T = E.tab + tabs[q] * NT;
@@ -143,10 +131,26 @@ __device__ __forceinline__ void cons_to_prim_cell(const EosDev& E, double d, dou
     v += coeff1 * (double)T[o10];
     v += coeff2 * (double)T[o01];
     v += coeff3 * (double)T[o00];
-    r[q] = exp(v);
+    if (NOUT == 2 && q == 1) {
+      tt = v;
+      r[q] = 0.0;
+    } else {
+      r[q] = exp(v);
+    }
   }
   double pres = r[0];
   double temp = r[1];
+  if (NOUT == 2) {
+    // The temperature only decides whether p is valid (cons_to_prim.C:195).
+    // exp(tt) is finite and positive for |tt| < 700, so it is evaluated only
+    // when a fallback branch uses it or tt lies outside that range.
+    const bool plain = !(lgr < E.xlr[0]) && !(eps > 0.8 * E.eps_max);
+    if (plain && tt > -700.0 && tt < 700.0) {
+      temp = 1.0;
+    } else {
+      temp = exp(tt);
+    }
+  }
   double nel = NOUT > 2 ? r[2] : 0.0;
   if (lgr < E.xlr[0]) {
     pres = c_pres * e;
@@ -296,7 +300,7 @@ __device__ __forceinline__ void sweep_residual(const double v[5][MRM_NV], const 
 // one sweep_residual (and mhdres_SR.C:361-412) performs.
 template <int D, class GET>
 __device__ __forceinline__ void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
-                                                  double inv_va2max, double res[MRM_NV]) {
+                                                  double lf, double res[MRM_NV]) {
   constexpr int D2 = D == 0 ? 1 : 0;
   constexpr int D3 = D == 2 ? 1 : 2;
   double d1[13], d2[13];   // 8 fluxes, 3 stresses, B_D2, B_D3
@@ -342,7 +346,6 @@ __device__ __forceinline__ void sweep_residual_lr(GET get, const double c[9], do
   double lrt0 = w1 * d1[8] + w2 * d2[8];
   double lrt1 = w1 * d1[9] + w2 * d2[9];
   double lrt2 = w1 * d1[10] + w2 * d2[10];
-  double lf = lfac_of(c, inv_va2max);
   lrt0 = lf * lrt0 + (1.0 - lf) * cb[0];
   lrt1 = lf * lrt1 + (1.0 - lf) * cb[1];
   lrt2 = lf * lrt2 + (1.0 - lf) * cb[2];
@@ -353,10 +356,6 @@ __device__ __forceinline__ void sweep_residual_lr(GET get, const double c[9], do
 }
 
 // ------------------------------------------------------------------ sources
-struct SrcParams {
-  double wdt, invRdt, g, inv_va2max;
-  double rho_min, eps_min, eps_max, vmax;
-};
 
 // Tcheck clamp (Add_Sources_Integrate_SR.C:447-465)
 __device__ __forceinline__ void tcheck_vals(double u[MRM_NV], double rho_min, double eps_min, double eps_max,
@@ -419,19 +418,6 @@ __device__ __forceinline__ void source_integrate_cell(const SrcParams& P, const 
 // evaluates point t, the slope at t-1, the interface flux at t-2 and updates
 // point t-2 in place -- the same per-point, per-slope and per-interface
 // arithmetic as the reference's phase loops over whole lines.
-struct TvdParams {
-  double rho_min, eps_min, rho_lev, q_rho_max;
-  double visc_coeff_bot, visc_slope_bot, visc_coeff_top, visc_slope_top;
-  double eta_coeff_bot, eta_slope_bot, eta_coeff_top, eta_slope_top;
-  double B_par_diff, vhyp, tvd_pm_v, tvd_pm_B, vmax_lim, CME_thresh;
-  double tvd_h[4], tvd_cs[4], tc[4];
-  double inv_va2max, vsqr_diff, cmax, idxd1, dt_tvd;
-  int need_tvd_coeff, rho_log, is_gbeg0, is_gend0;
-  const double* hfb;
-  const double* hft;
-  const double* qft;
-  const double* tau;   // NULL: Tau == 0 everywhere
-};
 
 struct TvdPt {
   double v[MRM_NV];   // primitives; ln rho and ln eps when rho_log
@@ -602,3 +588,5 @@ __device__ __forceinline__ void tvd_update(const TvdParams& P, const TvdPt& b, c
   u[0] = smax(P.rho_min, u[0]);
   u[4] = smax(u[4], P.eps_min * u[0] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]);
 }
+
+}  // namespace MRM_NS

@@ -15,21 +15,9 @@
 #pragma once
 #include "mrm_kernels.h"
 
-#define ST_SX 16
-#define ST_SY 16
-#define ST_KC 32
-#define ST_LX (ST_SX + 4)
-#define ST_LY (ST_SY + 4)
-#define ST_NF 9            // rho, v0, v1, v2, e_int, B0, B1, B2, p
+namespace MRM_NS {
 
-struct StageArgs {
-  Geom G;
-  EosDev E;
-  ResidParams RP;
-  SrcParams SP;
-  int ntx, nty, nkc;       // tiles in dim 0, dim 1, chunks in dim 2
-  int write_p;             // compute the EOS pressure of the new state
-};
+
 
 // conservative point -> primitive (ConsToPrim's arithmetic, cons_to_prim.C:110-118)
 __device__ __forceinline__ void load_prim(const double* __restrict__ U, const double* __restrict__ P, long bs,
@@ -139,18 +127,19 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
       double acc[MRM_NV], res[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] = 0.0;
+      const double lfc = lfac_of(w[2], A.RP.inv_va2max);
       auto getx = [&](int s, double o[9]) {
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + 2][tx + s];
       };
-      sweep_residual_lr<0>(getx, w[2], A.RP.w1[0], A.RP.w2[0], A.RP.inv_va2max, res);
+      sweep_residual_lr<0>(getx, w[2], A.RP.w1[0], A.RP.w2[0], lfc, res);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
       auto gety = [&](int s, double o[9]) {
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + s][tx + 2];
       };
-      sweep_residual_lr<1>(gety, w[2], A.RP.w1[1], A.RP.w2[1], A.RP.inv_va2max, res);
+      sweep_residual_lr<1>(gety, w[2], A.RP.w1[1], A.RP.w2[1], lfc, res);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
       auto getz = [&](int s, double o[9]) {
@@ -161,7 +150,7 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
           load_prim(Uin, Pin, bs, col + (long)(k + s - 2) * G.s2, o);
         }
       };
-      sweep_residual_lr<2>(getz, w[2], A.RP.w1[2], A.RP.w2[2], A.RP.inv_va2max, res);
+      sweep_residual_lr<2>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
       // sources, Boris, RK update, Tcheck
@@ -352,11 +341,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p(Geom G, EosDev E, const dou
 // ------------------------------------------------------------------ TVD, dim-0 sweep, one block per line
 // thread t owns point t of the line; the per-point, per-slope, per-interface
 // and per-update arithmetic is tvd_point / tvd_slope / tvd_iface / tvd_update
-#define TX_MAXN 1024
-struct TvdLineArgs {
-  Geom G;
-  TvdParams P;
-};
 
 __global__ void __launch_bounds__(TX_MAXN) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
                                                         const double* __restrict__ Pr) {
@@ -437,3 +421,5 @@ __global__ void __launch_bounds__(TX_MAXN) k_tvd_x_lines(TvdLineArgs A, double* 
     for (int q = 0; q < MRM_NV; q++) U[q * bs + base + t] = u[q];
   }
 }
+
+}  // namespace MRM_NS

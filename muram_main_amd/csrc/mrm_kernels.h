@@ -4,16 +4,9 @@
 #pragma once
 #include "mrm_device.h"
 
-#define RED_CMAX 0      // uint64 bits of the max characteristic speed
-#define RED_VMAX 1      // uint64 bits of max |v|
-#define RED_CS2MAX 2    // uint64 bits of max 5/3 p/rho
-#define RED_EMAX 3      // uint64 bits of max eps
-#define RED_CNT_V 4     // uint64 count |v| >= 0.95 v_lim
-#define RED_CNT_E 5     // uint64 count eps >= 0.95 e_lim
-#define RED_SUM0 8      // double sums written by k_sum_partials
-#define RED_NSLOT 16
+namespace MRM_NS {
 
-#define MRM_BLOCK 256
+
 
 // ---------------------------------------------------------------- helpers
 // non-negative doubles order like their bit patterns: max via integer atomics
@@ -114,10 +107,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_valf_reduce(Geom G, const double*
 }
 
 // ---------------------------------------------------------------- MHD_Residual
-struct ResidParams {
-  double w1[3], w2[3];
-  double inv_va2max;
-};
 
 __device__ __forceinline__ void load_pt(const double* __restrict__ U, const double* __restrict__ P, long bs,
                                         long node, double v[MRM_NV], double& p) {
@@ -342,10 +331,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_sum_partials(const double* __rest
   }
 }
 
-struct BcParams {
-  double p_bc, s_bc, B_crit, eps_top, cc;
-  int open_top;
-};
 
 // bottom ghost layers (boundary_pdmp_1_fftw3.C:240-318), all (j,k) incl. ghosts;
 // psum[0..1] are the layer sums of p written by k_sum_partials
@@ -431,12 +416,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double
 // (Add_Sources_Integrate_SR.C:591-624; every interior cell is finished by the
 // dim-2 sweep), writes its EOS pressure and reduces its characteristic speed,
 // which is what the next step's ConsToPrim and MHD_Residual compute from it.
-struct TvdEpilogue {
-  EosDev E;
-  double rho_min, eps_min, eps_max, vmax, inv_va2max;
-  double* Pnew;
-  unsigned long long* cmax;
-};
 
 __device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
   double m1 = u[1], m2 = u[2], m3 = u[3], e = u[4], pr[5];
@@ -523,3 +502,5 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
     if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
   }
 }
+
+}  // namespace MRM_NS

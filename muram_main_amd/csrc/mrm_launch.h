@@ -1,0 +1,43 @@
+// mrm_launch.h -- host-side launchers of the fused-step kernels, compiled in
+// each translation unit's kernel namespace (exact: no FMA contraction;
+// fast: contraction allowed) so the host code can pick either build.
+#pragma once
+#include "mrm_fused.h"
+
+namespace MRM_NS {
+
+inline hipError_t launch_stage(hipStream_t st, unsigned nb, bool zwin, const StageArgs& A, const double* Uin,
+                               const double* Pin, const double* U0, double* Uout, double* Pout,
+                               const double* Q) {
+  if (zwin) k_stage<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  else k_stage<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  return hipGetLastError();
+}
+
+inline hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const TvdLineArgs& A,
+                                     double* U, const double* Pr) {
+  k_tvd_x_lines<<<nl, bsz, lds, st>>>(A, U, Pr);
+  return hipGetLastError();
+}
+
+// marching sweep along dim d (0, 1, 2); epi applies the finishing epilogue (d = 2)
+inline hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const TvdEpilogue* epi, unsigned nb,
+                                   const Geom& G, const TvdParams& P, double* U, const double* Pr) {
+  TvdEpilogue X{};
+  if (d == 0) {
+    if (tau) k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+    else k_tvd_sweep<0, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+  } else if (d == 1) {
+    if (tau) k_tvd_sweep<1, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+    else k_tvd_sweep<1, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+  } else if (epi) {
+    if (tau) k_tvd_sweep<2, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
+    else k_tvd_sweep<2, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
+  } else {
+    if (tau) k_tvd_sweep<2, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+    else k_tvd_sweep<2, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace MRM_NS

@@ -15,7 +15,19 @@
 #include <vector>
 
 #include "../../include/muram_step.h"
-#include "mrm_fused.h"
+#include "mrm_launch.h"
+
+using namespace exact;
+
+// the contracted build of the fused-step kernels (mrm_fast.hip)
+namespace fast {
+extern hipError_t (*p_launch_stage)(hipStream_t, unsigned, bool, const StageArgs&, const double*, const double*,
+                                    const double*, double*, double*, const double*);
+extern hipError_t (*p_launch_tvd_x_lines)(hipStream_t, unsigned, unsigned, size_t, const TvdLineArgs&, double*,
+                                          const double*);
+extern hipError_t (*p_launch_tvd_sweep)(hipStream_t, int, bool, const TvdEpilogue*, unsigned, const Geom&,
+                                        const TvdParams&, double*, const double*);
+}  // namespace fast
 
 struct mrm_ctx {
   int device = 0;
@@ -38,6 +50,7 @@ struct mrm_ctx {
   bool fast_valid = false;                       // pres and cmax describe S[cur]
   bool force_sequence = false;
   bool stage_zwin = false;                       // stage kernel variant
+  bool fma = true;                               // fused step uses the contracted build
   double* dF = nullptr;   // MRM_NFIELD arrays of bs doubles
   bool qtot_set = false, tau_set = false;
 
@@ -205,7 +218,9 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   const char* path = getenv("MRM_STEP_PATH");
   c->force_sequence = path && strcmp(path, "sequence") == 0;
   const char* zw = getenv("MRM_STAGE_ZWIN");
-  c->stage_zwin = zw && zw[0] == '1';
+  c->stage_zwin = !(zw && zw[0] == '0');
+  const char* fm = getenv("MRM_FMA");
+  c->fma = !(fm && fm[0] == '0');
   if (!rc && (e = hipMemsetAsync(c->dR, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipMemsetAsync(c->dF, 0, MRM_NFIELD * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipStreamSynchronize(c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "sync", e);
@@ -763,42 +778,33 @@ static size_t tvd_x_lds(const mrm_ctx* c) { return (size_t)(3 * MRM_NV + 5) * c-
 
 // the three sweeps of TVDlimit on buffer U with pressure Pr; with epi the
 // dim-2 sweep finishes each cell (TCheck, new pressure, cmax)
-static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi) {
+static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi,
+                   bool contracted = false) {
   TvdParams P;
   int rc = tvd_params(c, dt_tvd, P);
   if (rc) return rc;
   const Geom& G = c->G;
   const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
   const bool tau = P.tau != nullptr;
-  TvdEpilogue X{};
   for (int d = 0; d < 3; d++) {
     P.cmax = 0.975 * c->dx[d] / dt_tvd;
     P.idxd1 = idx[d];
     long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
     unsigned nb = nblk(nl);
     TScope ts(c, T_TVD0 + d);
-    if (d == 0) {
-      size_t lds = tvd_x_lds(c);
-      if (G.n0 <= TX_MAXN && lds <= 64 * 1024) {
-        TvdLineArgs A{G, P};
-        unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
-        k_tvd_x_lines<<<(unsigned)nl, bsz, lds, c->stream>>>(A, U, Pr);
-      } else if (tau) {
-        k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
-      } else {
-        k_tvd_sweep<0, false, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
-      }
-    } else if (d == 1) {
-      if (tau) k_tvd_sweep<1, true, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
-      else k_tvd_sweep<1, false, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
-    } else if (epi) {
-      if (tau) k_tvd_sweep<2, true, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, *epi);
-      else k_tvd_sweep<2, false, true><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, *epi);
+    hipError_t e;
+    size_t lds = tvd_x_lds(c);
+    if (d == 0 && G.n0 <= TX_MAXN && lds <= 64 * 1024) {
+      TvdLineArgs A{G, P};
+      unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
+      e = contracted ? fast::p_launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, A, U, Pr)
+                     : exact::launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, A, U, Pr);
     } else {
-      if (tau) k_tvd_sweep<2, true, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
-      else k_tvd_sweep<2, false, false><<<nb, MRM_BLOCK, 0, c->stream>>>(G, P, U, Pr, X);
+      const TvdEpilogue* x = d == 2 ? epi : nullptr;
+      e = contracted ? fast::p_launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr)
+                     : exact::launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr);
     }
-    LAUNCHCHK(c, "tvd sweep");
+    if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep", e);
   }
   return MRM_OK;
 }
@@ -872,9 +878,9 @@ static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const do
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
   unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
   TScope ts(c, T_NAMED);
-  if (c->stage_zwin) k_stage<true><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-  else k_stage<false><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-  LAUNCHCHK(c, "k_stage");
+  hipError_t e = c->fma ? fast::p_launch_stage(c->stream, nb, c->stage_zwin, A, Uin, Pin, U0, Uout, Pout, Q)
+                        : exact::launch_stage(c->stream, nb, c->stage_zwin, A, Uin, Pin, U0, Uout, Pout, Q);
+  if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "k_stage", e);
   return MRM_OK;
 }
 
@@ -926,11 +932,11 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.Pnew = Pc;
   X.cmax = cmax;
   if (cfl <= run->CFL_tvd) {
-    if ((rc = tvd_buf(c, Uf, Pa, run->dt, &X))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, run->dt, &X, c->fma))) return rc;
   } else {
-    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, nullptr))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, nullptr, c->fma))) return rc;
     if ((rc = exchange_buf(c, Uf, nullptr))) return rc;
-    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, &X))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, &X, c->fma))) return rc;
   }
   if ((rc = exchange_buf(c, Uf, Pc))) return rc;
   if ((rc = bc_buf(c, Uf, Pc, cmax))) return rc;

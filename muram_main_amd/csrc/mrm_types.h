@@ -1,0 +1,92 @@
+// mrm_types.h -- parameter blocks shared by the host code and the kernels of
+// the MHD step (plain structs passed by value to kernels).
+#pragma once
+
+#define MRM_NV 8
+#define RED_CMAX 0
+#define RED_VMAX 1
+#define RED_CS2MAX 2
+#define RED_EMAX 3
+#define RED_CNT_V 4
+#define RED_CNT_E 5
+#define RED_SUM0 8
+#define RED_NSLOT 16
+#define MRM_BLOCK 256
+#define ST_SX 16
+#define ST_SY 16
+#define ST_KC 32
+#define ST_LX (ST_SX + 4)
+#define ST_LY (ST_SY + 4)
+#define ST_NF 9            // rho, v0, v1, v2, e_int, B0, B1, B2, p
+#define TX_MAXN 1024
+
+struct Geom {
+  int n0, n1, n2;          // buffer extents (gsize + 2*ghosts)
+  int g0, g1, g2;          // ghosts
+  int lb0, lb1, lb2;       // lbeg
+  int le0, le1, le2;       // lend
+  long s1, s2;             // strides of dims 1, 2 (stride of dim 0 is 1)
+  long bs;                 // bufsize
+};
+
+struct EosDev {
+  int N_eps, N_lr, N_lp3, N_s3;
+  double inv_del_eps, inv_del_lr, inv_del_lp3, inv_del_s3;
+  double eps_off, ss_off, eps_max;
+  const double* xeps;
+  const double* xlr;
+  const double* xlp3;
+  const double* xs3;
+  const float* tab;    // [6][N_eps][N_lr]: ln p, ln T, s, ln ne, ln rhoi, ln amb (pre-divided)
+  const float* itab;   // [2][N_lp3][N_s3]: ln(eps+eps_off), ln rho (pre-divided)
+};
+
+struct SrcParams {
+  double wdt, invRdt, g, inv_va2max;
+  double rho_min, eps_min, eps_max, vmax;
+};
+
+struct TvdParams {
+  double rho_min, eps_min, rho_lev, q_rho_max;
+  double visc_coeff_bot, visc_slope_bot, visc_coeff_top, visc_slope_top;
+  double eta_coeff_bot, eta_slope_bot, eta_coeff_top, eta_slope_top;
+  double B_par_diff, vhyp, tvd_pm_v, tvd_pm_B, vmax_lim, CME_thresh;
+  double tvd_h[4], tvd_cs[4], tc[4];
+  double inv_va2max, vsqr_diff, cmax, idxd1, dt_tvd;
+  int need_tvd_coeff, rho_log, is_gbeg0, is_gend0;
+  const double* hfb;
+  const double* hft;
+  const double* qft;
+  const double* tau;   // NULL: Tau == 0 everywhere
+};
+
+struct ResidParams {
+  double w1[3], w2[3];
+  double inv_va2max;
+};
+
+struct BcParams {
+  double p_bc, s_bc, B_crit, eps_top, cc;
+  int open_top;
+};
+
+struct TvdEpilogue {
+  EosDev E;
+  double rho_min, eps_min, eps_max, vmax, inv_va2max;
+  double* Pnew;
+  unsigned long long* cmax;
+};
+
+struct StageArgs {
+  Geom G;
+  EosDev E;
+  ResidParams RP;
+  SrcParams SP;
+  int ntx, nty, nkc;       // tiles in dim 0, dim 1, chunks in dim 2
+  int write_p;             // compute the EOS pressure of the new state
+};
+
+struct TvdLineArgs {
+  Geom G;
+  TvdParams P;
+};
