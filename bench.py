@@ -4,7 +4,12 @@
 then TVDlimit, TCheck, ghost fill) over the stratified 256x512x512 box
 (vertical 256, dx = 16 km), synthetic seeded state, inputs resident in HBM.
 
-Prints one JSON line.  value = interior cells x steps / wall time (Mcell-updates/s).
+Prints one JSON line.  value = interior cells x steps / wall time
+(Mcell-updates/s) of the plain FP64 step.  fp32_mixed: the same workload in
+the FP32-mixed form (FP32 rho, M, B, p and stencils; FP64 e, e_int recovery,
+dt).  precision_study: both forms run 100 steps from one state of the
+3 km/s box; per-field relative L2 / Linf error of the mixed form against the
+FP64 form (which matches the oracle to 1e-12) and the Mcell/s of both.
 roofline: the dominant kernel's algorithmic bytes per launch / its mean launch
 time from HIP events on the library's stream during the timed steps.
 cpu_baseline: the oracle (C restatement, FP64, OpenMP) on a bounded sample of
@@ -41,6 +46,14 @@ KERNEL_BYTES = {
     "stage_fused": (208.0, "interior"),
     "eos_p": (72.0, "buffer"),
 }
+# FP32-mixed form: a cell is rho, M, B in FP32 (28 B) + e in FP64 (8 B) = 36 B, p 4 B
+KERNEL_BYTES_MX = {
+    "stage_fused": (116.0, "interior"),   # read U_in 36 + p 4 + U0 36; write U_out 36 + p 4
+    "tvd_sweep_x": (76.0, "interior"),    # read U 36 + p 4; write U 36
+    "tvd_sweep_y": (76.0, "interior"),
+    "tvd_sweep_z": (80.0, "interior"),    # + new p 4
+    "eos_p": (40.0, "buffer"),
+}
 
 
 def log(msg):
@@ -74,13 +87,115 @@ def cpu_baseline(table, phys, box_gsize, dx, steps=3):
                 s_per_step=dt / steps)
 
 
+def timed_run(m, r, steps, warmup):
+    """warm-up steps, then `steps` timed steps with the per-kernel timers on."""
+    import ctypes as C
+    import torch
+    from muram_main_amd import abi, config
+    for _ in range(warmup):
+        m.step(r, config.DT_RAD_OFF, 1)
+    m.synchronize()
+    lib = m.lib
+    lib.mrm_timers(m.h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.step(r, config.DT_RAD_OFF, 1)
+    m.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = (C.c_double * abi.NTIMER)()
+    cnt = (C.c_long * abi.NTIMER)()
+    lib.mrm_get_timers(m.h, ms, cnt)
+    lib.mrm_timers(m.h, 0)
+    kernels = {}
+    for i in range(abi.NTIMER):
+        if cnt[i]:
+            kernels[lib.mrm_timer_name(i).decode()] = dict(total_ms=ms[i], launches=cnt[i], mean_ms=ms[i] / cnt[i])
+    return wall, kernels
+
+
+def roofline(kernels, table, ncell, nbuf, traffic_key):
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    per, unit = table.get(dom, (0.0, "interior"))
+    units = nbuf if unit == "buffer" else ncell
+    bytes_launch = per * units
+    achieved = bytes_launch / (kernels[dom]["mean_ms"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath)).get(traffic_key, {}).get(dom)
+        if tr:
+            traffic = tr.get("bytes_per_launch")
+    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                bytes_per_launch=bytes_launch, bytes_per_unit=per, units=units)
+
+
+def kernel_summary(kernels, wall):
+    return {k: {"mean_ms": round(v["mean_ms"], 4), "launches": v["launches"],
+                "share": round(v["total_ms"] / (wall * 1e3), 4)} for k, v in kernels.items()}
+
+
+def make_grid(wl, table, U, precision):
+    from muram_main_amd import config
+    from muram_main_amd.mhd import MhdGrid
+    g = config.grid_desc(wl.box.gsize, wl.box.dx)
+    m = MhdGrid(g, wl.phys, precision=precision)
+    m.eos_init(table)
+    m.set_state(U)
+    r = config.run_desc()
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
+    return m, r
+
+
+def precision_study(wl, table, tab, nsteps):
+    """FP64 and FP32-mixed forms from one state, nsteps steps each; per-field
+    errors of the mixed state against the FP64 one over the interior."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from orc import field_errors
+    from muram_main_amd import abi, config, initial
+    t = time.perf_counter()
+    U = initial.make_state(tab, wl.box)
+    log(f"precision study state {wl.box.gsize} in {time.perf_counter() - t:.1f}s")
+    out = {"workload": wl.name, "steps": nsteps}
+    states, dts = {}, {}
+    ncell = wl.box.gsize[0] * wl.box.gsize[1] * wl.box.gsize[2]
+    for name, prec in (("fp64", abi.PREC_FP64), ("fp32_mixed", abi.PREC_FP32_MIXED)):
+        m, r = make_grid(wl, table, U, prec)
+        m.synchronize()
+        t0 = time.perf_counter()
+        m.step(r, config.DT_RAD_OFF, nsteps)
+        m.synchronize()
+        wall = time.perf_counter() - t0
+        out[f"{name}_mcells"] = round(ncell * nsteps / wall / 1e6, 2)
+        states[name] = m.get_state()
+        dts[name] = r.dt
+        m.close()
+        log(f"precision study {name}: {nsteps} steps in {wall:.2f}s")
+    del U
+    sl = (slice(None),) + tuple(slice(2, -2) for _ in range(3))
+    errs = field_errors(states["fp32_mixed"][sl], states["fp64"][sl])
+    out["speedup"] = round(out["fp32_mixed_mcells"] / out["fp64_mcells"], 3)
+    out["dt_rel_diff"] = abs(dts["fp32_mixed"] / dts["fp64"] - 1.0)
+    out["errors"] = {k: {"l2": float(f"{v['l2']:.3e}"), "linf": float(f"{v['linf']:.3e}")} for k, v in errs.items()}
+    out["max_l2"] = max(v["l2"] for v in errs.values())
+    out["max_linf"] = max(v["linf"] for v in errs.values())
+    out["reference"] = "FP64 form of this library (parity with the FP64 oracle <= 1e-12)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="box512")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
+    ap.add_argument("--error-steps", type=int, default=100,
+                    help="steps of the FP64 vs FP32-mixed precision study (0: skip)")
+    ap.add_argument("--error-workload", default="box512_v3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-n", type=int, default=512)
     ap.add_argument("--path", default="fused", choices=["fused", "sequence"])
@@ -101,7 +216,6 @@ def main():
 
     import torch
     from muram_main_amd import abi, config, eos_table, initial
-    from muram_main_amd.mhd import MhdGrid
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
@@ -120,70 +234,49 @@ def main():
     t = time.perf_counter()
     U = initial.make_state(tab, box)
     log(f"state {box.gsize} in {time.perf_counter() - t:.1f}s")
-    g = config.grid_desc(box.gsize, box.dx)
-    m = MhdGrid(g, wl.phys)
-    m.eos_init(table)
-    m.set_state(U)
-    del U
-    r = config.run_desc()
-    m.exchange_grid()
-    m.SetBoundaryConditions(r, 0)
-    for _ in range(args.warmup):
-        m.step(r, config.DT_RAD_OFF, 1)
-    m.synchronize()
-    log(f"warm-up done, dt = {r.dt:.4e} s")
-
-    lib = m.lib
-    lib.mrm_timers(m.h, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        m.step(r, config.DT_RAD_OFF, 1)
-    m.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    import ctypes as C
-    ms = (C.c_double * abi.NTIMER)()
-    cnt = (C.c_long * abi.NTIMER)()
-    lib.mrm_get_timers(m.h, ms, cnt)
-    lib.mrm_timers(m.h, 0)
-    kernels = {}
-    for i in range(abi.NTIMER):
-        if cnt[i]:
-            kernels[lib.mrm_timer_name(i).decode()] = dict(total_ms=ms[i], launches=cnt[i],
-                                                           mean_ms=ms[i] / cnt[i])
     ncell = box.gsize[0] * box.gsize[1] * box.gsize[2]
-    nbuf = m.bufsize
-    value = ncell * args.steps / wall / 1e6
-    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    per, unit = KERNEL_BYTES.get(dom, (0.0, "interior"))
-    units = nbuf if unit == "buffer" else ncell
-    bytes_launch = per * units
-    achieved = bytes_launch / (kernels[dom]["mean_ms"] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        tr = json.load(open(tpath)).get(dom)
-        if tr:
-            traffic = tr.get("bytes_per_launch")
-    roof = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                bytes_per_launch=bytes_launch, bytes_per_unit=per, units=units)
+    forms = {"fp64": ["fp64"], "fp32": ["fp32"], "both": ["fp64", "fp32"]}[args.precision]
+    res_forms = {}
+    dt_s = None
+    for form in forms:
+        prec = abi.PREC_FP64 if form == "fp64" else abi.PREC_FP32_MIXED
+        m, r = make_grid(wl, table, U, prec)
+        wall, kernels = timed_run(m, r, args.steps, args.warmup)
+        nbuf = m.bufsize
+        m.close()
+        value = ncell * args.steps / wall / 1e6
+        kb = KERNEL_BYTES if form == "fp64" else KERNEL_BYTES_MX
+        res_forms[form] = dict(
+            value=round(value, 2), ms_per_step=round(wall / args.steps * 1e3, 3),
+            hbm_frac_model=round(value * 1e6 * MODEL_BYTES[form] / (HBM_PEAK_GBS * 1e9), 4),
+            roofline=roofline(kernels, kb, ncell, nbuf, form), kernels=kernel_summary(kernels, wall),
+            dt_s=r.dt)
+        log(f"{form}: {res_forms[form]['ms_per_step']} ms/step, {value:.1f} Mcell/s")
+    del U
+    main_form = forms[0]
+    R = res_forms[main_form]
     res = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": R["value"], "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": R["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic (seeded stratified box, generated EOS table)",
         "config": {"workload": wl.name, "gsize": list(box.gsize), "dx_cm": box.dx[0],
-                   "precision": args.precision, "steps_timed": args.steps, "dt_s": r.dt,
+                   "precision": "fp64 plain" if main_form == "fp64" else "fp32-mixed",
+                   "steps_timed": args.steps, "dt_s": R["dt_s"],
                    "path": args.path, "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "1"),
                    "fma": os.environ.get("MRM_FMA", "1")},
-        "hbm_frac_model": round(value * 1e6 * MODEL_BYTES[args.precision] / (HBM_PEAK_GBS * 1e9), 4),
-        "roofline": roof,
-        "kernels": {k: {"mean_ms": round(v["mean_ms"], 4), "launches": v["launches"],
-                        "share": round(v["total_ms"] / (wall * 1e3), 4)} for k, v in kernels.items()},
+        "hbm_frac_model": R["hbm_frac_model"],
+        "roofline": R["roofline"],
+        "kernels": R["kernels"],
     }
-    m.close()
+    if "fp32" in res_forms and main_form == "fp64":
+        F = res_forms["fp32"]
+        res["fp32_mixed"] = {k: F[k] for k in ("value", "ms_per_step", "hbm_frac_model", "roofline", "kernels")}
+        res["fp32_mixed"]["speedup_vs_fp64"] = round(F["value"] / R["value"], 3)
+    if args.error_steps > 0 and "fp32" in res_forms:
+        res["precision_study"] = precision_study(config.workloads()[args.error_workload], table, tab,
+                                                 args.error_steps)
     if not args.no_cpu:
         t = time.perf_counter()
         res["cpu_baseline"] = cpu_baseline(table, wl.phys, (256, 128, 128), box.dx)

@@ -20,8 +20,10 @@ FLAGS = [
 
 # translation units and their contraction mode: the drop-in operations keep
 # plain IEEE arithmetic in the reference's operation order (no FMA); the
-# fused-step kernels are also built with contraction in mrm_fast.hip
-UNITS = [("mrm_step.hip", "-ffp-contract=off"), ("mrm_fast.hip", "-ffp-contract=fast")]
+# fused-step kernels are also built with contraction in mrm_fast.hip; the
+# FP32-mixed kernels (mrm_mixed.hip) are contracted
+UNITS = [("mrm_step.hip", "-ffp-contract=off"), ("mrm_fast.hip", "-ffp-contract=fast"),
+         ("mrm_mixed.hip", "-ffp-contract=fast")]
 
 
 def _run(cmd, verbose):

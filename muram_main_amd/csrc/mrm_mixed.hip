@@ -1,0 +1,79 @@
+// mrm_mixed.hip -- the FP32-mixed step kernels (built with FMA contraction)
+// and their launchers.
+#define MRM_NS mixed
+#include "mrm_mixed.h"
+#include "mrm_mixed_api.h"
+
+namespace mixed {
+
+static inline unsigned nblk_(long n) { return (unsigned)((n + MRM_BLOCK - 1) / MRM_BLOCK); }
+
+hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
+                        MxState out, float* Pout, const double* Q) {
+  k_stage_mx<<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  return hipGetLastError();
+}
+
+hipError_t launch_exchange(hipStream_t st, const Geom& G, MxState S, float* P) {
+  const int nx = G.le0 - G.lb0 + 1;
+  long n1 = (long)nx * (G.le2 - G.lb2 + 1) * G.g1;
+  k_exchange_mx<1><<<nblk_(n1), MRM_BLOCK, 0, st>>>(G, S, P);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  long n2 = (long)nx * G.n1 * G.g2;
+  k_exchange_mx<2><<<nblk_(n2), MRM_BLOCK, 0, st>>>(G, S, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_sums(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, int mode,
+                          double* part) {
+  k_bc_sums_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, S, mode, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, const BcParams& B,
+                     const double* psum, MxState S, float* P, double inv_va2max, unsigned long long* cmax) {
+  k_bc_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, B, psum, S, P, inv_va2max, cmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_eos_p(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, float* P,
+                        double inv_va2max, unsigned long long* cmax) {
+  k_eos_p_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, S, P, inv_va2max, cmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const Geom& G,
+                              const MxTvdParams& P, MxState S, const float* Pr) {
+  k_tvd_x_lines_mx<<<nl, bsz, lds, st>>>(G, P, S, Pr);
+  return hipGetLastError();
+}
+
+hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const MxEpilogue* epi, unsigned nb, const Geom& G,
+                            const MxTvdParams& P, MxState S, const float* Pr) {
+  MxEpilogue X{};
+  if (d == 0) {
+    if (tau) k_tvd_sweep_mx<0, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+    else k_tvd_sweep_mx<0, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+  } else if (d == 1) {
+    if (tau) k_tvd_sweep_mx<1, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+    else k_tvd_sweep_mx<1, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+  } else if (epi) {
+    if (tau) k_tvd_sweep_mx<2, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, *epi);
+    else k_tvd_sweep_mx<2, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, *epi);
+  } else {
+    if (tau) k_tvd_sweep_mx<2, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+    else k_tvd_sweep_mx<2, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(hipStream_t st, bool to_mixed, long bs, double* U, MxState S) {
+  unsigned nb = nblk_(bs);
+  if (nb > 65536u) nb = 65536u;
+  if (to_mixed) k_to_mixed<<<nb, MRM_BLOCK, 0, st>>>(bs, U, S);
+  else k_from_mixed<<<nb, MRM_BLOCK, 0, st>>>(bs, S, U);
+  return hipGetLastError();
+}
+
+}  // namespace mixed

@@ -1,0 +1,22 @@
+// mrm_mixed_api.h -- host-side launchers of the FP32-mixed step kernels
+// (compiled in mrm_mixed.hip, namespace mixed).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "mrm_types.h"
+
+namespace mixed {
+hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
+                        MxState out, float* Pout, const double* Q);
+hipError_t launch_exchange(hipStream_t st, const Geom& G, MxState S, float* P);
+hipError_t launch_bc_sums(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, int mode,
+                          double* part);
+hipError_t launch_bc(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, const BcParams& B,
+                     const double* psum, MxState S, float* P, double inv_va2max, unsigned long long* cmax);
+hipError_t launch_eos_p(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, float* P,
+                        double inv_va2max, unsigned long long* cmax);
+hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const Geom& G,
+                              const MxTvdParams& P, MxState S, const float* Pr);
+hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const MxEpilogue* epi, unsigned nb, const Geom& G,
+                            const MxTvdParams& P, MxState S, const float* Pr);
+hipError_t launch_convert(hipStream_t st, bool to_mixed, long bs, double* U, MxState S);
+}  // namespace mixed

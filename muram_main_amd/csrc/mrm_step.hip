@@ -16,6 +16,7 @@
 
 #include "../../include/muram_step.h"
 #include "mrm_launch.h"
+#include "mrm_mixed_api.h"
 
 using namespace exact;
 
@@ -52,6 +53,11 @@ struct mrm_ctx {
   bool stage_zwin = false;                       // stage kernel variant
   bool fma = true;                               // fused step uses the contracted build
   double* dF = nullptr;   // MRM_NFIELD arrays of bs doubles
+  // FP32-mixed contexts: three mixed state buffers and two FP32 pressures in
+  // place of S[], dP2 and the FP64 pressure; dR is the FP64 staging buffer
+  MxState M[3] = {};
+  void* mxmem[3] = {nullptr, nullptr, nullptr};
+  float* Pm[2] = {nullptr, nullptr};
   bool qtot_set = false, tau_set = false;
 
   bool eos_ok = false;
@@ -143,6 +149,13 @@ static int fail(mrm_ctx* c, int code, const char* what, hipError_t e = hipSucces
     if (e_ != hipSuccess) return fail(c, MRM_ERR_HIP, #x, e_); \
   } while (0)
 
+// drop-in per-operation calls work on FP64 state; FP32-mixed contexts run whole steps
+#define NEED_FP64(c, fn)                              \
+  do {                                                \
+    if ((c)->precision != MRM_PREC_FP64)              \
+      return fail(c, MRM_ERR_UNSUPPORTED, fn ": FP32-mixed contexts run whole steps (mrm_step) only"); \
+  } while (0)
+
 #define LAUNCHCHK(c, name)                            \
   do {                                                \
     hipError_t e_ = hipGetLastError();                \
@@ -163,7 +176,7 @@ static inline double bits_to_double(unsigned long long b) {
 extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, int precision, mrm_ctx** out) {
   if (!grid || !phys || !out) return MRM_ERR_ARG;
   *out = nullptr;
-  if (precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
+  if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
   if (phys->params[3] > 0.0 || phys->params[4] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
@@ -199,11 +212,33 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   int rc = MRM_OK;
   hipError_t e;
   if ((e = hipStreamCreate(&c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipStreamCreate", e);
-  for (int q = 0; q < 3 && !rc; q++)
-    if ((e = hipMalloc(&c->S[q], 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc state", e);
+  const bool mixed_prec = precision == MRM_PREC_FP32_MIXED;
+  // the mixed kernels address cells with 32-bit byte offsets (8-byte energy)
+  if (mixed_prec && 8 * bs >= (size_t(1) << 32)) {
+    fail(c, MRM_ERR_UNSUPPORTED, "FP32-mixed form: at most 2^29 cells per buffer");
+    fprintf(stderr, "mrm_create: %s\n", c->err.c_str());
+    mrm_destroy(c);
+    return MRM_ERR_UNSUPPORTED;
+  }
+  const size_t fbytes = (7 * bs * sizeof(float) + 255) / 256 * 256;
+  for (int q = 0; q < 3 && !rc; q++) {
+    if (mixed_prec) {
+      if ((e = hipMalloc(&c->mxmem[q], fbytes + bs * sizeof(double))) != hipSuccess)
+        rc = fail(c, MRM_ERR_HIP, "hipMalloc mixed state", e);
+      else {
+        c->M[q].f = (float*)c->mxmem[q];
+        c->M[q].e = (double*)((char*)c->mxmem[q] + fbytes);
+      }
+    } else if ((e = hipMalloc(&c->S[q], 8 * bs * sizeof(double))) != hipSuccess) {
+      rc = fail(c, MRM_ERR_HIP, "hipMalloc state", e);
+    }
+  }
   c->dU = c->S[0];
   c->dU0 = c->S[1];
-  if (!rc && (e = hipMalloc(&c->dP2, bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc P2", e);
+  for (int q = 0; q < 2 && mixed_prec && !rc; q++)
+    if ((e = hipMalloc(&c->Pm[q], bs * sizeof(float))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc P", e);
+  if (!rc && !mixed_prec && (e = hipMalloc(&c->dP2, bs * sizeof(double))) != hipSuccess)
+    rc = fail(c, MRM_ERR_HIP, "hipMalloc P2", e);
   if (!rc && (e = hipMalloc(&c->dR, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc Res", e);
   if (!rc && (e = hipMalloc(&c->dF, MRM_NFIELD * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc fields", e);
   c->npart = 8 * (long)nblk((long)n[1] * n[2]) + 64;
@@ -212,9 +247,18 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (!rc && (e = hipHostMalloc(&c->h_red, RED_NSLOT * sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipHostMalloc", e);
   if (!rc && (e = hipMalloc(&c->d_prof, 3 * n[0] * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc prof", e);
   // Res starts at zero (the reference relies on it, grid.C:279); all arrays defined
-  for (int q = 0; q < 3 && !rc; q++)
-    if ((e = hipMemsetAsync(c->S[q], 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
-  if (!rc && (e = hipMemsetAsync(c->dP2, 0, bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  for (int q = 0; q < 3 && !rc; q++) {
+    if (mixed_prec) {
+      if ((e = hipMemsetAsync(c->mxmem[q], 0, fbytes + bs * sizeof(double), c->stream)) != hipSuccess)
+        rc = fail(c, MRM_ERR_HIP, "memset", e);
+    } else if ((e = hipMemsetAsync(c->S[q], 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) {
+      rc = fail(c, MRM_ERR_HIP, "memset", e);
+    }
+  }
+  for (int q = 0; q < 2 && mixed_prec && !rc; q++)
+    if ((e = hipMemsetAsync(c->Pm[q], 0, bs * sizeof(float), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && !mixed_prec && (e = hipMemsetAsync(c->dP2, 0, bs * sizeof(double), c->stream)) != hipSuccess)
+    rc = fail(c, MRM_ERR_HIP, "memset", e);
   const char* path = getenv("MRM_STEP_PATH");
   c->force_sequence = path && strcmp(path, "sequence") == 0;
   const char* zw = getenv("MRM_STAGE_ZWIN");
@@ -238,7 +282,8 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   c->harvest();
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
-  for (int q = 0; q < 3; q++) hipFree(c->S[q]);
+  for (int q = 0; q < 3; q++) { hipFree(c->S[q]); hipFree(c->mxmem[q]); }
+  for (int q = 0; q < 2; q++) hipFree(c->Pm[q]);
   hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
   if (c->h_red) hipHostFree(c->h_red);
@@ -281,7 +326,7 @@ extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   double del_lr = (lr1 - lr0) / ((double)(N_lr - 1));
   double del_lp3 = (lp1 - lp0) / ((double)(N_lp3 - 1));
   double del_s3 = (s1 - s0) / ((double)(N_s3 - 1));
-  std::vector<double> ax(N_eps + N_lr + N_lp3 + N_s3);
+  std::vector<double> ax(2 * N_eps + 2 * N_lr + N_lp3 + N_s3);
   double* xeps = ax.data();
   double* xlr = xeps + N_eps;
   double* xlp3 = xlr + N_lr;
@@ -290,6 +335,10 @@ extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   for (int i = 0; i < N_lr; i++) xlr[i] = lr0 + del_lr * ((double)i);
   for (int i = 0; i < N_lp3; i++) xlp3[i] = lp0 + del_lp3 * ((double)i);
   for (int i = 0; i < N_s3; i++) xs3[i] = s0 + del_s3 * ((double)i);
+  double* rxeps = xs3 + N_s3;
+  double* rxlr = rxeps + N_eps;
+  for (int i = 0; i < N_eps; i++) rxeps[i] = exp(-xeps[i]);
+  for (int i = 0; i < N_lr; i++) rxlr[i] = exp(-xlr[i]);
   std::vector<float> tab(6 * nt), itab(2 * ni);
   const float* fw = buf + 14;
   double sc = del_eps * del_lr;
@@ -321,6 +370,8 @@ extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   E.eps_max = exp(xeps[N_eps - 1]) - E.eps_off;   // cons_to_prim.C:52, host libm
   double* dax = (double*)base;
   E.xeps = dax; E.xlr = dax + N_eps; E.xlp3 = dax + N_eps + N_lr; E.xs3 = dax + N_eps + N_lr + N_lp3;
+  E.rxeps = E.xs3 + N_s3;
+  E.rxlr = E.rxeps + N_eps;
   E.tab = (const float*)(base + axb);
   E.itab = (const float*)(base + axb + tab.size() * sizeof(float));
   c->eos_ok = true;
@@ -332,9 +383,13 @@ static double* state_ptr(mrm_ctx* c, int which) {
   return which == MRM_U ? c->dU : which == MRM_U0 ? c->dU0 : which == MRM_RES ? c->dR : nullptr;
 }
 
+// In FP32-mixed contexts U is staged through dR (FP64) and converted on the
+// device: rho, M, B rounded to FP32, e kept in FP64.
 extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layout) {
   if (!c || !host) return MRM_ERR_ARG;
-  double* d = state_ptr(c, which);
+  const bool mx = c->precision == MRM_PREC_FP32_MIXED;
+  if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_set_state: FP32-mixed contexts hold U only");
+  double* d = mx ? c->dR : state_ptr(c, which);
   if (!d) return fail(c, MRM_ERR_ARG, "mrm_set_state: bad array id");
   c->fast_valid = false;
   const size_t bs = c->G.bs;
@@ -349,15 +404,19 @@ extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layo
   } else {
     return fail(c, MRM_ERR_ARG, "mrm_set_state: bad layout");
   }
+  if (mx) HIPCHK(c, mixed::launch_convert(c->stream, true, c->G.bs, c->dR, c->M[c->cur]));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRM_OK;
 }
 
 extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
   if (!c || !host) return MRM_ERR_ARG;
-  double* d = state_ptr(c, which);
+  const bool mx = c->precision == MRM_PREC_FP32_MIXED;
+  if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_get_state: FP32-mixed contexts hold U only");
+  double* d = mx ? c->dR : state_ptr(c, which);
   if (!d) return fail(c, MRM_ERR_ARG, "mrm_get_state: bad array id");
   const size_t bs = c->G.bs;
+  if (mx) HIPCHK(c, mixed::launch_convert(c->stream, false, c->G.bs, c->dR, c->M[c->cur]));
   if (layout == MRM_LAYOUT_SOA) {
     HIPCHK(c, hipMemcpyAsync(host, d, 8 * bs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -387,6 +446,14 @@ extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
 extern "C" int mrm_get_field(mrm_ctx* c, int f, double* host) {
   if (!c || !host || f < 0 || f >= MRM_NFIELD) return MRM_ERR_ARG;
   const size_t bs = c->G.bs;
+  if (c->precision == MRM_PREC_FP32_MIXED && f == MRM_PRES) {
+    // the FP32 pressure carried by the mixed step, widened
+    std::vector<float> pf(bs);
+    HIPCHK(c, hipMemcpyAsync(pf.data(), c->Pm[0], bs * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t n = 0; n < bs; n++) host[n] = (double)pf[n];
+    return MRM_OK;
+  }
   HIPCHK(c, hipMemcpyAsync(host, c->dF + f * bs, bs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRM_OK;
@@ -418,6 +485,7 @@ extern "C" int mrm_set_bc(mrm_ctx* c, double p_bc, double s_bc) {
 // ------------------------------------------------------------------ ConsToPrim
 extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_cons_to_prim");
   c->fast_valid = false;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_cons_to_prim: no EOS table loaded");
   unsigned nb = nblk(c->G.bs);
@@ -433,6 +501,7 @@ extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
 extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
   (void)run;
   if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_adjust_valf_max");
   c->fast_valid = false;
   HIPCHK(c, hipMemsetAsync(c->d_red, 0, RED_NSLOT * sizeof(unsigned long long), c->stream));
   long n = ninterior(c);
@@ -520,6 +589,7 @@ static int read_cmax(mrm_ctx* c, double* cmax) {
 
 extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_cfl) {
   if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_mhd_residual");
   if (run && run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   int rc = launch_residual(c);
   if (rc) return rc;
@@ -571,6 +641,7 @@ static SrcParams src_params(const mrm_ctx* c, const mrm_run_desc* run, int stage
 
 extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_source_integrate_tcheck");
   c->fast_valid = false;
   SrcParams SP = src_params(c, run, stage);
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
@@ -582,6 +653,7 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
 
 extern "C" int mrm_save_cons(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_save_cons");
   c->fast_valid = false;
   TScope ts(c, T_SAVECONS);
   k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
@@ -592,6 +664,7 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
 extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
   (void)run;
   if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_tcheck");
   c->fast_valid = false;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
@@ -616,18 +689,28 @@ static int exchange_buf(mrm_ctx* c, double* U, double* P) {
   return MRM_OK;
 }
 
+static int exchange_mx(mrm_ctx* c, MxState M, float* P) {
+  TScope ts(c, T_EXCH);
+  HIPCHK(c, mixed::launch_exchange(c->stream, c->G, M, P));
+  return MRM_OK;
+}
+
 extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   c->fast_valid = false;
+  if (c->precision == MRM_PREC_FP32_MIXED) return exchange_mx(c, c->M[c->cur], nullptr);
   return exchange_buf(c, c->dU, nullptr);
 }
 
 // ------------------------------------------------------------------ boundary conditions
-static int bottom_sums(mrm_ctx* c, int mode, const double* U, double* dst_dev) {
+// partial sums over the two lowest layers of U (FP64) or of the mixed state M
+static int bottom_sums(mrm_ctx* c, int mode, const double* U, double* dst_dev, const MxState* M = nullptr) {
   const long nyz = (long)c->lsize[1] * c->lsize[2];
   unsigned nb = nblk(nyz);
   TScope ts(c, T_BCSUM);
-  if (mode == 0) {
+  if (M) {
+    HIPCHK(c, mixed::launch_bc_sums(c->stream, nb, c->G, c->E, *M, mode, c->d_part));
+  } else if (mode == 0) {
     k_bc_sums<0><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, U, c->d_part);
     LAUNCHCHK(c, "k_bc_sums<0>");
   } else {
@@ -640,9 +723,9 @@ static int bottom_sums(mrm_ctx* c, int mode, const double* U, double* dst_dev) {
 }
 
 // set_p_s_bc (boundary_pdmp_1_fftw3.C:496-565), one rank
-static int set_p_s_bc(mrm_ctx* c, const double* U) {
+static int set_p_s_bc(mrm_ctx* c, const double* U, const MxState* M = nullptr) {
   double* dsum = (double*)(c->d_red + RED_SUM0);
-  int rc = bottom_sums(c, 1, U, dsum);
+  int rc = bottom_sums(c, 1, U, dsum, M);
   if (rc) return rc;
   double h[5];
   HIPCHK(c, hipMemcpyAsync(h, dsum, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -704,11 +787,33 @@ static int bc_buf(mrm_ctx* c, double* U, double* P, unsigned long long* cmax) {
   return MRM_OK;
 }
 
+// SetBoundaryConditions on the mixed state M (ghost pressures with P, the
+// dim-0 ghost faces in the cmax reduction with cmax)
+static int bc_mx(mrm_ctx* c, MxState M, float* P, unsigned long long* cmax) {
+  int rc;
+  if (c->bnd_ini) {
+    if ((c->p_bc < 0.0) || (c->s_bc < 0.0)) {
+      rc = set_p_s_bc(c, nullptr, &M);
+      if (rc) return rc;
+    }
+    c->bnd_ini = 0;
+  }
+  double* dsum = (double*)(c->d_red + RED_SUM0);
+  rc = bottom_sums(c, 0, nullptr, dsum, &M);
+  if (rc) return rc;
+  BcParams B = bc_params(c);
+  const long nyz = (long)c->G.n1 * c->G.n2;
+  TScope ts(c, T_BCBOT);
+  HIPCHK(c, mixed::launch_bc(c->stream, nblk(nyz), c->G, c->E, B, dsum, M, P, c->inv_va2max, cmax));
+  return MRM_OK;
+}
+
 extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   (void)run; (void)stage;
   if (!c) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
+  if (c->precision == MRM_PREC_FP32_MIXED) return bc_mx(c, c->M[c->cur], nullptr, nullptr);
   return bc_buf(c, c->dU, nullptr, nullptr);
 }
 
@@ -811,6 +916,7 @@ static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const
 
 extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
   if (!c || !run) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_tvdlimit");
   c->fast_valid = false;
   return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, dt_tvd, nullptr);
 }
@@ -947,13 +1053,155 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   return MRM_OK;
 }
 
+// ------------------------------------------------------------------ FP32-mixed step
+static MxTvdParams mx_tvd_params(const TvdParams& P) {
+  MxTvdParams Q;
+  Q.rho_min = (float)P.rho_min;
+  Q.rho_lev = (float)P.rho_lev;
+  Q.q_rho_max = (float)P.q_rho_max;
+  Q.visc_coeff_bot = (float)P.visc_coeff_bot; Q.visc_slope_bot = (float)P.visc_slope_bot;
+  Q.visc_coeff_top = (float)P.visc_coeff_top; Q.visc_slope_top = (float)P.visc_slope_top;
+  Q.eta_coeff_bot = (float)P.eta_coeff_bot; Q.eta_slope_bot = (float)P.eta_slope_bot;
+  Q.eta_coeff_top = (float)P.eta_coeff_top; Q.eta_slope_top = (float)P.eta_slope_top;
+  Q.B_par_diff = (float)P.B_par_diff; Q.vhyp = (float)P.vhyp;
+  Q.tvd_pm_v = (float)P.tvd_pm_v; Q.tvd_pm_B = (float)P.tvd_pm_B;
+  Q.vmax_lim = (float)P.vmax_lim; Q.CME_thresh = (float)P.CME_thresh;
+  for (int q = 0; q < 4; q++) {
+    Q.tvd_h[q] = (float)P.tvd_h[q]; Q.tvd_cs[q] = (float)P.tvd_cs[q]; Q.tc[q] = (float)P.tc[q];
+  }
+  Q.inv_va2max = (float)P.inv_va2max;
+  Q.vsqr_diff = (float)P.vsqr_diff;
+  Q.cmax = (float)P.cmax;
+  Q.idxd1 = (float)P.idxd1;
+  Q.dt_tvd = (float)P.dt_tvd;
+  Q.eps_min = P.eps_min;
+  Q.need_tvd_coeff = P.need_tvd_coeff; Q.rho_log = P.rho_log;
+  Q.is_gbeg0 = P.is_gbeg0; Q.is_gend0 = P.is_gend0;
+  Q.hfb = P.hfb; Q.hft = P.hft; Q.qft = P.qft; Q.tau = P.tau;
+  return Q;
+}
+
+// the three TVD sweeps on the mixed state; with epi the dim-2 sweep finishes
+// each cell (TCheck, new pressure, cmax)
+static int tvd_mx(mrm_ctx* c, MxState M, const float* Pr, double dt_tvd, const MxEpilogue* epi) {
+  TvdParams P;
+  int rc = tvd_params(c, dt_tvd, P);
+  if (rc) return rc;
+  const Geom& G = c->G;
+  const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
+  const bool tau = P.tau != nullptr;
+  for (int d = 0; d < 3; d++) {
+    P.cmax = 0.975 * c->dx[d] / dt_tvd;
+    P.idxd1 = idx[d];
+    MxTvdParams Q = mx_tvd_params(P);
+    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
+    TScope ts(c, T_TVD0 + d);
+    hipError_t e;
+    size_t lds = (size_t)(3 * MRM_NV + 5) * G.n0 * sizeof(float);
+    if (d == 0 && G.n0 <= TX_MAXN && lds <= 64 * 1024) {
+      unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
+      e = mixed::launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, G, Q, M, Pr);
+    } else {
+      e = mixed::launch_tvd_sweep(c->stream, d, tau, d == 2 ? epi : nullptr, nblk(nl), G, Q, M, Pr);
+    }
+    if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep (mixed)", e);
+  }
+  return MRM_OK;
+}
+
+static int stage_mx(mrm_ctx* c, const mrm_run_desc* run, int stage, MxState in, const float* Pin, MxState s0,
+                    MxState out, float* Pout) {
+  MxStageArgs A;
+  A.G = c->G;
+  A.E = c->E;
+  for (int d = 0; d < 3; d++) {
+    A.w1[d] = (float)(8. / (12. * c->dx[d]));
+    A.w2[d] = (float)(-1. / (12. * c->dx[d]));
+  }
+  A.inv_va2max = (float)c->inv_va2max;
+  SrcParams SP = src_params(c, run, stage);
+  A.wdt = SP.wdt;
+  A.invRdt = (float)SP.invRdt;
+  A.g = (float)SP.g;
+  A.rho_min = (float)SP.rho_min;
+  A.vmax = (float)SP.vmax;
+  A.eps_min = SP.eps_min;
+  A.eps_max = SP.eps_max;
+  A.ntx = (c->lsize[0] + ST_SX - 1) / ST_SX;
+  A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
+  A.nkc = (c->lsize[2] + ST_KC - 1) / ST_KC;
+  A.write_p = Pout != nullptr;
+  const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
+  unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
+  TScope ts(c, T_NAMED);
+  HIPCHK(c, mixed::launch_stage(c->stream, nb, A, in, Pin, s0, out, Pout, Q));
+  return MRM_OK;
+}
+
+// One time step in the FP32-mixed form: the fused step's schedule on mixed
+// state buffers (FP32 rho, M, B, p; FP64 e), with the cmax reduction and dt
+// on the host in FP64.
+static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
+  const Geom& G = c->G;
+  float* Pc = c->Pm[0];
+  float* Pa = c->Pm[1];
+  unsigned long long* cmax = c->d_red + RED_CMAX;
+  int rc;
+  valf_constants(c);
+  if (!c->fast_valid) {
+    HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+    unsigned nb = nblk(G.bs);
+    if (nb > 65536u) nb = 65536u;
+    TScope ts(c, T_NAMED + 1);
+    HIPCHK(c, mixed::launch_eos_p(c->stream, nb, G, c->E, c->M[c->cur], Pc, c->inv_va2max, cmax));
+  }
+  double cm;
+  if ((rc = read_cmax(c, &cm))) return rc;
+  double cfl = 0.0;
+  if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
+  const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
+  MxState* M = c->M;
+  struct { int in; float* pin; int out; float* pout; } st[4] = {
+      {i0, Pc, i1, Pa}, {i1, Pa, i2, Pc}, {i2, Pc, i1, Pa}, {i1, Pa, i2, nullptr}};
+  for (int s = 0; s < 4; s++) {
+    if ((rc = stage_mx(c, run, s + 1, M[st[s].in], st[s].pin, M[i0], M[st[s].out], st[s].pout))) return rc;
+    if ((rc = exchange_mx(c, M[st[s].out], st[s].pout))) return rc;
+    if ((rc = bc_mx(c, M[st[s].out], st[s].pout, nullptr))) return rc;
+  }
+  HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+  MxEpilogue X;
+  X.E = c->E;
+  X.rho_min = (float)c->phys.tchk[1];
+  X.eps_min = c->phys.tchk[0];
+  X.vmax = (float)((c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3]);
+  X.eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  X.inv_va2max = c->inv_va2max;
+  X.Pnew = Pc;
+  X.cmax = cmax;
+  if (cfl <= run->CFL_tvd) {
+    if ((rc = tvd_mx(c, M[i2], Pa, run->dt, &X))) return rc;
+  } else {
+    if ((rc = tvd_mx(c, M[i2], Pa, 0.5 * run->dt, nullptr))) return rc;
+    if ((rc = exchange_mx(c, M[i2], nullptr))) return rc;
+    if ((rc = tvd_mx(c, M[i2], Pa, 0.5 * run->dt, &X))) return rc;
+  }
+  if ((rc = exchange_mx(c, M[i2], Pc))) return rc;
+  if ((rc = bc_mx(c, M[i2], Pc, cmax))) return rc;
+  c->cur = i2;
+  c->fast_valid = true;
+  return MRM_OK;
+}
+
 extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
   if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   const bool fused = !c->force_sequence && !(c->phys.params[2] > 0.0);
+  const bool mx = c->precision == MRM_PREC_FP32_MIXED;
+  if (mx && !fused)
+    return fail(c, MRM_ERR_UNSUPPORTED, "mrm_step: the FP32-mixed form runs the fused step (va_adjust = 0) only");
   for (int it = 0; it < nsteps; it++) {
-    int rc = fused ? step_fused(c, run, dt_rad) : step_sequence(c, run, dt_rad);
+    int rc = mx ? step_mixed(c, run, dt_rad) : fused ? step_fused(c, run, dt_rad) : step_sequence(c, run, dt_rad);
     if (rc) return rc;
     run->iteration++;
     run->globiter++;

@@ -37,6 +37,8 @@ struct EosDev {
   const double* xlr;
   const double* xlp3;
   const double* xs3;
+  const double* rxeps;  // exp(-xeps[i]) and exp(-xlr[j]): offsets of a point from its
+  const double* rxlr;   // table node as logs of ratios near 1 (FP32-mixed lookup)
   const float* tab;    // [6][N_eps][N_lr]: ln p, ln T, s, ln ne, ln rhoi, ln amb (pre-divided)
   const float* itab;   // [2][N_lp3][N_s3]: ln(eps+eps_off), ln rho (pre-divided)
 };
@@ -89,4 +91,46 @@ struct StageArgs {
 struct TvdLineArgs {
   Geom G;
   TvdParams P;
+};
+
+// ---- FP32-mixed form: rho, M, B and p in FP32, the total energy in FP64
+struct MxState {
+  float* f;    // [7][bs]: rho, M0, M1, M2, B0, B1, B2
+  double* e;   // [bs]
+};
+
+struct MxStageArgs {
+  Geom G;
+  EosDev E;
+  float w1[3], w2[3];
+  float inv_va2max;
+  double wdt;
+  float invRdt, g;
+  float rho_min, vmax;
+  double eps_min, eps_max;
+  int ntx, nty, nkc;
+  int write_p;
+};
+
+struct MxTvdParams {
+  float rho_min, rho_lev, q_rho_max;
+  float visc_coeff_bot, visc_slope_bot, visc_coeff_top, visc_slope_top;
+  float eta_coeff_bot, eta_slope_bot, eta_coeff_top, eta_slope_top;
+  float B_par_diff, vhyp, tvd_pm_v, tvd_pm_B, vmax_lim, CME_thresh;
+  float tvd_h[4], tvd_cs[4], tc[4];
+  float inv_va2max, vsqr_diff, cmax, idxd1, dt_tvd;
+  double eps_min;
+  int need_tvd_coeff, rho_log, is_gbeg0, is_gend0;
+  const double* hfb;
+  const double* hft;
+  const double* qft;
+  const double* tau;
+};
+
+struct MxEpilogue {
+  EosDev E;
+  float rho_min, vmax;
+  double eps_min, eps_max, inv_va2max;
+  float* Pnew;
+  unsigned long long* cmax;
 };

@@ -179,3 +179,23 @@ def relerr(a, b):
     if den == 0:
         return float(np.max(np.abs(a - b)))
     return float(np.max(np.abs(a - b)) / den)
+
+
+FIELD_NAMES = ("rho", "M0", "M1", "M2", "e", "B0", "B1", "B2")
+
+
+def field_errors(a, b, interior=None):
+    """Per-field relative L2 (||a-b||_2 / ||b||_2) and Linf (max|a-b| / max|b|)
+    errors of state a [8, n2, n1, n0] against b, optionally over a slice."""
+    out = {}
+    for v, name in enumerate(FIELD_NAMES):
+        x = np.asarray(a[v], dtype=np.float64)
+        y = np.asarray(b[v], dtype=np.float64)
+        if interior is not None:
+            x, y = x[interior], y[interior]
+        d = x - y
+        ny2 = float(np.sqrt(np.sum(y * y)))
+        nyi = float(np.max(np.abs(y)))
+        out[name] = {"l2": float(np.sqrt(np.sum(d * d))) / ny2 if ny2 > 0 else float(np.sqrt(np.sum(d * d))),
+                     "linf": float(np.max(np.abs(d))) / nyi if nyi > 0 else float(np.max(np.abs(d)))}
+    return out
