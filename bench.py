@@ -60,13 +60,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(table, phys, box_gsize, dx, steps=3):
-    """Oracle FP64 step on a bounded sample box with the same distributions."""
+def cpu_baseline(table, phys, box, steps=2):
+    """Oracle FP64 step (C restatement of the reference, OpenMP) on the same
+    workload: one warm-up step, then `steps` timed steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from muram_main_amd import config, eos_table, initial
     from orc import Oracle
     tab = eos_table.load_tables(table)
-    box = initial.BoxSpec(gsize=box_gsize, dx=dx, seed=1)
     U = initial.make_state(tab, box)
     g = config.grid_desc(box.gsize, box.dx)
     o = Oracle(g, phys, table)
@@ -79,11 +79,11 @@ def cpu_baseline(table, phys, box_gsize, dx, steps=3):
     t0 = time.perf_counter()
     o.step(r, config.DT_RAD_OFF, steps)
     dt = time.perf_counter() - t0
-    ncell = box_gsize[0] * box_gsize[1] * box_gsize[2]
+    ncell = box.gsize[0] * box.gsize[1] * box.gsize[2]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=ncell * steps / dt / 1e6, unit="Mcell-updates/s", cores=threads, kind="port",
-                sample=f"oracle FP64 step, {box_gsize[0]}x{box_gsize[1]}x{box_gsize[2]} box of the same "
-                       f"distributions, {steps} steps after 1 warm-up, {threads} OpenMP threads",
+                sample=f"oracle FP64 step on the same {box.gsize[0]}x{box.gsize[1]}x{box.gsize[2]} seeded state, "
+                       f"{steps} steps after 1 warm-up, {threads} OpenMP threads",
                 s_per_step=dt / steps)
 
 
@@ -150,11 +150,33 @@ def make_grid(wl, table, U, precision):
     return m, r
 
 
-def precision_study(wl, table, tab, nsteps):
+FIELDS = ("rho", "M0", "M1", "M2", "e", "B0", "B1", "B2")
+
+
+def _errs(A, B, sl):
+    """per-field relative L2 (||a-b|| / ||b||) and Linf (max|a-b| / max|b|) over the slice"""
+    out = {}
+    for v, name in enumerate(FIELDS):
+        a, b = A[sl][v], B[sl][v]
+        d = a - b
+        l2 = float(np.sqrt(np.sum(d * d)) / max(np.sqrt(np.sum(b * b)), 1e-300))
+        li = float(np.max(np.abs(d)) / max(np.max(np.abs(b)), 1e-300))
+        out[name] = {"l2": float(f"{l2:.3e}"), "linf": float(f"{li:.3e}")}
+    return out
+
+
+def _eint_stats(U):
+    """min internal energy e - M^2/(2 rho) over the interior and the count of cells with e_int <= 0."""
+    sl = (slice(2, -2),) * 3
+    ei = U[4][sl] - 0.5 * (U[1][sl] ** 2 + U[2][sl] ** 2 + U[3][sl] ** 2) / U[0][sl]
+    return {"min_eint": float(ei.min()), "n_nonpositive": int(np.count_nonzero(~(ei > 0)))}
+
+
+def precision_study(wl, table, tab, nsteps, control=True, positivity=False):
     """FP64 and FP32-mixed forms from one state, nsteps steps each; per-field
-    errors of the mixed state against the FP64 one over the interior."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from orc import field_errors
+    errors of the mixed state against the FP64 one over the interior, its
+    lower and upper halves, and (control) the FP64 form started from the
+    FP32-rounded state: the step's own sensitivity to storage rounding."""
     from muram_main_amd import abi, config, initial
     t = time.perf_counter()
     U = initial.make_state(tab, wl.box)
@@ -162,26 +184,46 @@ def precision_study(wl, table, tab, nsteps):
     out = {"workload": wl.name, "steps": nsteps}
     states, dts = {}, {}
     ncell = wl.box.gsize[0] * wl.box.gsize[1] * wl.box.gsize[2]
-    for name, prec in (("fp64", abi.PREC_FP64), ("fp32_mixed", abi.PREC_FP32_MIXED)):
+    runs = [("fp64", abi.PREC_FP64), ("fp32_mixed", abi.PREC_FP32_MIXED)]
+    if control:
+        runs.append(("control", abi.PREC_FP64))
+    for name, prec in runs:
+        if name == "control":
+            for v in (0, 1, 2, 3, 5, 6, 7):
+                U[v] = U[v].astype(np.float32).astype(np.float64)
         m, r = make_grid(wl, table, U, prec)
         m.synchronize()
         t0 = time.perf_counter()
         m.step(r, config.DT_RAD_OFF, nsteps)
         m.synchronize()
         wall = time.perf_counter() - t0
-        out[f"{name}_mcells"] = round(ncell * nsteps / wall / 1e6, 2)
+        if name != "control":
+            out[f"{name}_mcells"] = round(ncell * nsteps / wall / 1e6, 2)
         states[name] = m.get_state()
         dts[name] = r.dt
         m.close()
         log(f"precision study {name}: {nsteps} steps in {wall:.2f}s")
     del U
-    sl = (slice(None),) + tuple(slice(2, -2) for _ in range(3))
-    errs = field_errors(states["fp32_mixed"][sl], states["fp64"][sl])
+    n0 = wl.box.gsize[0]
+    inner = (slice(2, -2),) * 2
+    sl_all = (slice(None),) + inner + (slice(2, 2 + n0),)
+    sl_lo = (slice(None),) + inner + (slice(2, 2 + n0 // 2),)
+    sl_hi = (slice(None),) + inner + (slice(2 + n0 // 2, 2 + n0),)
+    A, B = states["fp32_mixed"], states["fp64"]
     out["speedup"] = round(out["fp32_mixed_mcells"] / out["fp64_mcells"], 3)
     out["dt_rel_diff"] = abs(dts["fp32_mixed"] / dts["fp64"] - 1.0)
-    out["errors"] = {k: {"l2": float(f"{v['l2']:.3e}"), "linf": float(f"{v['linf']:.3e}")} for k, v in errs.items()}
-    out["max_l2"] = max(v["l2"] for v in errs.values())
-    out["max_linf"] = max(v["linf"] for v in errs.values())
+    out["errors"] = _errs(A, B, sl_all)
+    out["max_l2"] = max(v["l2"] for v in out["errors"].values())
+    out["max_linf"] = max(v["linf"] for v in out["errors"].values())
+    out["errors_lower_half"] = _errs(A, B, sl_lo)
+    out["errors_upper_half"] = _errs(A, B, sl_hi)
+    out["max_l2_lower_half"] = max(v["l2"] for v in out["errors_lower_half"].values())
+    if control:
+        out["control_fp64_from_rounded_state"] = _errs(states["control"], B, sl_all)
+        out["control_max_l2"] = max(v["l2"] for v in out["control_fp64_from_rounded_state"].values())
+    if positivity:
+        out["eint_fp64"] = _eint_stats(B)
+        out["eint_fp32_mixed"] = _eint_stats(A)
     out["reference"] = "FP64 form of this library (parity with the FP64 oracle <= 1e-12)"
     return out
 
@@ -196,6 +238,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=100,
                     help="steps of the FP64 vs FP32-mixed precision study (0: skip)")
     ap.add_argument("--error-workload", default="box512_v3")
+    ap.add_argument("--corona-steps", type=int, default=10,
+                    help="steps of the coronal low-beta FP64 vs FP32-mixed run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--table-n", type=int, default=512)
     ap.add_argument("--path", default="fused", choices=["fused", "sequence"])
@@ -277,9 +321,12 @@ def main():
     if args.error_steps > 0 and "fp32" in res_forms:
         res["precision_study"] = precision_study(config.workloads()[args.error_workload], table, tab,
                                                  args.error_steps)
+    if args.corona_steps > 0 and "fp32" in res_forms:
+        res["corona_study"] = precision_study(config.workloads()["corona"], table, tab, args.corona_steps,
+                                              control=False, positivity=True)
     if not args.no_cpu:
         t = time.perf_counter()
-        res["cpu_baseline"] = cpu_baseline(table, wl.phys, (256, 128, 128), box.dx)
+        res["cpu_baseline"] = cpu_baseline(table, wl.phys, box)
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s")
     else:
         res["cpu_baseline"] = None
