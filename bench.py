@@ -228,6 +228,75 @@ def precision_study(wl, table, tab, nsteps, control=True, positivity=False):
     return out
 
 
+def adjoint_study(table, tab, steps=50, every=10, ndir=2):
+    """Adjoint of J = sum v_vertical^2 over the interior layer 64 of the 128^3
+    box after `steps` FP64 steps, checkpointed every `every` steps: forward,
+    recorded-forward and backward wall times, HBM in use, and central
+    differences along random interior directions (dt held fixed)."""
+    import torch
+    from muram_main_amd import abi, config, initial
+    wl = config.workloads()["adjoint128"]
+    U = initial.make_state(tab, wl.box)
+    free0, total = torch.cuda.mem_get_info()
+    m, r = make_grid(wl, table, U, abi.PREC_FP64)
+    p_bc, s_bc = m.bc()
+    m.step(r, config.DT_RAD_OFF, 1)
+    dt = 0.5 * r.dt
+    L = 2 + 64
+    inner = (slice(2, -2),) * 3
+
+    def run(U0, record=False):
+        m.set_state(U0)
+        m.set_bc(p_bc, s_bc)
+        rr = config.run_desc()
+        rr.autostep = 0
+        rr.dt = dt
+        m.exchange_grid()
+        m.SetBoundaryConditions(rr, 0)
+        m.synchronize()
+        t0 = time.perf_counter()
+        if record:
+            m.adjoint_forward(rr, config.DT_RAD_OFF, steps, every)
+        else:
+            m.step(rr, config.DT_RAD_OFF, steps)
+        m.synchronize()
+        return m.get_state(), time.perf_counter() - t0
+
+    def J(UN):
+        v = UN[1, 2:-2, 2:-2, L] / UN[0, 2:-2, 2:-2, L]
+        return float(np.sum(v * v))
+
+    _, t_plain = run(U)
+    UN, t_rec = run(U, record=True)
+    seed = np.zeros_like(UN)
+    v0, rho = UN[1, 2:-2, 2:-2, L], UN[0, 2:-2, 2:-2, L]
+    seed[1, 2:-2, 2:-2, L] = 2.0 * v0 / rho ** 2
+    seed[0, 2:-2, 2:-2, L] = -2.0 * v0 ** 2 / rho ** 3
+    t0 = time.perf_counter()
+    grad = m.adjoint_backward(seed, fold_initial_ghosts=True)
+    t_back = time.perf_counter() - t0
+    free1, _ = torch.cuda.mem_get_info()
+    rng = np.random.default_rng(11)
+    scale = np.array([np.sqrt(np.mean(U[q][inner] ** 2)) for q in range(8)])
+    scale[1:4] = scale[1:4].max()
+    errs = []
+    for _ in range(ndir):
+        d = np.zeros_like(U)
+        for q in range(8):
+            d[q][inner] = scale[q] * rng.standard_normal(U[q][inner].shape)
+        h = 1e-8
+        fd = (J(run(U + h * d)[0]) - J(run(U - h * d)[0])) / (2 * h)
+        ad = float(np.sum(grad[(slice(None),) + inner] * d[(slice(None),) + inner]))
+        errs.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+    m.close()
+    return {"workload": wl.name, "steps": steps, "checkpoint_every": every, "dt_s": dt,
+            "forward_s": round(t_plain, 4), "forward_recorded_s": round(t_rec, 4), "backward_s": round(t_back, 4),
+            "backward_over_forward": round(t_back / t_plain, 2),
+            "hbm_in_use_bytes": int(free0 - free1),
+            "fd_rel_diff": [float(f"{e:.3e}") for e in errs],
+            "note": "FP64 adjoint; dt, TVD branch, p_bc/s_bc held at the forward's values"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +307,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=100,
                     help="steps of the FP64 vs FP32-mixed precision study (0: skip)")
     ap.add_argument("--error-workload", default="box512_v3")
+    ap.add_argument("--adjoint-steps", type=int, default=50,
+                    help="steps of the 128^3 adjoint study (0: skip)")
     ap.add_argument("--corona-steps", type=int, default=10,
                     help="steps of the coronal low-beta FP64 vs FP32-mixed run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -324,6 +395,10 @@ def main():
     if args.corona_steps > 0 and "fp32" in res_forms:
         res["corona_study"] = precision_study(config.workloads()["corona"], table, tab, args.corona_steps,
                                               control=False, positivity=True)
+    if args.adjoint_steps > 0:
+        t = time.perf_counter()
+        res["adjoint_study"] = adjoint_study(table, tab, steps=args.adjoint_steps)
+        log(f"adjoint study in {time.perf_counter() - t:.1f}s")
     if not args.no_cpu:
         t = time.perf_counter()
         res["cpu_baseline"] = cpu_baseline(table, wl.phys, box)

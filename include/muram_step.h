@@ -170,6 +170,22 @@ int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage
  * supplies (RT is outside the path). */
 int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
 
+/* Reverse mode of mrm_step.  The reference has no adjoint; these entry points
+ * differentiate the step it defines (solver.C:109-365, as mrm_step runs it)
+ * for callers that need sensitivities of a scalar objective to the state.
+ * mrm_adjoint_forward: nsteps steps as mrm_step, keeping the state every
+ * `every` steps and each step's dt and TVD branch (FP64 contexts, va_adjust = 0,
+ * interior extents >= 4).  mrm_adjoint_backward: seed = dJ/dU after the steps
+ * (SoA, 8 x bufsize doubles, host); grad receives dJ/dU before them.  Each
+ * segment of `every` steps is recomputed from its checkpoint.  dt, the TVD
+ * branch, p_bc/s_bc, Qtot and Tau are held fixed; min/max, clamps and table
+ * cells are differentiated along the forward's branch.  With
+ * fold_initial_ghosts != 0 the initial state's ghost cells are taken as filled
+ * by mrm_exchange_grid + mrm_set_boundary_conditions, and their cotangents
+ * are folded into the interior. */
+int mrm_adjoint_forward(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps, int every);
+int mrm_adjoint_backward(mrm_ctx* ctx, const double* seed, double* grad, int fold_initial_ghosts);
+
 /* Per-kernel device timers: the counterpart of the reference's per-routine
  * timers printed at verbose > 1 (solver.C:41-43, :428-471).  When enabled,
  * every launch is bracketed by HIP events on the context's stream.

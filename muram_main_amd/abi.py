@@ -28,6 +28,7 @@ EXPORTS = [
     "mrm_cons_to_prim", "mrm_adjust_valf_max", "mrm_mhd_residual", "mrm_sync_timestep",
     "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit",
     "mrm_exchange_grid", "mrm_set_boundary_conditions", "mrm_step",
+    "mrm_adjoint_forward", "mrm_adjoint_backward",
     "mrm_timers", "mrm_get_timers", "mrm_timer_name",
 ]
 NTIMER = 24
@@ -97,6 +98,8 @@ def load(path: str = LIB_PATH):
         "mrm_exchange_grid": ([P], C.c_int),
         "mrm_set_boundary_conditions": ([P, C.POINTER(RunDesc), C.c_int], C.c_int),
         "mrm_step": ([P, C.POINTER(RunDesc), C.c_double, C.c_int], C.c_int),
+        "mrm_adjoint_forward": ([P, C.POINTER(RunDesc), C.c_double, C.c_int, C.c_int], C.c_int),
+        "mrm_adjoint_backward": ([P, dp, dp, C.c_int], C.c_int),
         "mrm_timers": ([P, C.c_int], C.c_int),
         "mrm_get_timers": ([P, dp, C.POINTER(C.c_long)], C.c_int),
         "mrm_timer_name": ([C.c_int], C.c_char_p),

@@ -19,26 +19,32 @@
 #define MRM_NS exact
 #endif
 
+// cell functions compile for the host too, so that their derivatives can be
+// checked against finite differences on the CPU
+#ifndef MRM_HD
+#define MRM_HD __host__ __device__ __forceinline__
+#endif
+
 namespace MRM_NS {
 
 
 
 
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ int imax_(int a, int b) { return (a < b) ? b : a; }
-__device__ __forceinline__ int imin_(int a, int b) { return (b < a) ? b : a; }
+MRM_HD double smax(double a, double b) { return (a < b) ? b : a; }
+MRM_HD double smin(double a, double b) { return (b < a) ? b : a; }
+MRM_HD int imax_(int a, int b) { return (a < b) ? b : a; }
+MRM_HD int imin_(int a, int b) { return (b < a) ? b : a; }
 
 // (int)x as the reference's x86-64 build evaluates it: cvttsd2si returns
 // INT_MIN for NaN and out-of-range values; the GPU conversion saturates.
-__device__ __forceinline__ int cvt_int(double x) {
+MRM_HD int cvt_int(double x) {
   return (x >= -2147483648.0 && x < 2147483648.0) ? (int)x : (int)0x80000000;
 }
 
 // ------------------------------------------------------------------ EOS
 // Bilinear lookups of eos_init.C:216-416 (scalar interpolations used by the
 // boundary condition); term order follows each reference function.
-__device__ __forceinline__ double eos_fwd_raw(const EosDev& E, int t, double ee, double dd) {
+MRM_HD double eos_fwd_raw(const EosDev& E, int t, double ee, double dd) {
   double ee1 = log(ee + E.eps_off);
   double logr = log(dd);
   int i = cvt_int((ee1 - E.xeps[0]) * E.inv_del_eps);
@@ -55,13 +61,13 @@ __device__ __forceinline__ double eos_fwd_raw(const EosDev& E, int t, double ee,
            + (E.xlr[j + 1] - logr) * (E.xeps[i + 1] - ee1) * (double)T[i * L + j];
   return v;
 }
-__device__ __forceinline__ double p_interp(const EosDev& E, double ee, double dd) {
+MRM_HD double p_interp(const EosDev& E, double ee, double dd) {
   return exp(eos_fwd_raw(E, 0, ee, dd));
 }
-__device__ __forceinline__ double s_interp(const EosDev& E, double ee, double dd) {
+MRM_HD double s_interp(const EosDev& E, double ee, double dd) {
   return eos_fwd_raw(E, 2, ee, dd) - E.ss_off;
 }
-__device__ __forceinline__ double inv_raw(const EosDev& E, int t, double pp, double ss) {
+MRM_HD double inv_raw(const EosDev& E, int t, double pp, double ss) {
   double ss1 = ss + E.ss_off;
   double logp = log(pp);
   int i = cvt_int((logp - E.xlp3[0]) * E.inv_del_lp3);
@@ -77,10 +83,10 @@ __device__ __forceinline__ double inv_raw(const EosDev& E, int t, double pp, dou
        + (E.xlp3[i + 1] - logp) * (ss1 - E.xs3[j]) * (double)T[i * L + j + 1]
        + (E.xlp3[i + 1] - logp) * (E.xs3[j + 1] - ss1) * (double)T[i * L + j];
 }
-__device__ __forceinline__ double d3_interp(const EosDev& E, double pp, double ss) {
+MRM_HD double d3_interp(const EosDev& E, double pp, double ss) {
   return exp(inv_raw(E, 1, pp, ss));
 }
-__device__ __forceinline__ double eps3_interp(const EosDev& E, double pp, double ss) {
+MRM_HD double eps3_interp(const EosDev& E, double pp, double ss) {
   double ee = exp(inv_raw(E, 0, pp, ss));
   return ee - E.eps_off;
 }
@@ -90,7 +96,7 @@ __device__ __forceinline__ double eps3_interp(const EosDev& E, double pp, double
 // p and T are produced (T decides the validity fallback of p); NOUT = 5
 // produces all five.
 template <int NOUT>
-__device__ __forceinline__ void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
+MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
                                                   double& m3, double& e, double out[5]) {
   const double eos_gamma = 1.65, eos_mu = 0.62, eos_mu_n = 1.3;
   const double mh = 1.67262158e-24, kb = 1.380622e-16;
@@ -191,7 +197,7 @@ struct PtFlux {
 };
 
 template <int D>
-__device__ __forceinline__ void point_flux(const double var[MRM_NV], double pres, PtFlux& o) {
+MRM_HD void point_flux(const double var[MRM_NV], double pres, PtFlux& o) {
   const double p1 = D == 0 ? 1.0 : 0.0, p2 = D == 1 ? 1.0 : 0.0, p3 = D == 2 ? 1.0 : 0.0;
   double var0 = var[0], var1 = var[1], var2 = var[2], var3 = var[3];
   double var4 = var[4], var5 = var[5], var6 = var[6], var7 = var[7];
@@ -216,7 +222,7 @@ __device__ __forceinline__ void point_flux(const double var[MRM_NV], double pres
 }
 
 // lf = (1+x^2)/(1+x^2+x^4), x^2 = va^2 / va_max^2 (mhdres_SR.C:319-323)
-__device__ __forceinline__ double lfac_of(const double var[MRM_NV], double inv_va2max) {
+MRM_HD double lfac_of(const double var[MRM_NV], double inv_va2max) {
   double dn = 1.0 / var[0];
   double bb = var[5] * var[5] + var[6] * var[6] + var[7] * var[7];
   double va2 = bb * dn;
@@ -227,7 +233,7 @@ __device__ __forceinline__ double lfac_of(const double var[MRM_NV], double inv_v
 }
 
 // fastest characteristic speed of one point (mhdres_SR.C:310-328)
-__device__ __forceinline__ double point_cmax(const double var[MRM_NV], double pres, double inv_va2max) {
+MRM_HD double point_cmax(const double var[MRM_NV], double pres, double inv_va2max) {
   double var0 = var[0];
   double dn = 1.0 / var0;
   double vv = var[1] * var[1] + var[2] * var[2] + var[3] * var[3];
@@ -243,7 +249,7 @@ __device__ __forceinline__ double point_cmax(const double var[MRM_NV], double pr
 }
 
 // cmax of a conservative point, primitives formed as ConsToPrim forms them
-__device__ __forceinline__ double point_cmax_cons(const double u[MRM_NV], double pres, double inv_va2max) {
+MRM_HD double point_cmax_cons(const double u[MRM_NV], double pres, double inv_va2max) {
   double w[MRM_NV];
   double dn = 1.0 / u[0];
   w[0] = u[0];
@@ -260,7 +266,7 @@ __device__ __forceinline__ double point_cmax_cons(const double u[MRM_NV], double
 // Residual contribution of sweep D at the centre point given the point data of
 // the five points c-2..c+2 along D (mhdres_SR.C:361-412).  res must hold 0.
 template <int D>
-__device__ __forceinline__ void sweep_residual(const double v[5][MRM_NV], const double p[5], double w1,
+MRM_HD void sweep_residual(const double v[5][MRM_NV], const double p[5], double w1,
                                                double w2, double inv_va2max, double res[MRM_NV]) {
   constexpr int D2 = D == 0 ? 1 : 0;
   constexpr int D3 = D == 2 ? 1 : 2;
@@ -299,7 +305,7 @@ __device__ __forceinline__ void sweep_residual(const double v[5][MRM_NV], const 
 // F(+1)-F(-1) and F(+2)-F(-2) stay live.  Every arithmetic operation is the
 // one sweep_residual (and mhdres_SR.C:361-412) performs.
 template <int D, class GET>
-__device__ __forceinline__ void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
+MRM_HD void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
                                                   double lf, double res[MRM_NV]) {
   constexpr int D2 = D == 0 ? 1 : 0;
   constexpr int D3 = D == 2 ? 1 : 2;
@@ -358,7 +364,7 @@ __device__ __forceinline__ void sweep_residual_lr(GET get, const double c[9], do
 // ------------------------------------------------------------------ sources
 
 // Tcheck clamp (Add_Sources_Integrate_SR.C:447-465)
-__device__ __forceinline__ void tcheck_vals(double u[MRM_NV], double rho_min, double eps_min, double eps_max,
+MRM_HD void tcheck_vals(double u[MRM_NV], double rho_min, double eps_min, double eps_max,
                                             double vmax) {
   double dn = fmax(rho_min, u[0]);
   double vv = sqrt(u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / dn;
@@ -377,7 +383,7 @@ __device__ __forceinline__ void tcheck_vals(double u[MRM_NV], double rho_min, do
 // Sources, Boris correction, RK update and Tcheck of one cell
 // (Add_Sources_Integrate_SR.C:331-465).  up: primitive U at the cell; R: its
 // residual (modified); u0: U0.  Returns the new conservative state in out.
-__device__ __forceinline__ void source_integrate_cell(const SrcParams& P, const double up[MRM_NV],
+MRM_HD void source_integrate_cell(const SrcParams& P, const double up[MRM_NV],
                                                       double R[MRM_NV], const double u0[MRM_NV], double Qrad,
                                                       double out[MRM_NV]) {
   double dn = P.invRdt * up[4];
@@ -426,7 +432,7 @@ struct TvdPt {
 };
 
 // per-point quantities (tvdlimit_SR.C:269-385)
-__device__ __forceinline__ void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, double tau,
+MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, double tau,
                                           double hfb1, double hft1, bool xsweep, TvdPt& o) {
   double v0 = smax(P.rho_min, u[0]);
   double v1 = u[1], v2 = u[2], v3 = u[3], v4 = u[4];
@@ -482,7 +488,7 @@ __device__ __forceinline__ void tvd_point(const TvdParams& P, const double u[MRM
 }
 
 // limited slope at point i (tvdlimit_SR.C:388-434)
-__device__ __forceinline__ void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
+MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
                                           double hfb1, double hft1, double S[MRM_NV]) {
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
@@ -512,7 +518,7 @@ __device__ __forceinline__ void tvd_slope(const TvdParams& P, const TvdPt& a, co
 // a = i-1 and d = i+2 (tvdlimit_SR.C:436-563); returns the resistive-heating
 // interface term (:609-614)
 template <int D>
-__device__ __forceinline__ double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
+MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
                                             const TvdPt& d, const double Sb[MRM_NV], const double Sc[MRM_NV],
                                             double hfb1, double hft1, double qft1, bool hyper_on,
                                             bool zero_bflux, double F[MRM_NV]) {
@@ -567,7 +573,7 @@ __device__ __forceinline__ double tvd_iface(const TvdParams& P, const TvdPt& a, 
 }
 
 // in-place update of point i from the fluxes at i-1 and i (tvdlimit_SR.C:565-657)
-__device__ __forceinline__ void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_NV],
+MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_NV],
                                            const double F[MRM_NV], double qm, double q, double qft1,
                                            double u[MRM_NV]) {
   double r[MRM_NV];

@@ -221,7 +221,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double cm[1] = {0.0};
   if (t < (long)G.n1 * G.n2) {
-    const double p1_dmp = 0.8;
     const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
     int j = (int)(t % G.n1), k = (int)(t / G.n1);
     const bool inner = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
@@ -232,37 +231,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
       double u1[MRM_NV], u2[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
-      u1[4] -= 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
-      u2[4] -= 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
-      double eps1 = u1[4] / u1[0];
-      double eps2 = u2[4] / u2[0];
-      double pmag = 0.5 * u1[5] * u1[5];
-      double pmin1 = p_interp(E, eps1, u1[0]) - pmean1;
-      double smin1 = s_interp(E, eps1, u1[0]);
-      double smin2 = s_interp(E, eps2, u2[0]);
-      double s1, s2;
-      if (u1[1] >= 0.e0) { s1 = B.s_bc; s2 = B.s_bc; }
-      else { s1 = smin1; s2 = smin2; }
-      double c = B.p_bc / sqrt(pmean1 * pmean2);
-      double p1 = pmean1 * c;
-      double p2 = pmean1 * c * c;
-      p1 += pmin1 * p1_dmp;
-      p2 += pmin1 * p1_dmp * p1_dmp;
-      p1 -= pmag * (1.0 - p1_dmp);
-      p2 -= pmag * (1.0 - p1_dmp * p1_dmp);
-      c = u1[5] / B.B_crit;
-      c = pow(c, 4.0);
-      c = (1.0 - c) / (1.0 + c);
-      u1[1] *= c; u1[2] *= c; u1[3] *= c;
-      u2[1] *= c; u2[2] *= c; u2[3] *= c;
-      double rho1 = d3_interp(E, p1, s1);
-      double rho2 = d3_interp(E, p2, s2);
-      eps1 = eps3_interp(E, p1, s1);
-      eps2 = eps3_interp(E, p2, s2);
-      u1[0] = rho1;
-      u2[0] = rho2;
-      u1[4] = eps1 * rho1 + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
-      u2[4] = eps2 * rho2 + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+      bc_bottom_cells(E, B, pmean1, pmean2, u1, u2);
       long g1 = G.lb0 - 1 + base, g2 = G.lb0 - 2 + base;
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
@@ -280,16 +249,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
       double u1[MRM_NV], u2[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
-      if (B.open_top == 1) {
-        if (u1[1] < 0.0) { u1[1] *= -1; u2[1] *= -1; }
-      } else {
-        u1[1] *= -1; u2[1] *= -1;
-      }
-      if (B.eps_top > 0.0) {
-        u1[4] = B.eps_top * u1[0] + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
-        u2[4] = B.eps_top * u2[0] + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
-      }
-      u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+      bc_top_cells(B, u1, u2);
       long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }

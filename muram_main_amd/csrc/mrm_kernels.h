@@ -332,21 +332,11 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_sum_partials(const double* __rest
 }
 
 
-// bottom ghost layers (boundary_pdmp_1_fftw3.C:240-318), all (j,k) incl. ghosts;
-// psum[0..1] are the layer sums of p written by k_sum_partials
-__global__ void __launch_bounds__(MRM_BLOCK) k_bc_bottom(Geom G, EosDev E, BcParams B, const double* __restrict__ psum,
-                                                         double* __restrict__ U) {
-  const long bs = G.bs;
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)G.n1 * G.n2) return;
+// bottom ghost cells of one (j,k) column (boundary_pdmp_1_fftw3.C:245-318):
+// in u1 = U(lbeg), u2 = U(lbeg+1); out u1 = U(lbeg-1), u2 = U(lbeg-2)
+MRM_HD void bc_bottom_cells(const EosDev& E, const BcParams& B, double pmean1, double pmean2,
+                            double u1[MRM_NV], double u2[MRM_NV]) {
   const double p1_dmp = 0.8;
-  const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
-  int j = (int)(t % G.n1), k = (int)(t / G.n1);
-  long base = j * G.s1 + k * G.s2;
-  long n1 = G.lb0 + base, n2 = n1 + 1;
-  double u1[MRM_NV], u2[MRM_NV];
-#pragma unroll
-  for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
   u1[4] -= 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
   u2[4] -= 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
   double eps1 = u1[4] / u1[0];
@@ -378,22 +368,11 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_bottom(Geom G, EosDev E, BcPar
   u2[0] = rho2;
   u1[4] = eps1 * rho1 + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
   u2[4] = eps2 * rho2 + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
-  long g1 = G.lb0 - 1 + base, g2 = G.lb0 - 2 + base;
-#pragma unroll
-  for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
 }
 
-// top ghost layers, vertical field (boundary_pdmp_1_fftw3.C:402-474, bnd_pot = 0)
-__global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double* __restrict__ U) {
-  const long bs = G.bs;
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)G.n1 * G.n2) return;
-  int j = (int)(t % G.n1), k = (int)(t / G.n1);
-  long base = j * G.s1 + k * G.s2;
-  long n1 = G.le0 + base, n2 = n1 - 1;
-  double u1[MRM_NV], u2[MRM_NV];
-#pragma unroll
-  for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+// top ghost cells of one column, vertical field (boundary_pdmp_1_fftw3.C:402-474,
+// bnd_pot = 0): in u1 = U(lend), u2 = U(lend-1); out u1 = U(lend+1), u2 = U(lend+2)
+MRM_HD void bc_top_cells(const BcParams& B, double u1[MRM_NV], double u2[MRM_NV]) {
   if (B.open_top == 1) {
     if (u1[1] < 0.0) { u1[1] *= -1; u2[1] *= -1; }
   } else {
@@ -404,6 +383,40 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double
     u2[4] = B.eps_top * u2[0] + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
   }
   u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+}
+
+// bottom ghost layers, all (j,k) incl. ghosts; psum[0..1] are the layer sums
+// of p written by k_sum_partials
+__global__ void __launch_bounds__(MRM_BLOCK) k_bc_bottom(Geom G, EosDev E, BcParams B, const double* __restrict__ psum,
+                                                         double* __restrict__ U) {
+  const long bs = G.bs;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)G.n1 * G.n2) return;
+  const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
+  int j = (int)(t % G.n1), k = (int)(t / G.n1);
+  long base = j * G.s1 + k * G.s2;
+  long n1 = G.lb0 + base, n2 = n1 + 1;
+  double u1[MRM_NV], u2[MRM_NV];
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+  bc_bottom_cells(E, B, pmean1, pmean2, u1, u2);
+  long g1 = G.lb0 - 1 + base, g2 = G.lb0 - 2 + base;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
+}
+
+// top ghost layers
+__global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double* __restrict__ U) {
+  const long bs = G.bs;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)G.n1 * G.n2) return;
+  int j = (int)(t % G.n1), k = (int)(t / G.n1);
+  long base = j * G.s1 + k * G.s2;
+  long n1 = G.le0 + base, n2 = n1 - 1;
+  double u1[MRM_NV], u2[MRM_NV];
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+  bc_top_cells(B, u1, u2);
   long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }

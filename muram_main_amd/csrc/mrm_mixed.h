@@ -476,8 +476,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double cm[1] = {0.0};
   if (t < (long)G.n1 * G.n2) {
-    const double p1_dmp = 0.8;
-    const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
     int j = (int)(t % G.n1), k = (int)(t / G.n1);
     const bool inner = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
     long base = j * G.s1 + k * G.s2;
@@ -502,34 +500,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
       double u1[MRM_NV], u2[MRM_NV];
       ld(n1, u1);
       ld(n2, u2);
-      u1[4] -= 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
-      u2[4] -= 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
-      double eps1 = u1[4] / u1[0], eps2 = u2[4] / u2[0];
-      double pmag = 0.5 * u1[5] * u1[5];
-      double pmin1 = p_interp(E, eps1, u1[0]) - pmean1;
-      double smin1 = s_interp(E, eps1, u1[0]);
-      double smin2 = s_interp(E, eps2, u2[0]);
-      double s1, s2;
-      if (u1[1] >= 0.e0) { s1 = B.s_bc; s2 = B.s_bc; }
-      else { s1 = smin1; s2 = smin2; }
-      double c = B.p_bc / sqrt(pmean1 * pmean2);
-      double p1 = pmean1 * c, p2 = pmean1 * c * c;
-      p1 += pmin1 * p1_dmp;
-      p2 += pmin1 * p1_dmp * p1_dmp;
-      p1 -= pmag * (1.0 - p1_dmp);
-      p2 -= pmag * (1.0 - p1_dmp * p1_dmp);
-      c = u1[5] / B.B_crit;
-      c = pow(c, 4.0);
-      c = (1.0 - c) / (1.0 + c);
-      u1[1] *= c; u1[2] *= c; u1[3] *= c;
-      u2[1] *= c; u2[2] *= c; u2[3] *= c;
-      double rho1 = d3_interp(E, p1, s1), rho2 = d3_interp(E, p2, s2);
-      eps1 = eps3_interp(E, p1, s1);
-      eps2 = eps3_interp(E, p2, s2);
-      u1[0] = rho1;
-      u2[0] = rho2;
-      u1[4] = eps1 * rho1 + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
-      u2[4] = eps2 * rho2 + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
+      bc_bottom_cells(E, B, psum[0] * B.cc, psum[1] * B.cc, u1, u2);
       st(G.lb0 - 1 + base, u1);
       st(G.lb0 - 2 + base, u2);
     }
@@ -538,16 +509,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
       double u1[MRM_NV], u2[MRM_NV];
       ld(n1, u1);
       ld(n2, u2);
-      if (B.open_top == 1) {
-        if (u1[1] < 0.0) { u1[1] *= -1; u2[1] *= -1; }
-      } else {
-        u1[1] *= -1; u2[1] *= -1;
-      }
-      if (B.eps_top > 0.0) {
-        u1[4] = B.eps_top * u1[0] + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
-        u2[4] = B.eps_top * u2[0] + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
-      }
-      u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+      bc_top_cells(B, u1, u2);
       st(G.le0 + 1 + base, u1);
       st(G.le0 + 2 + base, u2);
     }

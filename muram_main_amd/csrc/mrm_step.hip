@@ -8,6 +8,7 @@
 // p_bc/s_bc) and in mrm_set_state/mrm_get_state.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <algorithm>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -17,6 +18,7 @@
 #include "../../include/muram_step.h"
 #include "mrm_launch.h"
 #include "mrm_mixed_api.h"
+#include "mrm_adjoint_api.h"
 
 using namespace exact;
 
@@ -52,10 +54,20 @@ struct mrm_ctx {
   bool force_sequence = false;
   bool stage_zwin = false;                       // stage kernel variant
   bool fma = true;                               // fused step uses the contracted build
+  double last_dt = 0.0;                          // dt and TVD split of the last fused step
+  bool last_split = false;
   double* dF = nullptr;   // MRM_NFIELD arrays of bs doubles
   // FP32-mixed contexts: three mixed state buffers and two FP32 pressures in
   // place of S[], dP2 and the FP64 pressure; dR is the FP64 staging buffer
   MxState M[3] = {};
+  // adjoint: checkpoints of the recorded forward, per-step records, work buffers
+  struct AdjRec { double dt; bool split; };
+  std::vector<AdjRec> adj_rec;
+  std::vector<double*> adj_ckpt;
+  int adj_every = 0;
+  bool adj_fill0 = false;
+  double* adj_mem = nullptr;   // one allocation for the work buffers below
+  size_t adj_bytes = 0;
   void* mxmem[3] = {nullptr, nullptr, nullptr};
   float* Pm[2] = {nullptr, nullptr};
   bool qtot_set = false, tau_set = false;
@@ -286,6 +298,8 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   for (int q = 0; q < 2; q++) hipFree(c->Pm[q]);
   hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
+  for (double* p : c->adj_ckpt) hipFree(p);
+  hipFree(c->adj_mem);
   if (c->h_red) hipHostFree(c->h_red);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -883,8 +897,17 @@ static size_t tvd_x_lds(const mrm_ctx* c) { return (size_t)(3 * MRM_NV + 5) * c-
 
 // the three sweeps of TVDlimit on buffer U with pressure Pr; with epi the
 // dim-2 sweep finishes each cell (TCheck, new pressure, cmax)
+// the parameters of sweep d (tvdlimit_SR.C:41-86, 199-212)
+static int tvd_sweep_params(mrm_ctx* c, double dt_tvd, int d, TvdParams& P) {
+  int rc = tvd_params(c, dt_tvd, P);
+  if (rc) return rc;
+  P.cmax = 0.975 * c->dx[d] / dt_tvd;
+  P.idxd1 = 1. / c->dx[d];
+  return MRM_OK;
+}
+
 static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi,
-                   bool contracted = false) {
+                   bool contracted = false, int only_d = -1) {
   TvdParams P;
   int rc = tvd_params(c, dt_tvd, P);
   if (rc) return rc;
@@ -892,6 +915,7 @@ static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const
   const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
   const bool tau = P.tau != nullptr;
   for (int d = 0; d < 3; d++) {
+    if (only_d >= 0 && d != only_d) continue;
     P.cmax = 0.975 * c->dx[d] / dt_tvd;
     P.idxd1 = idx[d];
     long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
@@ -1037,6 +1061,8 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.inv_va2max = c->inv_va2max;
   X.Pnew = Pc;
   X.cmax = cmax;
+  c->last_dt = run->dt;
+  c->last_split = !(cfl <= run->CFL_tvd);
   if (cfl <= run->CFL_tvd) {
     if ((rc = tvd_buf(c, Uf, Pa, run->dt, &X, c->fma))) return rc;
   } else {
@@ -1207,6 +1233,231 @@ extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps
     run->globiter++;
     run->time += run->dt;
   }
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ adjoint
+// Reverse mode of mrm_step (FP64, fused schedule): mrm_adjoint_forward runs
+// the steps, keeping the state every `every` steps and each step's dt and
+// TVD branch; mrm_adjoint_backward walks the segments from the last one,
+// recomputes each segment's step states from its checkpoint, and applies the
+// transpose of each step from the end.  dt, the TVD branch, p_bc/s_bc, Qtot
+// and Tau are held at their forward values.
+
+// work buffers, FP64 SoA over the grid buffer
+struct AdjBufs {
+  double *X[5], *P[4], *V[6], *W, *Z, *seg, *lamA, *lamB, *lamX0, *lamP, *lamP3, *Gb, *lWc, *T, *small;
+};
+
+static int adj_buffers(mrm_ctx* c, AdjBufs& B) {
+  const size_t bs = c->G.bs, f8 = 8 * bs;
+  const size_t nT = (size_t)adj::tvd_scratch_fields();
+  const size_t n = 5 * f8 + 4 * bs + 6 * f8 + 2 * f8 + (size_t)c->adj_every * f8 + 3 * f8 + 2 * bs + 39 * bs +
+                   8 * bs + nT * bs + 64;
+  const size_t bytes = n * sizeof(double);
+  if (c->adj_bytes < bytes) {
+    hipFree(c->adj_mem);
+    c->adj_mem = nullptr;
+    c->adj_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->adj_mem, bytes));
+    c->adj_bytes = bytes;
+  }
+  double* p = c->adj_mem;
+  for (int q = 0; q < 5; q++) { B.X[q] = p; p += f8; }
+  for (int q = 0; q < 4; q++) { B.P[q] = p; p += bs; }
+  for (int q = 0; q < 6; q++) { B.V[q] = p; p += f8; }
+  B.W = p; p += f8;
+  B.Z = p; p += f8;
+  B.seg = p; p += (size_t)c->adj_every * f8;
+  B.lamA = p; p += f8;
+  B.lamB = p; p += f8;
+  B.lamX0 = p; p += f8;
+  B.lamP = p; p += bs;
+  B.lamP3 = p; p += bs;
+  B.Gb = p; p += 39 * bs;
+  B.lWc = p; p += 8 * bs;
+  B.T = p; p += nT * bs;
+  B.small = p;   // [0..7] stage layer sums, [8..9] final, [10..11] initial fill, [16..17] mean cotangents
+  return MRM_OK;
+}
+
+static StageArgs adj_stage_args(mrm_ctx* c, const mrm_run_desc* r, int stage) {
+  StageArgs A;
+  A.G = c->G;
+  A.E = c->E;
+  for (int d = 0; d < 3; d++) {
+    A.RP.w1[d] = 8. / (12. * c->dx[d]);
+    A.RP.w2[d] = -1. / (12. * c->dx[d]);
+  }
+  A.RP.inv_va2max = c->inv_va2max;
+  A.SP = src_params(c, r, stage);
+  A.ntx = A.nty = A.nkc = 0;
+  A.write_p = 0;
+  return A;
+}
+
+// forward of step n from Un, keeping the stage states, the TVD sweep inputs,
+// the pre-boundary state and the layer sums every boundary fill read; with
+// Unext, also the new state
+static int adj_recompute(mrm_ctx* c, int n, const double* Un, AdjBufs& B, double* Unext) {
+  const Geom& G = c->G;
+  const size_t f8b = 8 * (size_t)G.bs * sizeof(double);
+  mrm_run_desc r = {};
+  r.dt = c->adj_rec[n].dt;
+  const bool split = c->adj_rec[n].split;
+  int rc;
+  HIPCHK(c, hipMemcpyAsync(B.X[0], Un, f8b, hipMemcpyDeviceToDevice, c->stream));
+  {
+    unsigned nb = nblk(G.bs);
+    if (nb > 65536u) nb = 65536u;
+    k_eos_p<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->E, B.X[0], B.P[0], c->inv_va2max, nullptr);
+    LAUNCHCHK(c, "k_eos_p");
+  }
+  for (int s = 1; s <= 4; s++) {
+    double* Po = s < 4 ? B.P[s] : nullptr;
+    if ((rc = stage_launch(c, &r, s, B.X[s - 1], B.P[s - 1], B.X[0], B.X[s], Po))) return rc;
+    if ((rc = exchange_buf(c, B.X[s], Po))) return rc;
+    if ((rc = bc_buf(c, B.X[s], Po, nullptr))) return rc;
+    if ((rc = bottom_sums(c, 0, B.X[s], B.small + 2 * (s - 1)))) return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(B.W, B.X[4], f8b, hipMemcpyDeviceToDevice, c->stream));
+  const int passes = split ? 2 : 1;
+  const double dtv = split ? 0.5 * r.dt : r.dt;
+  for (int pass = 0; pass < passes; pass++) {
+    for (int d = 0; d < 3; d++) {
+      HIPCHK(c, hipMemcpyAsync(B.V[3 * pass + d], B.W, f8b, hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = tvd_buf(c, B.W, B.P[3], dtv, nullptr, c->fma, d))) return rc;
+    }
+    if (split && pass == 0 && (rc = exchange_buf(c, B.W, nullptr))) return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(B.Z, B.W, f8b, hipMemcpyDeviceToDevice, c->stream));
+  const double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
+  const double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  k_tcheck<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax, B.Z);
+  LAUNCHCHK(c, "k_tcheck");
+  if ((rc = exchange_buf(c, B.Z, nullptr))) return rc;
+  if ((rc = bottom_sums(c, 0, B.Z, B.small + 8))) return rc;
+  if (Unext) {
+    HIPCHK(c, hipMemcpyAsync(Unext, B.Z, f8b, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = bc_buf(c, Unext, nullptr, nullptr))) return rc;
+  }
+  return MRM_OK;
+}
+
+// transpose of step n: B.lamA holds the cotangent of U(n+1) on entry and of
+// U(n) on return (adj_recompute(n) must have run)
+static int adj_step_back(mrm_ctx* c, int n, AdjBufs& B) {
+  const Geom& G = c->G;
+  const size_t bs = G.bs, f8b = 8 * bs * sizeof(double);
+  hipStream_t st = c->stream;
+  mrm_run_desc r = {};
+  r.dt = c->adj_rec[n].dt;
+  const bool split = c->adj_rec[n].split;
+  const BcParams BP = bc_params(c);
+  double* lpm = B.small + 16;
+  // final ghost fill, TCheck
+  HIPCHK(c, adj::launch_bc(st, G, c->E, BP, B.small + 8, B.Z, B.lamA, c->d_part, lpm));
+  HIPCHK(c, adj::launch_exchange(st, G, B.lamA));
+  const double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
+  const double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  HIPCHK(c, adj::launch_tcheck(st, G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax, B.W, B.lamA));
+  // TVD sweeps in reverse
+  HIPCHK(c, hipMemsetAsync(B.lamP3, 0, bs * sizeof(double), st));
+  const int passes = split ? 2 : 1;
+  const double dtv = split ? 0.5 * r.dt : r.dt;
+  for (int pass = passes - 1; pass >= 0; pass--) {
+    if (split && pass == 0) HIPCHK(c, adj::launch_exchange(st, G, B.lamA));
+    for (int d = 2; d >= 0; d--) {
+      TvdParams P;
+      int rc = tvd_sweep_params(c, dtv, d, P);
+      if (rc) return rc;
+      HIPCHK(c, adj::launch_tvd(st, d, G, P, B.V[3 * pass + d], B.P[3], B.T, B.lamA, B.lamB, B.lamP3));
+      std::swap(B.lamA, B.lamB);
+    }
+  }
+  // stages 4..1
+  HIPCHK(c, hipMemsetAsync(B.lamX0, 0, f8b, st));
+  const double* Q = c->qtot_set ? c->dF + MRM_QTOT * G.bs : nullptr;
+  for (int s = 4; s >= 1; s--) {
+    HIPCHK(c, adj::launch_bc(st, G, c->E, BP, B.small + 2 * (s - 1), B.X[s], B.lamA, c->d_part, lpm));
+    HIPCHK(c, adj::launch_exchange(st, G, B.lamA));
+    double* lprev = s > 1 ? B.lamB : B.lamX0;
+    if (s > 1) HIPCHK(c, hipMemsetAsync(B.lamB, 0, f8b, st));
+    double* lP = (s - 1 == 3) ? B.lamP3 : B.lamP;
+    if (s - 1 != 3) HIPCHK(c, hipMemsetAsync(B.lamP, 0, bs * sizeof(double), st));
+    const StageArgs A = adj_stage_args(c, &r, s);
+    HIPCHK(c, adj::launch_stage(st, A, B.X[s - 1], B.P[s - 1], B.X[0], B.lamA, Q, B.Gb, B.lWc, B.lamX0, lprev, lP));
+    HIPCHK(c, adj::launch_eos(st, G, c->E, B.X[s - 1], lP, lprev));
+    if (s > 1) std::swap(B.lamA, B.lamB);
+  }
+  HIPCHK(c, hipMemcpyAsync(B.lamA, B.lamX0, f8b, hipMemcpyDeviceToDevice, st));
+  return MRM_OK;
+}
+
+extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps, int every) {
+  if (!c || !run || nsteps < 1 || every < 1) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_adjoint_forward");
+  if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
+  if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
+  if (c->force_sequence || c->phys.params[2] > 0.0)
+    return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: the adjoint follows the fused step (va_adjust = 0)");
+  for (int d = 0; d < 3; d++)
+    if (c->lsize[d] < 4) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: interior extents below 4");
+  for (double* p : c->adj_ckpt) hipFree(p);
+  c->adj_ckpt.clear();
+  c->adj_rec.clear();
+  c->adj_every = every;
+  const size_t f8b = 8 * (size_t)c->G.bs * sizeof(double);
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0) {
+      double* p = nullptr;
+      HIPCHK(c, hipMalloc(&p, f8b));
+      c->adj_ckpt.push_back(p);
+      HIPCHK(c, hipMemcpyAsync(p, c->S[c->cur], f8b, hipMemcpyDeviceToDevice, c->stream));
+    }
+    int rc = step_fused(c, run, dt_rad);
+    if (rc) return rc;
+    c->adj_rec.push_back({c->last_dt, c->last_split});
+    run->iteration++;
+    run->globiter++;
+    run->time += run->dt;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRM_OK;
+}
+
+extern "C" int mrm_adjoint_backward(mrm_ctx* c, const double* seed, double* grad, int fold_initial_ghosts) {
+  if (!c || !seed || !grad) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_adjoint_backward");
+  if (c->adj_rec.empty()) return fail(c, MRM_ERR_STATE, "mrm_adjoint_backward: no recorded forward");
+  AdjBufs B;
+  int rc = adj_buffers(c, B);
+  if (rc) return rc;
+  const size_t f8b = 8 * (size_t)c->G.bs * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync(B.lamA, seed, f8b, hipMemcpyHostToDevice, c->stream));
+  const int N = (int)c->adj_rec.size(), every = c->adj_every;
+  const int nseg = (N + every - 1) / every;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    const int n0 = sg * every, n1 = std::min(N, n0 + every);
+    double* segs = B.seg;
+    HIPCHK(c, hipMemcpyAsync(segs, c->adj_ckpt[sg], f8b, hipMemcpyDeviceToDevice, c->stream));
+    for (int n = n0; n < n1 - 1; n++)
+      if ((rc = adj_recompute(c, n, segs + (size_t)(n - n0) * 8 * c->G.bs, B,
+                              segs + (size_t)(n - n0 + 1) * 8 * c->G.bs))) return rc;
+    for (int n = n1 - 1; n >= n0; n--) {
+      if ((rc = adj_recompute(c, n, segs + (size_t)(n - n0) * 8 * c->G.bs, B, nullptr))) return rc;
+      if ((rc = adj_step_back(c, n, B))) return rc;
+    }
+  }
+  if (fold_initial_ghosts) {
+    // the initial state's ghosts came from mrm_exchange_grid + mrm_set_boundary_conditions
+    if ((rc = bottom_sums(c, 0, c->adj_ckpt[0], B.small + 10))) return rc;
+    HIPCHK(c, adj::launch_bc(c->stream, c->G, c->E, bc_params(c), B.small + 10, c->adj_ckpt[0], B.lamA, c->d_part,
+                             B.small + 16));
+    HIPCHK(c, adj::launch_exchange(c->stream, c->G, B.lamA));
+  }
+  HIPCHK(c, hipMemcpyAsync(grad, B.lamA, f8b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRM_OK;
 }
 

@@ -136,3 +136,17 @@ class MhdGrid:
 
     def step(self, run, dt_rad: float = 1.0e30, nsteps: int = 1):
         self._chk(self.lib.mrm_step(self.h, C.byref(run), dt_rad, nsteps), "step")
+
+    def adjoint_forward(self, run, dt_rad: float = 1.0e30, nsteps: int = 1, every: int = 1):
+        """nsteps steps as step(), recorded for adjoint_backward (checkpoint every `every` steps)."""
+        self._chk(self.lib.mrm_adjoint_forward(self.h, C.byref(run), dt_rad, nsteps, every), "adjoint_forward")
+
+    def adjoint_backward(self, seed: np.ndarray, fold_initial_ghosts: bool = True) -> np.ndarray:
+        """dJ/dU before the recorded steps from seed = dJ/dU after them ([8, n2, n1, n0] float64)."""
+        seed = np.ascontiguousarray(seed, dtype=np.float64)
+        assert seed.size == 8 * self.bufsize
+        grad = np.empty((8,) + self.shape, dtype=np.float64)
+        self._chk(self.lib.mrm_adjoint_backward(self.h, seed.ctypes.data_as(C.POINTER(C.c_double)),
+                                                grad.ctypes.data_as(C.POINTER(C.c_double)),
+                                                1 if fold_initial_ghosts else 0), "adjoint_backward")
+        return grad
