@@ -91,18 +91,21 @@ def gradient(prob):
 
 
 def directions(prob, k, seed):
-    """random interior directions scaled per field"""
+    """random interior directions scaled by each cell's own magnitudes"""
     rng = np.random.default_rng(seed)
-    U = prob.U
-    scale = np.empty(8)
-    for q in range(8):
-        scale[q] = np.sqrt(np.mean(U[q][prob.inner] ** 2))
-    scale[1:4] = max(scale[1:4].max(), 1e-3 * scale[0] * 1e5)
+    U, inner = prob.U, prob.inner
+    rho = U[0][inner]
+    v_rms = np.sqrt(np.mean((U[1][inner] / rho) ** 2 + (U[2][inner] / rho) ** 2 + (U[3][inner] / rho) ** 2) / 3)
+    b_rms = np.sqrt(np.mean(U[5][inner] ** 2 + U[6][inner] ** 2 + U[7][inner] ** 2) / 3)
     out = []
     for _ in range(k):
         d = np.zeros_like(U)
-        for q in range(8):
-            d[q][prob.inner] = scale[q] * rng.standard_normal(U[q][prob.inner].shape)
+        d[0][inner] = rho * rng.standard_normal(rho.shape)
+        for q in (1, 2, 3):
+            d[q][inner] = rho * v_rms * rng.standard_normal(rho.shape)
+        d[4][inner] = U[4][inner] * rng.standard_normal(rho.shape)
+        for q in (5, 6, 7):
+            d[q][inner] = b_rms * rng.standard_normal(rho.shape)
         out.append(d)
     return out
 
