@@ -37,6 +37,31 @@ __device__ __forceinline__ void load_prim(const double* __restrict__ U, const do
   w[8] = P[node];
 }
 
+// the nine values load_prim reads, as stored (conservative U, p), so that a
+// load can be issued ahead of its use; prim_of_raw then forms the primitives
+// with load_prim's arithmetic
+__device__ __forceinline__ void load_raw(const double* __restrict__ U, const double* __restrict__ P, long bs,
+                                         unsigned node, double r[ST_NF]) {
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) r[q] = ld32(U + q * bs, node);
+  r[8] = ld32(P, node);
+}
+__device__ __forceinline__ void prim_of_raw(const double r[ST_NF], double w[ST_NF]) {
+  double d = r[0];
+  double dn = 1.0 / d;
+  double m1 = r[1] * dn, m2 = r[2] * dn, m3 = r[3] * dn;
+  double vv = (m1 * m1 + m2 * m2 + m3 * m3);
+  w[0] = d;
+  w[1] = m1;
+  w[2] = m2;
+  w[3] = m3;
+  w[4] = r[4] - 0.5 * vv * d;
+  w[5] = r[5];
+  w[6] = r[6];
+  w[7] = r[7];
+  w[8] = r[8];
+}
+
 // map block index to a tile so that consecutive tiles share an XCD (blocks are
 // dealt to the 8 XCDs round robin)
 __device__ __forceinline__ long xcd_remap(long b, long nb) {
@@ -79,19 +104,27 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
   const bool hx_ok = hx_c >= 0 && hx_i < G.n0 && j < G.n1;
   const bool hy_ok = hy_r >= 0 && hy_j < G.n1 && i < G.n0;
   const bool own_ok = i < G.n0 && j < G.n1;
-  const long col = (long)i + (long)j * G.s1;
+  const unsigned col = (unsigned)i + (unsigned)j * (unsigned)G.s1;
+  const unsigned s1u = (unsigned)G.s1, s2u = (unsigned)G.s2;
 
   double w[5][ST_NF];   // planes k-2 .. k+2 of this thread's column (w[2] is the centre)
+  // with the register window, the next plane's raw values (own column k+3,
+  // the x and y halo of k+1) are loaded during the current plane's work
+  double rz[ST_NF], rhx[ST_NF], rhy[ST_NF];
   if (ZWIN) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       int kk = kbeg - 2 + q;
-      if (own_ok) load_prim(Uin, Pin, bs, col + kk * G.s2, w[q + 1]);
+      if (own_ok) { double r[ST_NF]; load_raw(Uin, Pin, bs, col + (unsigned)kk * s2u, r); prim_of_raw(r, w[q + 1]); }
       else {
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0;
       }
     }
+    // kbeg + 2 <= lend2 + 2 = n2 - 1
+    if (own_ok) load_raw(Uin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
+    if (hx_ok) load_raw(Uin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)kbeg * s2u, rhx);
+    if (hy_ok) load_raw(Uin, Pin, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)kbeg * s2u, rhy);
   }
   for (int k = kbeg; k < kend; k++) {
     if (ZWIN) {
@@ -99,9 +132,9 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
       for (int q = 0; q < 4; q++)
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
-      if (own_ok) load_prim(Uin, Pin, bs, col + (long)(k + 2) * G.s2, w[4]);
+      if (own_ok) prim_of_raw(rz, w[4]);     // plane k + 2, loaded during plane k - 1
     } else {
-      if (own_ok) load_prim(Uin, Pin, bs, col + (long)k * G.s2, w[2]);
+      if (own_ok) { double r[ST_NF]; load_raw(Uin, Pin, bs, col + (unsigned)k * s2u, r); prim_of_raw(r, w[2]); }
       else {
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) w[2][f] = 1.0;
@@ -112,15 +145,23 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
     for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][tx + 2] = w[2][f];
     if (hx_ok) {
       double h[ST_NF];
-      load_prim(Uin, Pin, bs, (long)hx_i + (long)j * G.s1 + (long)k * G.s2, h);
+      if (ZWIN) prim_of_raw(rhx, h);
+      else { double r[ST_NF]; load_raw(Uin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)k * s2u, r); prim_of_raw(r, h); }
 #pragma unroll
       for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][hx_c] = h[f];
     }
     if (hy_ok) {
       double h[ST_NF];
-      load_prim(Uin, Pin, bs, (long)i + (long)hy_j * G.s1 + (long)k * G.s2, h);
+      if (ZWIN) prim_of_raw(rhy, h);
+      else { double r[ST_NF]; load_raw(Uin, Pin, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)k * s2u, r); prim_of_raw(r, h); }
 #pragma unroll
       for (int f = 0; f < ST_NF; f++) lds[f][hy_r][tx + 2] = h[f];
+    }
+    if (ZWIN && k + 1 < kend) {
+      // k + 3 <= kend + 1 <= lend2 + 2 = n2 - 1: inside the buffer
+      if (own_ok) load_raw(Uin, Pin, bs, col + (unsigned)(k + 3) * s2u, rz);
+      if (hx_ok) load_raw(Uin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)(k + 1) * s2u, rhx);
+      if (hy_ok) load_raw(Uin, Pin, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)(k + 1) * s2u, rhy);
     }
     __syncthreads();
     if (valid) {
@@ -147,25 +188,27 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
 #pragma unroll
           for (int f = 0; f < ST_NF; f++) o[f] = w[s][f];
         } else {
-          load_prim(Uin, Pin, bs, col + (long)(k + s - 2) * G.s2, o);
+          double r[ST_NF];
+          load_raw(Uin, Pin, bs, col + (unsigned)(k + s - 2) * s2u, r);
+          prim_of_raw(r, o);
         }
       };
       sweep_residual_lr<2>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
       // sources, Boris, RK update, Tcheck
-      const long node = col + (long)k * G.s2;
+      const unsigned node = col + (unsigned)k * s2u;
       double u0[MRM_NV], out[MRM_NV];
 #pragma unroll
-      for (int q = 0; q < MRM_NV; q++) u0[q] = U0[q * bs + node];
-      double Qrad = Qtot ? Qtot[node] : 0.0;
+      for (int q = 0; q < MRM_NV; q++) u0[q] = ld32(U0 + q * bs, node);
+      double Qrad = Qtot ? ld32(Qtot, node) : 0.0;
       source_integrate_cell(A.SP, w[2], acc, u0, Qrad, out);
 #pragma unroll
-      for (int q = 0; q < MRM_NV; q++) Uout[q * bs + node] = out[q];
+      for (int q = 0; q < MRM_NV; q++) st32(Uout + q * bs, node, out[q]);
       if (A.write_p) {
         double m1 = out[1], m2 = out[2], m3 = out[3], e = out[4], pr[5];
         cons_to_prim_cell<2>(A.E, out[0], m1, m2, m3, e, pr);
-        Pout[node] = pr[0];
+        st32(Pout, node, pr[0]);
       }
     }
     __syncthreads();
@@ -302,7 +345,10 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p(Geom G, EosDev E, const dou
 // thread t owns point t of the line; the per-point, per-slope, per-interface
 // and per-update arithmetic is tvd_point / tvd_slope / tvd_iface / tvd_update
 
-__global__ void __launch_bounds__(TX_MAXN) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
+// MAXT: the block size bound (lines of up to MAXT points); a 512 bound
+// removes the register spill but measured slower on box512 lines
+template <int MAXT>
+__global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
                                                         const double* __restrict__ Pr) {
   extern __shared__ double sm[];
   const Geom& G = A.G;

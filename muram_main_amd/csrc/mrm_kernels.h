@@ -10,6 +10,19 @@ namespace MRM_NS {
 
 // ---------------------------------------------------------------- helpers
 // non-negative doubles order like their bit patterns: max via integer atomics
+// Loads and stores at a 32-bit byte offset from a wave-uniform base pointer:
+// the address is base + zext(offset), which the compiler issues as an
+// SGPR-base + VGPR-offset access instead of 64-bit address arithmetic per
+// lane.  The host keeps 8 * bufsize below 2^32.
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, unsigned n) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)(n * (unsigned)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, unsigned n, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)(n * (unsigned)sizeof(T))) = v;
+}
+
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* slot, double v) {
   if (v > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
 }

@@ -16,7 +16,7 @@ inline hipError_t launch_stage(hipStream_t st, unsigned nb, bool zwin, const Sta
 
 inline hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const TvdLineArgs& A,
                                      double* U, const double* Pr) {
-  k_tvd_x_lines<<<nl, bsz, lds, st>>>(A, U, Pr);
+  k_tvd_x_lines<TX_MAXN><<<nl, bsz, lds, st>>>(A, U, Pr);
   return hipGetLastError();
 }
 

@@ -21,19 +21,6 @@ __device__ __forceinline__ int mx_slot(int q) { return q < 4 ? q : q - 1; }
 __device__ __forceinline__ float fsmax(float a, float b) { return (a < b) ? b : a; }
 __device__ __forceinline__ float fsmin(float a, float b) { return (b < a) ? b : a; }
 
-// Loads and stores at a 32-bit byte offset from a wave-uniform base pointer:
-// the address is base + zext(offset), which the compiler issues as an
-// SGPR-base + VGPR-offset access instead of 64-bit address arithmetic per
-// lane.  The host keeps 8 * bufsize below 2^32 for the mixed form.
-template <class T>
-__device__ __forceinline__ T ld32(const T* base, unsigned n) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)(n * (unsigned)sizeof(T)));
-}
-template <class T>
-__device__ __forceinline__ void st32(T* base, unsigned n, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)(n * (unsigned)sizeof(T))) = v;
-}
-
 // FP32 reciprocal, quotient and square root from the hardware estimates
 // (v_rcp_f32, v_sqrt_f32: about 1 ulp), adequate for the FP32 stencils
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }

@@ -225,9 +225,9 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   hipError_t e;
   if ((e = hipStreamCreate(&c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipStreamCreate", e);
   const bool mixed_prec = precision == MRM_PREC_FP32_MIXED;
-  // the mixed kernels address cells with 32-bit byte offsets (8-byte energy)
-  if (mixed_prec && 8 * bs >= (size_t(1) << 32)) {
-    fail(c, MRM_ERR_UNSUPPORTED, "FP32-mixed form: at most 2^29 cells per buffer");
+  // the step kernels address cells with 32-bit byte offsets of 8-byte values
+  if (8 * bs >= (size_t(1) << 32)) {
+    fail(c, MRM_ERR_UNSUPPORTED, "at most 2^29 cells per buffer");
     fprintf(stderr, "mrm_create: %s\n", c->err.c_str());
     mrm_destroy(c);
     return MRM_ERR_UNSUPPORTED;
