@@ -261,15 +261,19 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
                                                         double inv_va2max, unsigned long long* cmax_out) {
   __shared__ double lds[MRM_BLOCK / 64];
   const long bs = G.bs;
+  // one thread per (column, side): bottom and top run in separate threads
+  const long ncol = (long)G.n1 * G.n2;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double cm[1] = {0.0};
-  if (t < (long)G.n1 * G.n2) {
+  const bool top = t >= ncol;
+  if (top) t -= ncol;
+  if (t < ncol) {
     const double pmean1 = psum[0] * B.cc, pmean2 = psum[1] * B.cc;
     int j = (int)(t % G.n1), k = (int)(t / G.n1);
     const bool inner = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
     long base = j * G.s1 + k * G.s2;
     // ---- bottom (boundary_pdmp_1_fftw3.C:240-318)
-    {
+    if (!top) {
       long n1 = G.lb0 + base, n2 = n1 + 1;
       double u1[MRM_NV], u2[MRM_NV];
 #pragma unroll
@@ -287,7 +291,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
       }
     }
     // ---- top (boundary_pdmp_1_fftw3.C:402-474), vertical field
-    {
+    else {
       long n1 = G.le0 + base, n2 = n1 - 1;
       double u1[MRM_NV], u2[MRM_NV];
 #pragma unroll

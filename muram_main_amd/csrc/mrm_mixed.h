@@ -39,23 +39,41 @@ __device__ __forceinline__ double drcp(double x) {
 // ------------------------------------------------------------------ point data
 // primitive point (rho, v, e_int, B, p) in FP32 from the mixed state; e_int
 // recovered in FP64
-__device__ __forceinline__ void mx_load_prim(const MxState& S, const float* __restrict__ P, long bs, unsigned n,
-                                             float w[ST_NF]) {
-  float d = ld32(S.f, n);
-  float m0 = ld32(S.f + bs, n), m1 = ld32(S.f + 2 * bs, n), m2 = ld32(S.f + 3 * bs, n);
+// the stored values of one point (rho, M, B, p in FP32; e in FP64), so that
+// the load can be issued ahead of its use
+struct MxRaw {
+  float f[8];   // rho, M0, M1, M2, B0, B1, B2, p
+  double e;
+};
+__device__ __forceinline__ void mx_load_raw(const MxState& S, const float* __restrict__ P, long bs, unsigned n,
+                                            MxRaw& r) {
+#pragma unroll
+  for (int q = 0; q < 7; q++) r.f[q] = ld32(S.f + q * bs, n);
+  r.f[7] = ld32(P, n);
+  r.e = ld32(S.e, n);
+}
+// primitive point (rho, v, e_int, B, p) in FP32, e_int recovered in FP64
+__device__ __forceinline__ void mx_prim_of_raw(const MxRaw& r, float w[ST_NF]) {
+  const float d = r.f[0], m0 = r.f[1], m1 = r.f[2], m2 = r.f[3];
   double dd = (double)d;
   double mm = (double)m0 * (double)m0 + (double)m1 * (double)m1 + (double)m2 * (double)m2;
-  double ei = ld32(S.e, n) - 0.5 * mm * drcp(dd);
+  double ei = r.e - 0.5 * mm * drcp(dd);
   float dn = frcp(d);
   w[0] = d;
   w[1] = m0 * dn;
   w[2] = m1 * dn;
   w[3] = m2 * dn;
   w[4] = (float)ei;
-  w[5] = ld32(S.f + 4 * bs, n);
-  w[6] = ld32(S.f + 5 * bs, n);
-  w[7] = ld32(S.f + 6 * bs, n);
-  w[8] = ld32(P, n);
+  w[5] = r.f[4];
+  w[6] = r.f[5];
+  w[7] = r.f[6];
+  w[8] = r.f[7];
+}
+__device__ __forceinline__ void mx_load_prim(const MxState& S, const float* __restrict__ P, long bs, unsigned n,
+                                             float w[ST_NF]) {
+  MxRaw r;
+  mx_load_raw(S, P, bs, n, r);
+  mx_prim_of_raw(r, w);
 }
 
 template <int D>
@@ -278,25 +296,37 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
       for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0f;
     }
   }
+  // the next plane's values (own column k+3, x and y halo of k+1) are loaded
+  // during the current plane's work; kbeg + 2 <= lend2 + 2 = n2 - 1
+  MxRaw rz, rhx, rhy;
+  if (own_ok) mx_load_raw(Sin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
+  if (hx_ok) mx_load_raw(Sin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)kbeg * s2u, rhx);
+  if (hy_ok) mx_load_raw(Sin, Pin, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)kbeg * s2u, rhy);
   for (int k = kbeg; k < kend; k++) {
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
-    if (own_ok) mx_load_prim(Sin, Pin, bs, col + (unsigned)(k + 2) * s2u, w[4]);
+    if (own_ok) mx_prim_of_raw(rz, w[4]);
 #pragma unroll
     for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][tx + 2] = w[2][f];
     if (hx_ok) {
       float h[ST_NF];
-      mx_load_prim(Sin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)k * s2u, h);
+      mx_prim_of_raw(rhx, h);
 #pragma unroll
       for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][hx_c] = h[f];
     }
     if (hy_ok) {
       float h[ST_NF];
-      mx_load_prim(Sin, Pin, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)k * s2u, h);
+      mx_prim_of_raw(rhy, h);
 #pragma unroll
       for (int f = 0; f < ST_NF; f++) lds[f][hy_r][tx + 2] = h[f];
+    }
+    if (k + 1 < kend) {
+      // k + 3 <= kend + 1 <= lend2 + 2 = n2 - 1
+      if (own_ok) mx_load_raw(Sin, Pin, bs, col + (unsigned)(k + 3) * s2u, rz);
+      if (hx_ok) mx_load_raw(Sin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)(k + 1) * s2u, rhx);
+      if (hy_ok) mx_load_raw(Sin, Pin, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)(k + 1) * s2u, rhy);
     }
     __syncthreads();
     if (valid) {
@@ -460,9 +490,14 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
                                                      unsigned long long* cmax_out) {
   __shared__ double lds[MRM_BLOCK / 64];
   const long bs = G.bs;
+  // one thread per (column, side): the bottom's inverse-EOS chain and the top
+  // run in separate threads
+  const long ncol = (long)G.n1 * G.n2;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   double cm[1] = {0.0};
-  if (t < (long)G.n1 * G.n2) {
+  const bool top = t >= ncol;
+  if (top) t -= ncol;
+  if (t < ncol) {
     int j = (int)(t % G.n1), k = (int)(t / G.n1);
     const bool inner = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
     long base = j * G.s1 + k * G.s2;
@@ -482,7 +517,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
           cm[0] = smax(cm[0], mx_cmax(f0, f1, f2, f3, (float)u[5], (float)u[6], (float)u[7], p, inv_va2max));
       }
     };
-    {
+    if (!top) {
       long n1 = G.lb0 + base, n2 = n1 + 1;
       double u1[MRM_NV], u2[MRM_NV];
       ld(n1, u1);
@@ -490,8 +525,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
       bc_bottom_cells(E, B, psum[0] * B.cc, psum[1] * B.cc, u1, u2);
       st(G.lb0 - 1 + base, u1);
       st(G.lb0 - 2 + base, u2);
-    }
-    {
+    } else {
       long n1 = G.le0 + base, n2 = n1 - 1;
       double u1[MRM_NV], u2[MRM_NV];
       ld(n1, u1);
@@ -779,14 +813,33 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
     MxTvdPt W0, W1, W2, W3;
     float S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
     float qm = 0.0f;
+    // the stored values of points s-2, s-1 stay in registers for the update
+    // of point s-2; point s+1 is loaded while point s is processed
+    float u1[MRM_NV], u2[MRM_NV], un[MRM_NV];
+    double e1 = 0.0, e2 = 0.0, en;
+    float pn_ = 0.0f, tn = 0.0f;
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) { u1[q] = 0.0f; u2[q] = 0.0f; }
+    {
+      const unsigned n0_ = (unsigned)base;
+      mx_load_cons(S, bs, n0_, un, en);
+      pn_ = ld32(Pr, n0_);
+      tn = HAS_TAU ? (float)ld32(P.tau, n0_) : 0.0f;
+    }
     for (int s = 0; s < sz; s++) {
-      const unsigned node = (unsigned)base + (unsigned)s * (unsigned)str;
       float u[MRM_NV];
-      double e;
-      mx_load_cons(S, bs, node, u, e);
-      float tau = HAS_TAU ? (float)ld32(P.tau, node) : 0.0f;
+      double e = en;
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) u[q] = un[q];
+      const float pres = pn_, tau = tn;
+      if (s + 1 < sz) {
+        const unsigned nn = (unsigned)base + (unsigned)(s + 1) * (unsigned)str;
+        mx_load_cons(S, bs, nn, un, en);
+        pn_ = ld32(Pr, nn);
+        if (HAS_TAU) tn = (float)ld32(P.tau, nn);
+      }
       float hfb1 = D == 0 ? (float)P.hfb[s] : lhfb, hft1 = D == 0 ? (float)P.hft[s] : lhft;
-      mx_tvd_point(P, u, e, ld32(Pr, node), tau, hfb1, hft1, D == 0, W3);
+      mx_tvd_point(P, u, e, pres, tau, hfb1, hft1, D == 0, W3);
       if (s >= 2) {
         const int ip = s - 1;
 #pragma unroll
@@ -802,8 +855,9 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
         if (ii >= gh && ii <= sz - gh - 1) {
           const unsigned nd = (unsigned)base + (unsigned)ii * (unsigned)str;
           float uu[MRM_NV];
-          double ee;
-          mx_load_cons(S, bs, nd, uu, ee);
+          double ee = e2;
+#pragma unroll
+          for (int k = 0; k < MRM_NV; k++) uu[k] = u2[k];
           mx_tvd_update(P, W1, Fm, F, qm, q, iq, uu, ee);
           if (EPI) {
             mx_tcheck(uu, ee, X.rho_min, X.eps_min, X.eps_max, X.vmax);
@@ -818,6 +872,10 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
         qm = q;
       }
       W0 = W1; W1 = W2; W2 = W3;
+#pragma unroll
+      for (int k = 0; k < MRM_NV; k++) { u2[k] = u1[k]; u1[k] = u[k]; }
+      e2 = e1;
+      e1 = e;
     }
   }
   if (EPI) {

@@ -784,7 +784,7 @@ static int bc_buf(mrm_ctx* c, double* U, double* P, unsigned long long* cmax) {
   const long nyz = (long)c->G.n1 * c->G.n2;
   if (P) {
     TScope ts(c, T_BCBOT);
-    k_bc_fused<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, U, P, c->inv_va2max, cmax);
+    k_bc_fused<<<nblk(2 * nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->E, B, dsum, U, P, c->inv_va2max, cmax);
     LAUNCHCHK(c, "k_bc_fused");
   } else {
     {
@@ -818,7 +818,7 @@ static int bc_mx(mrm_ctx* c, MxState M, float* P, unsigned long long* cmax) {
   BcParams B = bc_params(c);
   const long nyz = (long)c->G.n1 * c->G.n2;
   TScope ts(c, T_BCBOT);
-  HIPCHK(c, mixed::launch_bc(c->stream, nblk(nyz), c->G, c->E, B, dsum, M, P, c->inv_va2max, cmax));
+  HIPCHK(c, mixed::launch_bc(c->stream, nblk(2 * nyz), c->G, c->E, B, dsum, M, P, c->inv_va2max, cmax));
   return MRM_OK;
 }
 
