@@ -170,6 +170,21 @@ int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage
  * supplies (RT is outside the path). */
 int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
 
+/* Restart files of one rank holding the whole grid, host side (no context):
+ * result_prim_<v>.<iter> (OutputSolution / RestoreSolution, io_xysl.C:99-209,
+ * 272-412), Header.<iter> (WriteHeaderFile, io_xysl.C:39-48) and backup.dat
+ * (Write/ReadBackupFile, boundary_pdmp_1_fftw3.C:44-62).  dir includes the
+ * trailing '/', resfile is Run.resfile (e.g. "result").  U is SoA FP64
+ * conservative, 8 x bufsize with ghosts; reading zeroes the ghosts. */
+int mrm_io_read_prim(const char* dir, const char* resfile, int iter, const mrm_grid_desc* grid, double* U);
+int mrm_io_write_prim(const char* dir, const char* resfile, int iter, const mrm_grid_desc* grid, const double* U);
+int mrm_io_write_header(const char* dir, int iter, const mrm_grid_desc* grid, double time, double dt,
+                        double va_max);
+int mrm_io_read_header(const char* dir, int iter, int gsize[3], double dx[3], double* time, double* dt,
+                       double* va_max);
+int mrm_io_write_backup(const char* file, int iter, double time, double p_bc, double s_bc);
+int mrm_io_read_backup(const char* file, int* iter, double* time, double* p_bc, double* s_bc);
+
 /* Reverse mode of mrm_step.  The reference has no adjoint; these entry points
  * differentiate the step it defines (solver.C:109-365, as mrm_step runs it)
  * for callers that need sensitivities of a scalar objective to the state.

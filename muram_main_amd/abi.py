@@ -28,6 +28,8 @@ EXPORTS = [
     "mrm_cons_to_prim", "mrm_adjust_valf_max", "mrm_mhd_residual", "mrm_sync_timestep",
     "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit",
     "mrm_exchange_grid", "mrm_set_boundary_conditions", "mrm_step",
+    "mrm_io_read_prim", "mrm_io_write_prim", "mrm_io_write_header", "mrm_io_read_header",
+    "mrm_io_write_backup", "mrm_io_read_backup",
     "mrm_adjoint_forward", "mrm_adjoint_backward",
     "mrm_timers", "mrm_get_timers", "mrm_timer_name",
 ]
@@ -98,6 +100,13 @@ def load(path: str = LIB_PATH):
         "mrm_exchange_grid": ([P], C.c_int),
         "mrm_set_boundary_conditions": ([P, C.POINTER(RunDesc), C.c_int], C.c_int),
         "mrm_step": ([P, C.POINTER(RunDesc), C.c_double, C.c_int], C.c_int),
+        "mrm_io_read_prim": ([C.c_char_p, C.c_char_p, C.c_int, C.POINTER(GridDesc), dp], C.c_int),
+        "mrm_io_write_prim": ([C.c_char_p, C.c_char_p, C.c_int, C.POINTER(GridDesc), dp], C.c_int),
+        "mrm_io_write_header": ([C.c_char_p, C.c_int, C.POINTER(GridDesc), C.c_double, C.c_double, C.c_double],
+                                C.c_int),
+        "mrm_io_read_header": ([C.c_char_p, C.c_int, C.POINTER(C.c_int), dp, dp, dp, dp], C.c_int),
+        "mrm_io_write_backup": ([C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
+        "mrm_io_read_backup": ([C.c_char_p, C.POINTER(C.c_int), dp, dp, dp], C.c_int),
         "mrm_adjoint_forward": ([P, C.POINTER(RunDesc), C.c_double, C.c_int, C.c_int], C.c_int),
         "mrm_adjoint_backward": ([P, dp, dp, C.c_int], C.c_int),
         "mrm_timers": ([P, C.c_int], C.c_int),

@@ -24,7 +24,8 @@ FLAGS = [
 # FP32-mixed kernels (mrm_mixed.hip) are contracted; the adjoint kernels
 # (mrm_adjoint.hip) are not
 UNITS = [("mrm_step.hip", "-ffp-contract=off"), ("mrm_fast.hip", "-ffp-contract=fast"),
-         ("mrm_mixed.hip", "-ffp-contract=fast"), ("mrm_adjoint.hip", "-ffp-contract=off")]
+         ("mrm_mixed.hip", "-ffp-contract=fast"), ("mrm_adjoint.hip", "-ffp-contract=off"),
+         ("mrm_io.cpp", "-ffp-contract=off")]
 
 
 def _run(cmd, verbose):
@@ -39,7 +40,7 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     objs = []
     for src, contract in UNITS:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         _run([HIPCC] + FLAGS + [contract, "-I", CSRC, "-c", "-o", obj, os.path.join(CSRC, src)], verbose)
         objs.append(obj)
     tmp = OUT + ".tmp"

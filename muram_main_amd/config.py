@@ -64,6 +64,19 @@ def stratified_phys(va_max: float = 3.0e8, va_adjust: float = 0.0) -> PhysDesc:
     return p
 
 
+def test3d_phys(va_adjust: float = 1.0) -> PhysDesc:
+    """TEST/Test_3D/parameters.dat with Spitzer conduction, damping, divB
+    cleaning and the potential-field top boundary off (outside the path):
+    va_max 1e7 with va_adjust 1 (:51-52), half-open top (:60), B_crit 2.5e30
+    (:63), hot plate eps_top 2e14 (:64), tchk eps_min 1.75e12, rho_min 1e-18,
+    eps_max 2e14, vmax 5e7 (:99-102)."""
+    p = stratified_phys(va_max=1.0e7, va_adjust=va_adjust)
+    p.bnd[I_BND_BCRIT] = 2.5e30
+    p.bnd[I_BND_EPS_TOP] = 2.0e14
+    p.tchk[:] = [1.75e12, 1.0e-18, 2.0e14, 5.0e7]
+    return p
+
+
 def grid_desc(gsize, dx, ghosts: int = 2, periods=(0, 1, 1)) -> GridDesc:
     g = GridDesc()
     for d in range(3):

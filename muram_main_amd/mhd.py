@@ -137,6 +137,26 @@ class MhdGrid:
     def step(self, run, dt_rad: float = 1.0e30, nsteps: int = 1):
         self._chk(self.lib.mrm_step(self.h, C.byref(run), dt_rad, nsteps), "step")
 
+    # ------------------------------------------------------------ restart files
+    def RestoreSolution(self, dir_3d: str, resfile: str, backfile: str, run):
+        """RestoreSolution (io_xysl.C:272-412): read backup.dat (globiter, time,
+        p_bc, s_bc), then result_prim_<v>.<globiter>; the state goes to the device."""
+        it, t, p_bc, s_bc = read_backup(backfile)
+        run.globiter = it
+        run.time = t
+        self.set_bc(p_bc, s_bc)
+        self.set_state(read_prim(dir_3d, resfile, it, self.grid, self.shape))
+
+    def BackupSolution(self, dir_3d: str, resfile: str, backfile: str, run):
+        """BackupSolution (io_xysl.C:224-270): result_prim_<v>.<globiter>,
+        backup.dat and Header.<globiter> from the device state."""
+        write_prim(dir_3d, resfile, run.globiter, self.grid, self.get_state())
+        p_bc, s_bc = self.bc()
+        write_backup(backfile, run.globiter, run.time, p_bc, s_bc)
+        inv_va2max = self.limits()[0]
+        va_max = 1.0 / np.sqrt(inv_va2max) if inv_va2max > 0 else 0.0
+        write_header(dir_3d, run.globiter, self.grid, run.time, run.dt, va_max)
+
     def adjoint_forward(self, run, dt_rad: float = 1.0e30, nsteps: int = 1, every: int = 1):
         """nsteps steps as step(), recorded for adjoint_backward (checkpoint every `every` steps)."""
         self._chk(self.lib.mrm_adjoint_forward(self.h, C.byref(run), dt_rad, nsteps, every), "adjoint_forward")
@@ -150,3 +170,52 @@ class MhdGrid:
                                                 grad.ctypes.data_as(C.POINTER(C.c_double)),
                                                 1 if fold_initial_ghosts else 0), "adjoint_backward")
         return grad
+
+
+# ---------------------------------------------------------------- restart files
+def _chk_io(rc, what):
+    if rc != 0:
+        raise MrmError(f"{what}: {abi.ERRORS.get(rc, rc)}")
+
+
+def read_prim(dir_3d: str, resfile: str, it: int, grid: abi.GridDesc, shape) -> np.ndarray:
+    """result_prim_<v>.<it> -> conservative SoA state [8, n2, n1, n0] (ghosts zero)."""
+    lib = abi.load()
+    U = np.empty((8,) + tuple(shape), dtype=np.float64)
+    _chk_io(lib.mrm_io_read_prim(dir_3d.encode(), resfile.encode(), it, C.byref(grid),
+                                 U.ctypes.data_as(C.POINTER(C.c_double))), "read_prim")
+    return U
+
+
+def write_prim(dir_3d: str, resfile: str, it: int, grid: abi.GridDesc, U: np.ndarray):
+    lib = abi.load()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    _chk_io(lib.mrm_io_write_prim(dir_3d.encode(), resfile.encode(), it, C.byref(grid),
+                                  U.ctypes.data_as(C.POINTER(C.c_double))), "write_prim")
+
+
+def write_header(dir_3d: str, it: int, grid: abi.GridDesc, time: float, dt: float, va_max: float):
+    _chk_io(abi.load().mrm_io_write_header(dir_3d.encode(), it, C.byref(grid), time, dt, va_max), "write_header")
+
+
+def read_header(dir_3d: str, it: int):
+    """-> (gsize, dx, time, dt, va_max)"""
+    g = (C.c_int * 3)()
+    dx = (C.c_double * 3)()
+    t, dt, va = C.c_double(), C.c_double(), C.c_double()
+    _chk_io(abi.load().mrm_io_read_header(dir_3d.encode(), it, g, dx, C.byref(t), C.byref(dt), C.byref(va)),
+            "read_header")
+    return tuple(g), tuple(dx), t.value, dt.value, va.value
+
+
+def write_backup(path: str, it: int, time: float, p_bc: float, s_bc: float):
+    _chk_io(abi.load().mrm_io_write_backup(path.encode(), it, time, p_bc, s_bc), "write_backup")
+
+
+def read_backup(path: str):
+    """-> (globiter, time, p_bc, s_bc); p_bc, s_bc = -1 when the file holds only two numbers"""
+    it = C.c_int()
+    t, p, s_ = C.c_double(), C.c_double(), C.c_double()
+    _chk_io(abi.load().mrm_io_read_backup(path.encode(), C.byref(it), C.byref(t), C.byref(p), C.byref(s_)),
+            "read_backup")
+    return it.value, t.value, p.value, s_.value
