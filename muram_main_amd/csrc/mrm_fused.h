@@ -315,34 +315,73 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
   }
 }
 
-// pressure (and optionally cmax over the interior and the dim-0 ghost faces)
-// of a conservative state over the whole buffer, U left untouched
+// cell n of the buffer: interior, or a ghost cell on an interior line of its
+// own direction (the points whose speeds MHD_Residual's sweeps reduce)
+__device__ __forceinline__ void cmax_cell_class(const Geom& G, long n, bool& interior, bool& visited) {
+  int i = (int)(n % G.n0);
+  long r = n / G.n0;
+  int j = (int)(r % G.n1), k = (int)(r / G.n1);
+  bool ii = i >= G.lb0 && i <= G.le0, ji = j >= G.lb1 && j <= G.le1, ki = k >= G.lb2 && k <= G.le2;
+  interior = ii && ji && ki;
+  visited = interior || (!ii && ji && ki) || (ii && !ji && ki) || (ii && ji && !ki);
+}
+
+// pressure (and optionally cmax over the interior and the ghost faces, and
+// the Adjust_Valf_Max statistics of the interior) of a conservative state
+// over the whole buffer, U left untouched
 __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p(Geom G, EosDev E, const double* __restrict__ U,
                                                      double* __restrict__ P, double inv_va2max,
-                                                     unsigned long long* cmax_out) {
-  __shared__ double lds[MRM_BLOCK / 64];
+                                                     unsigned long long* cmax_out, unsigned long long* valf = nullptr,
+                                                     double vlim95 = 0.0, double elim95 = 0.0) {
+  __shared__ double lds[5 * MRM_BLOCK / 64];
   const long bs = G.bs;
   double cm[1] = {0.0};
+  ValfAcc va;
   for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
     double u[MRM_NV];
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + n];
-    double p = ghost_pres(E, u);
+    double m1 = u[1], m2 = u[2], m3 = u[3], ei = u[4], pr[5];
+    cons_to_prim_cell<2>(E, u[0], m1, m2, m3, ei, pr);
+    const double p = pr[0];
     P[n] = p;
-    if (cmax_out) {
-      int i = (int)(n % G.n0);
-      long r = n / G.n0;
-      int j = (int)(r % G.n1), k = (int)(r / G.n1);
-      bool ii = i >= G.lb0 && i <= G.le0, ji = j >= G.lb1 && j <= G.le1, ki = k >= G.lb2 && k <= G.le2;
-      // interior, or a ghost cell on an interior line of its own direction
-      bool vis = (ii && ji && ki) || (!ii && ji && ki) || (ii && !ji && ki) || (ii && ji && !ki);
-      if (vis) cm[0] = smax(cm[0], point_cmax_cons(u, p, inv_va2max));
+    if (cmax_out || valf) {
+      bool inner, vis;
+      cmax_cell_class(G, n, inner, vis);
+      if (cmax_out && vis) cm[0] = smax(cm[0], point_cmax_cons(u, p, inv_va2max));
+      if (valf && inner) valf_cell(va, u[0], m1, m2, m3, ei, p, vlim95, elim95);
     }
   }
   if (cmax_out) {
     block_reduce<OpMax, 1>(cm, lds);
     if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
   }
+  if (valf) {
+    __syncthreads();
+    valf_flush(va, lds, valf);
+  }
+}
+
+// cmax alone over the same points, from a state and its pressure: the
+// stage-1 speed bound after Adjust_Valf_Max has changed inv_va2max
+__global__ void __launch_bounds__(MRM_BLOCK) k_cmax_visited(Geom G, const double* __restrict__ U,
+                                                            const double* __restrict__ P, double inv_va2max,
+                                                            unsigned long long* cmax_out) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const long bs = G.bs;
+  double cm[1] = {0.0};
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    bool inner, vis;
+    cmax_cell_class(G, n, inner, vis);
+    if (vis) {
+      double u[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + n];
+      cm[0] = smax(cm[0], point_cmax_cons(u, P[n], inv_va2max));
+    }
+  }
+  block_reduce<OpMax, 1>(cm, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
 }
 
 // ------------------------------------------------------------------ TVD, dim-0 sweep, one block per line

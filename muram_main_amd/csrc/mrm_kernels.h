@@ -119,6 +119,39 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_valf_reduce(Geom G, const double*
   }
 }
 
+// Adjust_Valf_Max's per-cell terms (Adjust_Valf_Max.C:58-81) of one interior
+// cell from ConsToPrim's primitive values, accumulated per thread: max |v|,
+// max (5/3) p/rho, max e_int/rho, and the counts of |v| >= 0.95 v_lim and of
+// e_int/rho >= 0.95 e_lim (kept as doubles: exact integers)
+struct ValfAcc {
+  double m[3] = {0.0, 0.0, 0.0};
+  double n[2] = {0.0, 0.0};
+};
+__device__ __forceinline__ void valf_cell(ValfAcc& a, double d, double v1, double v2, double v3, double eint,
+                                          double p, double vlim95, double elim95) {
+  const double vv = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+  const double ee = eint / d;
+  a.m[0] = fmax(a.m[0], vv);
+  a.m[1] = fmax(a.m[1], 5 / 3. * p / d);
+  a.m[2] = fmax(a.m[2], ee);
+  if (vv >= vlim95) a.n[0] += 1.0;
+  if (ee >= elim95) a.n[1] += 1.0;
+}
+// block reduction of the accumulators into red[RED_VMAX..RED_CNT_E]; every
+// thread of the block calls it; lds holds 5 * blockDim.x / 64 doubles
+__device__ __forceinline__ void valf_flush(ValfAcc& a, double* lds, unsigned long long* red) {
+  const int nw = blockDim.x >> 6;
+  block_reduce<OpMax, 3>(a.m, lds);
+  block_reduce<OpAdd, 2>(a.n, lds + 3 * nw);
+  if (threadIdx.x == 0) {
+    atomic_max_nonneg(&red[RED_VMAX], a.m[0]);
+    atomic_max_nonneg(&red[RED_CS2MAX], a.m[1]);
+    atomic_max_nonneg(&red[RED_EMAX], a.m[2]);
+    atomicAdd(&red[RED_CNT_V], (unsigned long long)a.n[0]);
+    atomicAdd(&red[RED_CNT_E], (unsigned long long)a.n[1]);
+  }
+}
+
 // ---------------------------------------------------------------- MHD_Residual
 
 __device__ __forceinline__ void load_pt(const double* __restrict__ U, const double* __restrict__ P, long bs,
@@ -449,10 +482,10 @@ __device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
   return pr[0];
 }
 
-template <int D, bool HAS_TAU, bool EPI>
+template <int D, bool HAS_TAU, bool EPI, bool VALF = false>
 __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U,
                                                          const double* __restrict__ Pr, TvdEpilogue X) {
-  __shared__ double lds[MRM_BLOCK / 64];
+  __shared__ double lds[(VALF ? 5 : 1) * MRM_BLOCK / 64];
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   long base = 0, str = 1;
@@ -473,6 +506,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
     else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
   }
   double cm[1] = {0.0};
+  ValfAcc va;
   if (active) {
     const double lhfb = D == 0 ? 0.0 : P.hfb[lineX];
     const double lhft = D == 0 ? 0.0 : P.hft[lineX];
@@ -509,7 +543,15 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
           tvd_update(P, W1, Fm, F, qm, q, iq, uu);
           if (EPI) {
             tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
-            double pn = ghost_pres_(X.E, uu);
+            double pn;
+            if (VALF) {
+              double m1 = uu[1], m2 = uu[2], m3 = uu[3], ei = uu[4], pr[5];
+              cons_to_prim_cell<2>(X.E, uu[0], m1, m2, m3, ei, pr);
+              pn = pr[0];
+              valf_cell(va, uu[0], m1, m2, m3, ei, pn, X.vlim95, X.elim95);
+            } else {
+              pn = ghost_pres_(X.E, uu);
+            }
             X.Pnew[nd] = pn;
             cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
           }
@@ -526,6 +568,10 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
   if (EPI) {
     block_reduce<OpMax, 1>(cm, lds);
     if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
+    if (VALF) {
+      __syncthreads();
+      valf_flush(va, lds, X.valf);
+    }
   }
 }
 

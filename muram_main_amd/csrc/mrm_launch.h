@@ -30,6 +30,9 @@ inline hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const TvdEpi
   } else if (d == 1) {
     if (tau) k_tvd_sweep<1, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
     else k_tvd_sweep<1, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+  } else if (epi && epi->valf) {
+    if (tau) k_tvd_sweep<2, true, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
+    else k_tvd_sweep<2, false, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
   } else if (epi) {
     if (tau) k_tvd_sweep<2, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
     else k_tvd_sweep<2, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);

@@ -541,15 +541,29 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
   }
 }
 
-// pressure of the whole buffer and cmax (interior + dim-d ghost faces)
+// Adjust_Valf_Max's per-cell terms from a mixed cell: velocity and specific
+// internal energy in FP64 from the stored FP32 density/momentum and FP64 energy
+__device__ __forceinline__ void mx_valf_cell(ValfAcc& a, float d, float m0, float m1, float m2, double e, float p,
+                                             double vlim95, double elim95) {
+  const double dd = d, dn = 1.0 / dd;
+  const double v0 = m0 * dn, v1 = m1 * dn, v2 = m2 * dn;
+  const double eint = e - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2) * dd;
+  valf_cell(a, dd, v0, v1, v2, eint, p, vlim95, elim95);
+}
+
+// pressure of the whole buffer, cmax (interior + dim-d ghost faces) and,
+// with valf, the Adjust_Valf_Max statistics of the interior
 __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p_mx(Geom G, EosDev E, MxState S, float* __restrict__ P,
-                                                        double inv_va2max, unsigned long long* cmax_out) {
-  __shared__ double lds[MRM_BLOCK / 64];
+                                                        double inv_va2max, unsigned long long* cmax_out,
+                                                        unsigned long long* valf, double vlim95, double elim95) {
+  __shared__ double lds[5 * MRM_BLOCK / 64];
   const long bs = G.bs;
   double cm[1] = {0.0};
+  ValfAcc va;
   for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
     float d = S.f[n], m0 = S.f[bs + n], m1 = S.f[2 * bs + n], m2 = S.f[3 * bs + n];
-    float p = mx_pres(E, d, m0, m1, m2, S.e[n]);
+    const double e = S.e[n];
+    float p = mx_pres(E, d, m0, m1, m2, e);
     P[n] = p;
     int i = (int)(n % G.n0);
     long r = n / G.n0;
@@ -558,6 +572,32 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p_mx(Geom G, EosDev E, MxStat
     bool vis = (ii && ji && ki) || (!ii && ji && ki) || (ii && !ji && ki) || (ii && ji && !ki);
     if (vis) cm[0] = smax(cm[0], mx_cmax(d, m0, m1, m2, S.f[4 * bs + n], S.f[5 * bs + n], S.f[6 * bs + n], p,
                                           inv_va2max));
+    if (valf && ii && ji && ki) mx_valf_cell(va, d, m0, m1, m2, e, p, vlim95, elim95);
+  }
+  block_reduce<OpMax, 1>(cm, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
+  if (valf) {
+    __syncthreads();
+    valf_flush(va, lds, valf);
+  }
+}
+
+// cmax alone over the same points from the state and its pressure (the
+// stage-1 bound after Adjust_Valf_Max has changed inv_va2max)
+__global__ void __launch_bounds__(MRM_BLOCK) k_cmax_visited_mx(Geom G, MxState S, const float* __restrict__ P,
+                                                               double inv_va2max, unsigned long long* cmax_out) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const long bs = G.bs;
+  double cm[1] = {0.0};
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    int i = (int)(n % G.n0);
+    long r = n / G.n0;
+    int j = (int)(r % G.n1), k = (int)(r / G.n1);
+    bool ii = i >= G.lb0 && i <= G.le0, ji = j >= G.lb1 && j <= G.le1, ki = k >= G.lb2 && k <= G.le2;
+    bool vis = (ii && ji && ki) || (!ii && ji && ki) || (ii && !ji && ki) || (ii && ji && !ki);
+    if (vis)
+      cm[0] = smax(cm[0], mx_cmax(S.f[n], S.f[bs + n], S.f[2 * bs + n], S.f[3 * bs + n], S.f[4 * bs + n],
+                                  S.f[5 * bs + n], S.f[6 * bs + n], P[n], inv_va2max));
   }
   block_reduce<OpMax, 1>(cm, lds);
   if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
@@ -781,10 +821,10 @@ __device__ __forceinline__ void mx_store_cons(const MxState& S, long bs, unsigne
 }
 
 // one thread marches one grid line (dims 1 and 2, and dim 0 for long columns)
-template <int D, bool HAS_TAU, bool EPI>
+template <int D, bool HAS_TAU, bool EPI, bool VALF = false>
 __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams P, MxState S,
                                                             const float* __restrict__ Pr, MxEpilogue X) {
-  __shared__ double lds[MRM_BLOCK / 64];
+  __shared__ double lds[(VALF ? 5 : 1) * MRM_BLOCK / 64];
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   long base = 0, str = 1;
@@ -805,6 +845,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
     else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
   }
   double cm[1] = {0.0};
+  ValfAcc va;
   if (active) {
     const float lhfb = D == 0 ? 0.0f : (float)P.hfb[lineX];
     const float lhft = D == 0 ? 0.0f : (float)P.hft[lineX];
@@ -864,6 +905,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
             float pn = mx_pres(X.E, uu[0], uu[1], uu[2], uu[3], ee);
             st32(X.Pnew, nd, pn);
             cm[0] = smax(cm[0], mx_cmax(uu[0], uu[1], uu[2], uu[3], uu[5], uu[6], uu[7], pn, X.inv_va2max));
+            if (VALF) mx_valf_cell(va, uu[0], uu[1], uu[2], uu[3], ee, pn, X.vlim95, X.elim95);
           }
           mx_store_cons(S, bs, nd, uu, ee);
         }
@@ -881,6 +923,10 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
   if (EPI) {
     block_reduce<OpMax, 1>(cm, lds);
     if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
+    if (VALF) {
+      __syncthreads();
+      valf_flush(va, lds, X.valf);
+    }
   }
 }
 

@@ -38,8 +38,15 @@ hipError_t launch_bc(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E
 }
 
 hipError_t launch_eos_p(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, float* P,
-                        double inv_va2max, unsigned long long* cmax) {
-  k_eos_p_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, S, P, inv_va2max, cmax);
+                        double inv_va2max, unsigned long long* cmax, unsigned long long* valf, double vlim95,
+                        double elim95) {
+  k_eos_p_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, S, P, inv_va2max, cmax, valf, vlim95, elim95);
+  return hipGetLastError();
+}
+
+hipError_t launch_cmax(hipStream_t st, unsigned nb, const Geom& G, MxState S, const float* P, double inv_va2max,
+                       unsigned long long* cmax) {
+  k_cmax_visited_mx<<<nb, MRM_BLOCK, 0, st>>>(G, S, P, inv_va2max, cmax);
   return hipGetLastError();
 }
 
@@ -58,6 +65,9 @@ hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const MxEpilogue* e
   } else if (d == 1) {
     if (tau) k_tvd_sweep_mx<1, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
     else k_tvd_sweep_mx<1, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, X);
+  } else if (epi && epi->valf) {
+    if (tau) k_tvd_sweep_mx<2, true, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, *epi);
+    else k_tvd_sweep_mx<2, false, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, *epi);
   } else if (epi) {
     if (tau) k_tvd_sweep_mx<2, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, *epi);
     else k_tvd_sweep_mx<2, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, S, Pr, *epi);

@@ -13,7 +13,10 @@ hipError_t launch_bc_sums(hipStream_t st, unsigned nb, const Geom& G, const EosD
 hipError_t launch_bc(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, const BcParams& B,
                      const double* psum, MxState S, float* P, double inv_va2max, unsigned long long* cmax);
 hipError_t launch_eos_p(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, float* P,
-                        double inv_va2max, unsigned long long* cmax);
+                        double inv_va2max, unsigned long long* cmax, unsigned long long* valf, double vlim95,
+                        double elim95);
+hipError_t launch_cmax(hipStream_t st, unsigned nb, const Geom& G, MxState S, const float* P, double inv_va2max,
+                       unsigned long long* cmax);
 hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const Geom& G,
                               const MxTvdParams& P, MxState S, const float* Pr);
 hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const MxEpilogue* epi, unsigned nb, const Geom& G,

@@ -511,6 +511,8 @@ extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
 }
 
 // ------------------------------------------------------------------ Adjust_Valf_Max
+static void valf_apply(mrm_ctx* c, double stats[5]);
+
 // device reductions + the host-side update of Adjust_Valf_Max.C:97-144
 extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
   (void)run;
@@ -528,6 +530,13 @@ extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double s
   LAUNCHCHK(c, "k_valf_reduce");
   HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, RED_NSLOT * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  valf_apply(c, stats);
+  return MRM_OK;
+}
+
+// the host-side update of Adjust_Valf_Max.C:86-144 from the reduction slots
+// in c->h_red (max |v|, max (5/3) p/rho, max e_int/rho, the two counts)
+static void valf_apply(mrm_ctx* c, double stats[5]) {
   double g_max[3] = {bits_to_double(c->h_red[RED_VMAX]), sqrt(bits_to_double(c->h_red[RED_CS2MAX])),
                      bits_to_double(c->h_red[RED_EMAX])};
   double g_sum[2] = {(double)c->h_red[RED_CNT_V], (double)c->h_red[RED_CNT_E]};
@@ -566,7 +575,6 @@ extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double s
     stats[0] = g_max[0]; stats[1] = g_max[1]; stats[2] = g_max[2];
     stats[3] = g_sum[0]; stats[4] = g_sum[1];
   }
-  return MRM_OK;
 }
 
 // ------------------------------------------------------------------ MHD_Residual
@@ -1025,16 +1033,33 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   double* Pa = c->dP2;
   unsigned long long* cmax = c->d_red + RED_CMAX;
   int rc;
-  valf_constants(c);
+  // va_adjust > 0: Adjust_Valf_Max's statistics of the step's input come
+  // from the previous step's last TVD sweep (or from k_eos_p), its update
+  // runs on the host, and cmax is reduced again with the new inv_va2max
+  const bool va_adj = c->phys.params[2] > 0.0;
+  if (!va_adj) valf_constants(c);
   if (!c->fast_valid) {
-    HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_red, 0, (RED_CNT_E + 1) * sizeof(unsigned long long), c->stream));
     unsigned nb = nblk(G.bs);
     if (nb > 65536u) nb = 65536u;
     TScope ts(c, T_NAMED + 1);
-    k_eos_p<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->E, c->S[c->cur], Pc, c->inv_va2max, cmax);
+    k_eos_p<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->E, c->S[c->cur], Pc, c->inv_va2max, cmax,
+                                             va_adj ? c->d_red : nullptr, 0.95 * c->v_lim, 0.95 * c->e_lim);
     LAUNCHCHK(c, "k_eos_p");
   }
   double cm;
+  if (va_adj) {
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, (RED_CNT_E + 1) * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    valf_apply(c, nullptr);
+    HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+    unsigned nb = nblk(G.bs);
+    if (nb > 65536u) nb = 65536u;
+    { TScope ts(c, T_VALF);
+      k_cmax_visited<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->S[c->cur], Pc, c->inv_va2max, cmax); }
+    LAUNCHCHK(c, "k_cmax_visited");
+  }
   if ((rc = read_cmax(c, &cm))) return rc;
   double cfl = 0.0;
   if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
@@ -1051,7 +1076,7 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   // TVD on U4 with the pressure of the stage-4 input (tvdlimit_SR.C:332 reads
   // Grid.pres of the last ConsToPrim)
   double* Uf = c->S[i2];
-  HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_red, 0, (va_adj ? RED_CNT_E + 1 : 1) * sizeof(unsigned long long), c->stream));
   TvdEpilogue X;
   X.E = c->E;
   X.rho_min = c->phys.tchk[1];
@@ -1061,6 +1086,9 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.inv_va2max = c->inv_va2max;
   X.Pnew = Pc;
   X.cmax = cmax;
+  X.valf = va_adj ? c->d_red : nullptr;
+  X.vlim95 = 0.95 * c->v_lim;
+  X.elim95 = 0.95 * c->e_lim;
   c->last_dt = run->dt;
   c->last_split = !(cfl <= run->CFL_tvd);
   if (cfl <= run->CFL_tvd) {
@@ -1173,15 +1201,30 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   float* Pa = c->Pm[1];
   unsigned long long* cmax = c->d_red + RED_CMAX;
   int rc;
-  valf_constants(c);
+  // va_adjust > 0 as in step_fused: statistics from the previous step's last
+  // TVD sweep (or k_eos_p_mx), host update, cmax again with the new limit
+  const bool va_adj = c->phys.params[2] > 0.0;
+  if (!va_adj) valf_constants(c);
   if (!c->fast_valid) {
-    HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_red, 0, (RED_CNT_E + 1) * sizeof(unsigned long long), c->stream));
     unsigned nb = nblk(G.bs);
     if (nb > 65536u) nb = 65536u;
     TScope ts(c, T_NAMED + 1);
-    HIPCHK(c, mixed::launch_eos_p(c->stream, nb, G, c->E, c->M[c->cur], Pc, c->inv_va2max, cmax));
+    HIPCHK(c, mixed::launch_eos_p(c->stream, nb, G, c->E, c->M[c->cur], Pc, c->inv_va2max, cmax,
+                                  va_adj ? c->d_red : nullptr, 0.95 * c->v_lim, 0.95 * c->e_lim));
   }
   double cm;
+  if (va_adj) {
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, (RED_CNT_E + 1) * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    valf_apply(c, nullptr);
+    HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+    unsigned nb = nblk(G.bs);
+    if (nb > 65536u) nb = 65536u;
+    TScope ts(c, T_VALF);
+    HIPCHK(c, mixed::launch_cmax(c->stream, nb, G, c->M[c->cur], Pc, c->inv_va2max, cmax));
+  }
   if ((rc = read_cmax(c, &cm))) return rc;
   double cfl = 0.0;
   if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
@@ -1194,7 +1237,7 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     if ((rc = exchange_mx(c, M[st[s].out], st[s].pout))) return rc;
     if ((rc = bc_mx(c, M[st[s].out], st[s].pout, nullptr))) return rc;
   }
-  HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_red, 0, (va_adj ? RED_CNT_E + 1 : 1) * sizeof(unsigned long long), c->stream));
   MxEpilogue X;
   X.E = c->E;
   X.rho_min = (float)c->phys.tchk[1];
@@ -1204,6 +1247,9 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.inv_va2max = c->inv_va2max;
   X.Pnew = Pc;
   X.cmax = cmax;
+  X.valf = va_adj ? c->d_red : nullptr;
+  X.vlim95 = 0.95 * c->v_lim;
+  X.elim95 = 0.95 * c->e_lim;
   if (cfl <= run->CFL_tvd) {
     if ((rc = tvd_mx(c, M[i2], Pa, run->dt, &X))) return rc;
   } else {
@@ -1222,10 +1268,8 @@ extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps
   if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
-  const bool fused = !c->force_sequence && !(c->phys.params[2] > 0.0);
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
-  if (mx && !fused)
-    return fail(c, MRM_ERR_UNSUPPORTED, "mrm_step: the FP32-mixed form runs the fused step (va_adjust = 0) only");
+  const bool fused = !c->force_sequence;
   for (int it = 0; it < nsteps; it++) {
     int rc = mx ? step_mixed(c, run, dt_rad) : fused ? step_fused(c, run, dt_rad) : step_sequence(c, run, dt_rad);
     if (rc) return rc;

@@ -77,6 +77,10 @@ struct TvdEpilogue {
   double rho_min, eps_min, eps_max, vmax, inv_va2max;
   double* Pnew;
   unsigned long long* cmax;
+  // Adjust_Valf_Max statistics of the finished cells (va_adjust > 0): the
+  // reduction slots red[RED_VMAX..RED_CNT_E], or null; 0.95 v_lim, 0.95 e_lim
+  unsigned long long* valf;
+  double vlim95, elim95;
 };
 
 struct StageArgs {
@@ -133,4 +137,6 @@ struct MxEpilogue {
   double eps_min, eps_max, inv_va2max;
   float* Pnew;
   unsigned long long* cmax;
+  unsigned long long* valf;   // Adjust_Valf_Max statistics (va_adjust > 0) or null
+  double vlim95, elim95;
 };

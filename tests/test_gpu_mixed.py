@@ -25,12 +25,12 @@ def table():
     return open(os.path.join(HERE, "golden", "eos_small.dat"), "rb").read()
 
 
-def setup_pair(table, gsize, seed, v_rms=2.0e5, cfl=0.8, fields=None):
+def setup_pair(table, gsize, seed, v_rms=2.0e5, cfl=0.8, fields=None, phys=None):
     tab = eos_table.load_tables(table)
     box = initial.BoxSpec(gsize=gsize, dx=(3.2e6,) * 3, seed=seed, v_rms=v_rms)
     U = initial.make_state(tab, box)
     g = config.grid_desc(box.gsize, box.dx)
-    p = config.stratified_phys()
+    p = phys or config.stratified_phys()
     o = Oracle(g, p, table)
     ro = config.run_desc(CFL=cfl)
     o.set_state(U)
@@ -77,6 +77,17 @@ def test_mixed_steps_match_oracle(table, nsteps):
     o.step(ro, 1e30, nsteps)
     m.step(r, 1e30, nsteps)
     check(o, ro, m, r)
+
+
+def test_mixed_va_adjust(table):
+    print("case: mixed 40x36x28, 5 steps, va_adjust 1", flush=True)
+    p = config.stratified_phys(va_max=3.0e7, va_adjust=1.0)
+    o, ro, m, r = setup_pair(table, (40, 36, 28), seed=45, phys=p)
+    o.step(ro, 1e30, 5)
+    m.step(r, 1e30, 5)
+    check(o, ro, m, r)
+    print("limits", m.limits(), o.limits())
+    np.testing.assert_allclose(m.limits(), o.limits(), rtol=TOL_DT)
 
 
 def test_mixed_split_tvd(table):

@@ -1,8 +1,9 @@
 """The reference's Test_3D initial state (a 192 x 10 x 10 crop of
 TEST/Test_3D/ini, solar-convection stratification from rho 9e-16 to 5e-6
 g/cm^3) restored through the restart-file path and stepped on the GPU:
-fused FP64 against the oracle at 1e-12, a BackupSolution/RestoreSolution
-round trip, and the FP32-mixed form's error against FP64 on real data."""
+FP64 (fused and drop-in sequence, with and without va_adjust) against the
+oracle at 1e-12, a BackupSolution/RestoreSolution round trip, and the
+FP32-mixed form's error against FP64 on real data."""
 import os
 
 import numpy as np
@@ -37,8 +38,16 @@ def restart_dir(tmp_path, crop):
     return d, config.grid_desc((n0, n1, n2), tuple(crop["dx"]))
 
 
-def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0):
-    m = MhdGrid(g, config.test3d_phys(va_adjust), precision)
+def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused"):
+    saved = os.environ.get("MRM_STEP_PATH")
+    os.environ["MRM_STEP_PATH"] = path
+    try:
+        m = MhdGrid(g, config.test3d_phys(va_adjust), precision)
+    finally:
+        if saved is None:
+            os.environ.pop("MRM_STEP_PATH", None)
+        else:
+            os.environ["MRM_STEP_PATH"] = saved
     m.eos_init(table)
     r = config.run_desc()
     m.RestoreSolution(d, "result", d + "backup.dat", r)
@@ -47,12 +56,13 @@ def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0):
     return m, r
 
 
-@pytest.mark.parametrize("va_adjust", [1.0, 0.0])
-def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust):
-    # va_adjust 1 (the case's own setting) runs the drop-in sequence, 0 the fused step
-    print(f"case: Test_3D crop 192x10x10, restore + 4 FP64 steps vs oracle, va_adjust {va_adjust}", flush=True)
+@pytest.mark.parametrize("va_adjust,path", [(1.0, "fused"), (1.0, "sequence"), (0.0, "fused")])
+def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, path):
+    # va_adjust 1 is the case's own setting
+    print(f"case: Test_3D crop 192x10x10, restore + 4 FP64 {path} steps vs oracle, va_adjust {va_adjust}",
+          flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m, r = restored(d, g, table, va_adjust=va_adjust)
+    m, r = restored(d, g, table, va_adjust=va_adjust, path=path)
     assert (r.globiter, r.time) == (0, 0.0)
     o = Oracle(g, config.test3d_phys(va_adjust), table)
     ro = config.run_desc()
@@ -97,17 +107,19 @@ def test_backup_restore_round_trip(tmp_path, crop, table):
         assert relerr(U2[v][inner], U[v][inner]) <= 1e-6, v
 
 
-def test_mixed_precision_error_on_test3d(tmp_path, crop, table):
-    print("case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust 0", flush=True)
+@pytest.mark.parametrize("va_adjust", [0.0, 1.0])
+def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust):
+    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}", flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m64, r64 = restored(d, g, table, va_adjust=0.0)
-    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=0.0)
+    m64, r64 = restored(d, g, table, va_adjust=va_adjust)
+    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust)
     m64.step(r64, config.DT_RAD_OFF, 20)
     mx.step(rx, config.DT_RAD_OFF, 20)
     errs = field_errors(mx.get_state(), m64.get_state(), interior=(slice(2, -2),) * 3)
     print("FP32-mixed vs FP64 after 20 steps (rel L2, rel Linf):")
     for k, e in errs.items():
         print(f"  {k:4s} {e['l2']:.2e} {e['linf']:.2e}")
-    print("dt", rx.dt, r64.dt)
+    print("dt", rx.dt, r64.dt, "limits", mx.limits(), m64.limits())
     assert max(e["l2"] for e in errs.values()) <= 1e-4
     assert rx.dt == pytest.approx(r64.dt, rel=1e-5)
+    np.testing.assert_allclose(mx.limits(), m64.limits(), rtol=1e-5)
