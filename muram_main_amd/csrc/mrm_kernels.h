@@ -137,6 +137,31 @@ __device__ __forceinline__ void valf_cell(ValfAcc& a, double d, double v1, doubl
   if (vv >= vlim95) a.n[0] += 1.0;
   if (ee >= elim95) a.n[1] += 1.0;
 }
+// the same accumulators held in LDS, one column per thread (acc[q * stride]),
+// for kernels whose registers are all taken by the main loop
+__device__ __forceinline__ void valf_lds_init(double* acc, int stride) {
+#pragma unroll
+  for (int q = 0; q < 5; q++) acc[q * stride] = 0.0;
+}
+__device__ __forceinline__ void valf_lds_cell(double* acc, int stride, double d, double v1, double v2, double v3,
+                                              double eint, double p, double vlim95, double elim95) {
+  const double vv = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+  const double ee = eint / d;
+  acc[0] = fmax(acc[0], vv);
+  acc[stride] = fmax(acc[stride], 5 / 3. * p / d);
+  acc[2 * stride] = fmax(acc[2 * stride], ee);
+  if (vv >= vlim95) acc[3 * stride] += 1.0;
+  if (ee >= elim95) acc[4 * stride] += 1.0;
+}
+__device__ __forceinline__ ValfAcc valf_lds_load(const double* acc, int stride) {
+  ValfAcc a;
+#pragma unroll
+  for (int q = 0; q < 3; q++) a.m[q] = acc[q * stride];
+  a.n[0] = acc[3 * stride];
+  a.n[1] = acc[4 * stride];
+  return a;
+}
+
 // block reduction of the accumulators into red[RED_VMAX..RED_CNT_E]; every
 // thread of the block calls it; lds holds 5 * blockDim.x / 64 doubles
 __device__ __forceinline__ void valf_flush(ValfAcc& a, double* lds, unsigned long long* red) {
@@ -482,10 +507,14 @@ __device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
   return pr[0];
 }
 
+// VALF: the dim-2 epilogue also reduces Adjust_Valf_Max's statistics
+// (va_adjust > 0); that variant is held to two waves per SIMD like the others
 template <int D, bool HAS_TAU, bool EPI, bool VALF = false>
-__global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U,
-                                                         const double* __restrict__ Pr, TvdEpilogue X) {
+__global__ void __launch_bounds__(MRM_BLOCK) __attribute__((amdgpu_waves_per_eu(VALF ? 2 : 1)))
+k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restrict__ Pr, TvdEpilogue X) {
   __shared__ double lds[(VALF ? 5 : 1) * MRM_BLOCK / 64];
+  __shared__ double vacc[VALF ? 5 * MRM_BLOCK : 1];
+  if (VALF) valf_lds_init(vacc + threadIdx.x, MRM_BLOCK);
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   long base = 0, str = 1;
@@ -506,7 +535,6 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
     else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
   }
   double cm[1] = {0.0};
-  ValfAcc va;
   if (active) {
     const double lhfb = D == 0 ? 0.0 : P.hfb[lineX];
     const double lhft = D == 0 ? 0.0 : P.hft[lineX];
@@ -543,14 +571,13 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
           tvd_update(P, W1, Fm, F, qm, q, iq, uu);
           if (EPI) {
             tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
-            double pn;
+            const double pn = ghost_pres_(X.E, uu);
             if (VALF) {
-              double m1 = uu[1], m2 = uu[2], m3 = uu[3], ei = uu[4], pr[5];
-              cons_to_prim_cell<2>(X.E, uu[0], m1, m2, m3, ei, pr);
-              pn = pr[0];
-              valf_cell(va, uu[0], m1, m2, m3, ei, pn, X.vlim95, X.elim95);
-            } else {
-              pn = ghost_pres_(X.E, uu);
+              // v and e_int as ConsToPrim forms them (cons_to_prim.C:110-118)
+              const double dn = 1.0 / uu[0];
+              const double v1 = uu[1] * dn, v2 = uu[2] * dn, v3 = uu[3] * dn;
+              const double ei = uu[4] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3) * uu[0];
+              valf_lds_cell(vacc + threadIdx.x, MRM_BLOCK, uu[0], v1, v2, v3, ei, pn, X.vlim95, X.elim95);
             }
             X.Pnew[nd] = pn;
             cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
@@ -570,6 +597,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep(Geom G, TvdParams P, do
     if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
     if (VALF) {
       __syncthreads();
+      ValfAcc va = valf_lds_load(vacc + threadIdx.x, MRM_BLOCK);
       valf_flush(va, lds, X.valf);
     }
   }
