@@ -49,7 +49,9 @@ extern "C" {
 #define MRM_AMB  4
 #define MRM_QTOT 5
 #define MRM_TAU  6
-#define MRM_NFIELD 7
+#define MRM_DIVB 7   /* Clean_divB's divergence and potential (grid.H:78-80) */
+#define MRM_PHI  8
+#define MRM_NFIELD 9
 
 /* host array layouts for mrm_set_state / mrm_get_state */
 #define MRM_LAYOUT_AOS 0   /* cState[bufsize]: 8 doubles per cell (mhd3d.H:24-27) */
@@ -79,6 +81,7 @@ typedef struct mrm_phys_desc {
   double tchk[4];
   double dmp[4];
   double rt[10];
+  double divB[3];     /* i_divB_switch, i_divB_itmax, i_divB_err (physics.H:43-45) */
   int    rt_ext_cor;
   int    pad_;
 } mrm_phys_desc;
@@ -155,6 +158,12 @@ int mrm_save_cons(mrm_ctx* ctx);
 int mrm_tcheck(mrm_ctx* ctx, const mrm_run_desc* run);
 /* TVDlimit (mhd_tvd.H:6, tvdlimit_SR.C:33-695). */
 int mrm_tvdlimit(mrm_ctx* ctx, const mrm_run_desc* run, double dt_tvd);
+/* Clean_divB (divB.H, clean_divB.C:35-369) on one rank: the damped potential
+ * phi (MRM_PHI, zero at mrm_create, kept between calls) corrects B and e over
+ * the adaptive number of passes; err (may be NULL) receives max |div B|
+ * before and after in the reference's units.  FP64 contexts; mrm_step calls
+ * it between TVDlimit and TCheck when phys.divB[0] > 0. */
+int mrm_clean_divB(mrm_ctx* ctx, const mrm_run_desc* run, double err[2]);
 /* exchange_grid_acc (exchange.H:8, exchange.C:532-782) on one rank:
  * periodic ghost layers of U in dims 1 and 2, filled on the device. */
 int mrm_exchange_grid(mrm_ctx* ctx);
@@ -163,10 +172,10 @@ int mrm_exchange_grid(mrm_ctx* ctx);
 int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage);
 
 /* nsteps iterations of the ComputeSolution loop body (solver.C:109-365) with
- * RT, divB cleaning and IO off: SaveCons, 4 x (ConsToPrim, Adjust_Valf_Max at
- * stage 1, MHD_Residual, Sync_timestep at stage 1, Source_Integrate_Tcheck,
- * exchange, BC), TVDlimit (split in two halves when cfl > CFL_tvd), TCheck,
- * exchange, BC, Run.Advance.  dt_rad is the radiative time step the caller
+ * RT and IO off: SaveCons, 4 x (ConsToPrim, Adjust_Valf_Max at stage 1,
+ * MHD_Residual, Sync_timestep at stage 1, Source_Integrate_Tcheck, exchange,
+ * BC), TVDlimit (split in two halves when cfl > CFL_tvd), Clean_divB when
+ * phys.divB[0] > 0, TCheck, exchange, BC, Run.Advance.  dt_rad is the radiative time step the caller
  * supplies (RT is outside the path). */
 int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
 

@@ -17,7 +17,7 @@ ERRORS = {-1: "MRM_ERR_ARG", -2: "MRM_ERR_HIP", -3: "MRM_ERR_EOS", -4: "MRM_ERR_
           -5: "MRM_ERR_CFL", -6: "MRM_ERR_STATE", -7: "MRM_ERR_UNSUPPORTED"}
 PREC_FP64, PREC_FP32_MIXED = 0, 1
 U, U0, RES = 0, 1, 2
-PRES, TEMP, NE, RHOI, AMB, QTOT, TAU = range(7)
+PRES, TEMP, NE, RHOI, AMB, QTOT, TAU, DIVB, PHI = range(9)
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
 
 # every symbol include/muram_step.h declares
@@ -26,7 +26,7 @@ EXPORTS = [
     "mrm_eos_load", "mrm_set_state", "mrm_get_state", "mrm_set_field", "mrm_get_field",
     "mrm_get_limits", "mrm_set_limits", "mrm_get_bc", "mrm_set_bc",
     "mrm_cons_to_prim", "mrm_adjust_valf_max", "mrm_mhd_residual", "mrm_sync_timestep",
-    "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit",
+    "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit", "mrm_clean_divB",
     "mrm_exchange_grid", "mrm_set_boundary_conditions", "mrm_step",
     "mrm_io_read_prim", "mrm_io_write_prim", "mrm_io_write_header", "mrm_io_read_header",
     "mrm_io_write_backup", "mrm_io_read_backup",
@@ -47,7 +47,7 @@ class PhysDesc(C.Structure):
                 ("tvd_h_bnd", C.c_double * 2), ("tvd_visc_bnd", C.c_double * 2),
                 ("tvd_eta_bnd", C.c_double * 2), ("tvd_coeff", C.c_double * 4),
                 ("tchk", C.c_double * 4), ("dmp", C.c_double * 4), ("rt", C.c_double * 10),
-                ("rt_ext_cor", C.c_int), ("pad_", C.c_int)]
+                ("divB", C.c_double * 3), ("rt_ext_cor", C.c_int), ("pad_", C.c_int)]
 
 
 class RunDesc(C.Structure):
@@ -97,6 +97,7 @@ def load(path: str = LIB_PATH):
         "mrm_save_cons": ([P], C.c_int),
         "mrm_tcheck": ([P, C.POINTER(RunDesc)], C.c_int),
         "mrm_tvdlimit": ([P, C.POINTER(RunDesc), C.c_double], C.c_int),
+        "mrm_clean_divB": ([P, C.POINTER(RunDesc), dp], C.c_int),
         "mrm_exchange_grid": ([P], C.c_int),
         "mrm_set_boundary_conditions": ([P, C.POINTER(RunDesc), C.c_int], C.c_int),
         "mrm_step": ([P, C.POINTER(RunDesc), C.c_double, C.c_int], C.c_int),

@@ -24,6 +24,8 @@ I_TVD_RHOLEV, I_TVD_RHOLOG, I_TVD_QRHO, I_TVD_BPAR, I_TVD_VHYP, I_TVD_QDIFF_BND,
     I_TVD_PM_V, I_TVD_PM_B, I_TVD_VMAX_LIM, I_TVD_CME_THRESH = range(10)
 I_TCHK_EPS_MIN, I_TCHK_RHO_MIN, I_TCHK_EPS_MAX, I_TCHK_VMAX = range(4)
 I_RT_TSTEP, I_RT_CFL = 3, 4
+I_DIVB_SWITCH, I_DIVB_ITMAX, I_DIVB_ERR = range(3)
+DIVB_FIELD, PHI_FIELD = 7, 8     # mrm field ids of div B and phi
 
 
 def default_phys() -> PhysDesc:
@@ -40,6 +42,7 @@ def default_phys() -> PhysDesc:
     p.tchk[:] = [1.0e11, 1.0e-16, 4.0e15, 100.0e5]
     p.dmp[:] = [0, 1.0e3, 1.0e3, 1.0e2]
     p.rt[:] = [1, 1.0e-8, 20.0e3, 0.0, 0.5, 0, 0.0, 0, 1e2, 1e2]
+    p.divB[:] = [1, 5, 0.2]
     p.rt_ext_cor = 0
     return p
 
@@ -52,6 +55,7 @@ def stratified_phys(va_max: float = 3.0e8, va_adjust: float = 0.0) -> PhysDesc:
     p.bnd[I_BND_TOP] = 1.0
     p.bnd[I_BND_POT] = 0.0
     p.bnd[I_BND_EPS_TOP] = 0.0
+    p.divB[0] = 0.0
     # TEST/Test_3D/parameters.dat:67-91
     p.tvd_h[:] = [2.0, 2.0, 1.0, 0.0]
     p.tvd_cs[:] = [0.2, 0.2, 0.2, 1.0]
@@ -64,13 +68,16 @@ def stratified_phys(va_max: float = 3.0e8, va_adjust: float = 0.0) -> PhysDesc:
     return p
 
 
-def test3d_phys(va_adjust: float = 1.0) -> PhysDesc:
-    """TEST/Test_3D/parameters.dat with Spitzer conduction, damping, divB
-    cleaning and the potential-field top boundary off (outside the path):
-    va_max 1e7 with va_adjust 1 (:51-52), half-open top (:60), B_crit 2.5e30
-    (:63), hot plate eps_top 2e14 (:64), tchk eps_min 1.75e12, rho_min 1e-18,
-    eps_max 2e14, vmax 5e7 (:99-102)."""
+def test3d_phys(va_adjust: float = 1.0, divb: bool = True) -> PhysDesc:
+    """TEST/Test_3D/parameters.dat with Spitzer conduction, damping and the
+    potential-field top boundary off (outside the path): va_max 1e7 with
+    va_adjust 1 (:51-52), half-open top (:60), B_crit 2.5e30 (:63), hot plate
+    eps_top 2e14 (:64), divB cleaning on with 5 passes at most and a 0.2 error
+    reduction target (:94-96), tchk eps_min 1.75e12, rho_min 1e-18, eps_max
+    2e14, vmax 5e7 (:99-102)."""
     p = stratified_phys(va_max=1.0e7, va_adjust=va_adjust)
+    if divb:
+        p.divB[:] = [1, 5, 0.2]
     p.bnd[I_BND_BCRIT] = 2.5e30
     p.bnd[I_BND_EPS_TOP] = 2.0e14
     p.tchk[:] = [1.75e12, 1.0e-18, 2.0e14, 5.0e7]

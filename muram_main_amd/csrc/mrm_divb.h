@@ -1,0 +1,140 @@
+// mrm_divb.h -- Clean_divB (clean_divB.C:35-369) on the device, FP64: the
+// 4th-order div B of the interior, the damped potential phi (kept from call to
+// call), its periodic and antisymmetric vertical ghosts, and the correction
+// B += grad phi, e += phi div B.  One thread per interior cell; the update
+// kernel can finish each cell as the step's last TVD sweep would (TCheck, EOS
+// pressure, characteristic speed, Adjust_Valf_Max statistics) when the
+// cleaning runs between TVDlimit and TCheck (solver.C:325-345).
+#pragma once
+#include "mrm_kernels.h"
+
+namespace MRM_NS {
+
+struct DivbArgs {
+  Geom G;
+  double w1[3], w2[3];     // 8/(12 dx), -1/(12 dx) per dimension (grid.C:339-340)
+  double fac, damp;        // divB_fac, divB_damp (clean_divB.C:47-48, 110)
+};
+
+__device__ __forceinline__ long divb_node(const Geom& G, long t, bool& ok) {
+  int i, j, k;
+  ok = interior_ijk(G, t, i, j, k);
+  return ok ? i + j * G.s1 + (long)k * G.s2 : 0;
+}
+
+// divB of the interior (clean_divB.C:125-162) and max |divB| into err; with
+// PHI, the next pass's potential update phi += fac divB - damp phi
+// (clean_divB.C:168-178) of the same cell
+template <bool PHI>
+__global__ void __launch_bounds__(MRM_BLOCK) k_divb(DivbArgs A, const double* __restrict__ U,
+                                                    double* __restrict__ divB, double* __restrict__ phi,
+                                                    unsigned long long* err) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const Geom& G = A.G;
+  double m[1] = {0.0};
+  bool ok;
+  const long n = divb_node(G, blockIdx.x * (long)blockDim.x + threadIdx.x, ok);
+  if (ok) {
+    const long s[3] = {1, G.s1, G.s2};
+    double d = 0.0;
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      const double* B = U + (5 + v) * G.bs;
+      d += (A.w1[v] * (B[n + s[v]] - B[n - s[v]]) + A.w2[v] * (B[n + 2 * s[v]] - B[n - 2 * s[v]]));
+    }
+    divB[n] = d;
+    m[0] = fabs(d);
+    if (PHI) phi[n] += A.fac * d - A.damp * phi[n];
+  }
+  block_reduce<OpMax, 1>(m, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(err, m[0]);
+}
+
+// periodic ghosts of na arrays base + q * vstride along dim DIM (the ranges of
+// exchange_B_acc / exchange_single_acc on one rank, exchange.C:785-1120)
+template <int DIM>
+__global__ void __launch_bounds__(MRM_BLOCK) k_exchange_arr(Geom G, double* __restrict__ base, int na,
+                                                            long vstride) {
+  const int nx = G.le0 - G.lb0 + 1;
+  const int gh = DIM == 1 ? G.g1 : G.g2;
+  const int lb = DIM == 1 ? G.lb1 : G.lb2, le = DIM == 1 ? G.le1 : G.le2;
+  const int nb = DIM == 1 ? (G.le2 - G.lb2 + 1) : G.n1;
+  const int ob = DIM == 1 ? G.lb2 : 0;
+  const long sd = DIM == 1 ? G.s1 : G.s2, so = DIM == 1 ? G.s2 : G.s1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nx * nb * gh) return;
+  int i = G.lb0 + (int)(t % nx);
+  long r = t / nx;
+  int b = ob + (int)(r % nb);
+  int igh = (int)(r / nb);
+  long c0 = i + b * so;
+  long src_r = c0 + (long)(lb + igh) * sd, dst_r = c0 + (long)(le + 1 + igh) * sd;
+  long src_l = c0 + (long)(le - gh + 1 + igh) * sd, dst_l = c0 + (long)(lb - gh + igh) * sd;
+  for (int q = 0; q < na; q++) {
+    double* a = base + q * vstride;
+    a[dst_r] = a[src_r];
+    a[dst_l] = a[src_l];
+  }
+}
+
+// phi antisymmetric across the bottom and top of every interior column
+// (clean_divB.C:186-217)
+__global__ void __launch_bounds__(MRM_BLOCK) k_phi_vertical(Geom G, double* __restrict__ phi) {
+  const int ny = G.le1 - G.lb1 + 1, nz = G.le2 - G.lb2 + 1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)ny * nz) return;
+  const long c0 = (G.lb1 + (int)(t % ny)) * G.s1 + (long)(G.lb2 + (int)(t / ny)) * G.s2;
+  const int lb = G.lb0, le = G.le0;
+  phi[c0 + lb - 1] = -phi[c0 + lb];
+  phi[c0 + lb - 2] = -phi[c0 + lb + 1];
+  phi[c0 + le + 1] = -phi[c0 + le];
+  phi[c0 + le + 2] = -phi[c0 + le - 1];
+}
+
+// B += grad phi, e += phi divB (clean_divB.C:219-262).  With EPI the cell is
+// then finished: TCheck (Add_Sources_Integrate_SR.C:591-624), its EOS
+// pressure, its characteristic speed and, with X.valf, the Adjust_Valf_Max
+// statistics for the next step.
+template <bool EPI>
+__global__ void __launch_bounds__(MRM_BLOCK) k_divb_update(DivbArgs A, double* __restrict__ U,
+                                                           const double* __restrict__ phi,
+                                                           const double* __restrict__ divB, TvdEpilogue X) {
+  __shared__ double lds[5 * MRM_BLOCK / 64];
+  const Geom& G = A.G;
+  const long bs = G.bs;
+  double cm[1] = {0.0};
+  ValfAcc va;
+  bool ok;
+  const long n = divb_node(G, blockIdx.x * (long)blockDim.x + threadIdx.x, ok);
+  if (ok) {
+    const long s[3] = {1, G.s1, G.s2};
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+      U[(5 + v) * bs + n] += (A.w1[v] * (phi[n + s[v]] - phi[n - s[v]]) +
+                              A.w2[v] * (phi[n + 2 * s[v]] - phi[n - 2 * s[v]]));
+    U[4 * bs + n] += phi[n] * divB[n];
+    if (EPI) {
+      double uu[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) uu[q] = U[q * bs + n];
+      tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
+#pragma unroll
+      for (int q = 0; q < 5; q++) U[q * bs + n] = uu[q];
+      double m1 = uu[1], m2 = uu[2], m3 = uu[3], ei = uu[4], pr[5];
+      cons_to_prim_cell<2>(X.E, uu[0], m1, m2, m3, ei, pr);
+      X.Pnew[n] = pr[0];
+      cm[0] = smax(0.0, point_cmax_cons(uu, pr[0], X.inv_va2max));
+      if (X.valf) valf_cell(va, uu[0], m1, m2, m3, ei, pr[0], X.vlim95, X.elim95);
+    }
+  }
+  if (EPI) {
+    block_reduce<OpMax, 1>(cm, lds);
+    if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
+    if (X.valf) {
+      __syncthreads();
+      valf_flush(va, lds, X.valf);
+    }
+  }
+}
+
+}  // namespace MRM_NS

@@ -17,6 +17,7 @@
 
 #include "../../include/muram_step.h"
 #include "mrm_launch.h"
+#include "mrm_divb.h"
 #include "mrm_mixed_api.h"
 #include "mrm_adjoint_api.h"
 
@@ -84,6 +85,11 @@ struct mrm_ctx {
   double* d_prof = nullptr;  // hfb, hft, qft (3 * n0)
   int tvd_ini = 1;
 
+  // Clean_divB (clean_divB.C:50): passes of the next call; the last call's
+  // errors wait in d_red[RED_DIVB0/1] until the next host round trip
+  int divb_next_iter = 3;
+  bool divb_pending = false;
+
   unsigned long long* d_red = nullptr;   // RED_NSLOT slots
   double* d_part = nullptr;
   long npart = 0;
@@ -123,7 +129,8 @@ enum {
 static const char* const timer_names[MRM_NTIMER] = {
   "cons_to_prim", "valf_reduce", "residual", "cmax_faces", "source_integrate", "save_cons", "tcheck",
   "exchange", "bc_sums", "bc_bottom", "bc_top", "tvd_sweep_x", "tvd_sweep_y", "tvd_sweep_z",
-  "stage_fused", "eos_p", "t16", "t17", "t18", "t19", "t20", "t21", "t22", "t23"};
+  "stage_fused", "eos_p", "clean_divb", "t17", "t18", "t19", "t20", "t21", "t22", "t23"};
+static const int T_DIVB = T_NAMED + 2;
 
 // brackets the launches of one timer id with events on the context's stream
 struct TScope {
@@ -192,6 +199,7 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   // physics outside the implemented path
   if (phys->params[3] > 0.0 || phys->params[4] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
+  if (phys->divB[0] > 0.0 && (precision != MRM_PREC_FP64 || phys->divB[1] < 1.0)) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
     if (grid->gsize[d] < 1 || grid->ghosts[d] != 2) return MRM_ERR_ARG;
     if (!(grid->gxmax[d] > grid->gxmin[d])) return MRM_ERR_ARG;
@@ -724,6 +732,108 @@ extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   return exchange_buf(c, c->dU, nullptr);
 }
 
+// ------------------------------------------------------------------ Clean_divB
+static DivbArgs divb_args(const mrm_ctx* c) {
+  DivbArgs A;
+  A.G = c->G;
+  for (int d = 0; d < 3; d++) {
+    A.w1[d] = 8. / (12. * c->dx[d]);
+    A.w2[d] = -1. / (12. * c->dx[d]);
+  }
+  // clean_divB.C:92-110 (the 3-D branch compares dx[2] with the updated dxmin)
+  double dxmin = std::min(c->dx[0], c->dx[1]);
+  double dxmax = std::max(c->dx[0], c->dx[1]);
+  dxmin = std::min(dxmin, c->dx[2]);
+  dxmax = std::max(dxmin, c->dx[2]);
+  A.fac = 1.0 * std::min(dxmin, 0.4 * dxmax) * dxmin;
+  A.damp = 0.65;
+  return A;
+}
+
+// the pass-count update of clean_divB.C:330-345 from the maxima in h_red
+static void divb_settle_host(mrm_ctx* c, double err_out[2]) {
+  double err_unit = 0.0;
+  for (int v = 0; v < 3; v++) err_unit += 1.0 / c->dx[v];
+  err_unit = 0.5 / err_unit * 3.5449;
+  const double e0 = bits_to_double(c->h_red[RED_DIVB0]) * err_unit;
+  const double e1 = bits_to_double(c->h_red[RED_DIVB1]) * err_unit;
+  const double err = e1 / e0;
+  if (err > c->phys.divB[2]) c->divb_next_iter += 1;
+  else c->divb_next_iter -= 1;
+  c->divb_next_iter = std::min(c->divb_next_iter, (int)c->phys.divB[1]);
+  c->divb_next_iter = std::max(1, c->divb_next_iter);
+  c->divb_pending = false;
+  if (err_out) { err_out[0] = e0; err_out[1] = e1; }
+}
+
+static int divb_settle(mrm_ctx* c, double err_out[2]) {
+  if (!c->divb_pending) return MRM_OK;
+  HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, (RED_DIVB1 + 1) * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  divb_settle_host(c, err_out);
+  return MRM_OK;
+}
+
+static int exchange_arrays(mrm_ctx* c, double* base, int na) {
+  const Geom& G = c->G;
+  long n1 = (long)c->lsize[0] * c->lsize[2] * G.g1;
+  k_exchange_arr<1><<<nblk(n1), MRM_BLOCK, 0, c->stream>>>(G, base, na, G.bs);
+  LAUNCHCHK(c, "k_exchange_arr<1>");
+  long n2 = (long)c->lsize[0] * G.n1 * G.g2;
+  k_exchange_arr<2><<<nblk(n2), MRM_BLOCK, 0, c->stream>>>(G, base, na, G.bs);
+  LAUNCHCHK(c, "k_exchange_arr<2>");
+  return MRM_OK;
+}
+
+// Clean_divB's passes on U (clean_divB.C:118-328).  The first divergence
+// kernel also makes the first potential update, each later one the next
+// pass's; with epi the last correction pass finishes every cell.  The two
+// maxima land in d_red[RED_DIVB0/1] and c->divb_pending is set.
+static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi) {
+  int rc;
+  if ((rc = divb_settle(c, nullptr))) return rc;
+  const Geom& G = c->G;
+  const DivbArgs A = divb_args(c);
+  double* divB = c->dF + MRM_DIVB * G.bs;
+  double* phi = c->dF + MRM_PHI * G.bs;
+  unsigned long long* red = c->d_red;
+  const unsigned nb = nblk(ninterior(c));
+  const int npass = c->divb_next_iter;
+  TScope ts(c, T_DIVB);
+  HIPCHK(c, hipMemsetAsync(red + RED_DIVB0, 0, 2 * sizeof(unsigned long long), c->stream));
+  if ((rc = exchange_arrays(c, U + 5 * G.bs, 3))) return rc;
+  k_divb<true><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, divB, phi, red + RED_DIVB0);
+  LAUNCHCHK(c, "k_divb");
+  for (int it = 0; it < npass; it++) {
+    const bool last = it == npass - 1;
+    if ((rc = exchange_arrays(c, phi, 1))) return rc;
+    const long ncol = (long)c->lsize[1] * c->lsize[2];
+    k_phi_vertical<<<nblk(ncol), MRM_BLOCK, 0, c->stream>>>(G, phi);
+    LAUNCHCHK(c, "k_phi_vertical");
+    if (last && epi) k_divb_update<true><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, phi, divB, *epi);
+    else k_divb_update<false><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, phi, divB, TvdEpilogue{});
+    LAUNCHCHK(c, "k_divb_update");
+    if ((rc = exchange_arrays(c, U + 5 * G.bs, 3))) return rc;
+    if (last) k_divb<false><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, divB, phi, red + RED_DIVB1);
+    else k_divb<true><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, divB, phi, red + RED_SCRATCH);
+    LAUNCHCHK(c, "k_divb");
+  }
+  c->divb_pending = true;
+  return MRM_OK;
+}
+
+extern "C" int mrm_clean_divB(mrm_ctx* c, const mrm_run_desc* run, double err[2]) {
+  if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_clean_divB");
+  if (run && run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
+  if (c->phys.divB[1] < 1.0) return fail(c, MRM_ERR_ARG, "mrm_clean_divB: divB_itmax < 1");
+  c->fast_valid = false;
+  int rc = clean_divb_dev(c, c->dU, nullptr);
+  if (rc) return rc;
+  return divb_settle(c, err);
+}
+
 // ------------------------------------------------------------------ boundary conditions
 // partial sums over the two lowest layers of U (FP64) or of the mixed state M
 static int bottom_sums(mrm_ctx* c, int mode, const double* U, double* dst_dev, const MxState* M = nullptr) {
@@ -981,6 +1091,7 @@ static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     STEPCHK(mrm_exchange_grid(c));
     STEPCHK(mrm_tvdlimit(c, run, 0.5 * run->dt));
   }
+  if (c->phys.divB[0] > 0.0) STEPCHK(mrm_clean_divB(c, run, nullptr));
   STEPCHK(mrm_tcheck(c, run));
   STEPCHK(mrm_exchange_grid(c));
   STEPCHK(mrm_set_boundary_conditions(c, run, 5));
@@ -1047,11 +1158,16 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
                                              va_adj ? c->d_red : nullptr, 0.95 * c->v_lim, 0.95 * c->e_lim);
     LAUNCHCHK(c, "k_eos_p");
   }
+  // one round trip brings cmax, the limiter statistics and the previous
+  // Clean_divB's maxima
   double cm;
+  const int nread = c->divb_pending ? RED_DIVB1 + 1 : va_adj ? RED_CNT_E + 1 : 1;
+  HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, nread * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->divb_pending) divb_settle_host(c, nullptr);
+  cm = bits_to_double(c->h_red[RED_CMAX]);
   if (va_adj) {
-    HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, (RED_CNT_E + 1) * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     valf_apply(c, nullptr);
     HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
     unsigned nb = nblk(G.bs);
@@ -1059,8 +1175,8 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     { TScope ts(c, T_VALF);
       k_cmax_visited<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->S[c->cur], Pc, c->inv_va2max, cmax); }
     LAUNCHCHK(c, "k_cmax_visited");
+    if ((rc = read_cmax(c, &cm))) return rc;
   }
-  if ((rc = read_cmax(c, &cm))) return rc;
   double cfl = 0.0;
   if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
@@ -1091,13 +1207,18 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.elim95 = 0.95 * c->e_lim;
   c->last_dt = run->dt;
   c->last_split = !(cfl <= run->CFL_tvd);
+  // with Clean_divB (between TVDlimit and TCheck, solver.C:341-349) its last
+  // correction pass finishes the cells instead of the last TVD sweep
+  const bool divb = c->phys.divB[0] > 0.0;
+  const TvdEpilogue* tvd_epi = divb ? nullptr : &X;
   if (cfl <= run->CFL_tvd) {
-    if ((rc = tvd_buf(c, Uf, Pa, run->dt, &X, c->fma))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, run->dt, tvd_epi, c->fma))) return rc;
   } else {
     if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, nullptr, c->fma))) return rc;
     if ((rc = exchange_buf(c, Uf, nullptr))) return rc;
-    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, &X, c->fma))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, tvd_epi, c->fma))) return rc;
   }
+  if (divb && (rc = clean_divb_dev(c, Uf, &X))) return rc;
   if ((rc = exchange_buf(c, Uf, Pc))) return rc;
   if ((rc = bc_buf(c, Uf, Pc, cmax))) return rc;
   c->cur = i2;
@@ -1443,8 +1564,9 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   NEED_FP64(c, "mrm_adjoint_forward");
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
-  if (c->force_sequence || c->phys.params[2] > 0.0)
-    return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: the adjoint follows the fused step (va_adjust = 0)");
+  if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0)
+    return fail(c, MRM_ERR_UNSUPPORTED,
+                "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0 and no Clean_divB");
   for (int d = 0; d < 3; d++)
     if (c->lsize[d] < 4) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: interior extents below 4");
   for (double* p : c->adj_ckpt) hipFree(p);

@@ -9,7 +9,10 @@
 #define RED_EMAX 3
 #define RED_CNT_V 4
 #define RED_CNT_E 5
+#define RED_DIVB0 6    // max |div B| before and after Clean_divB's passes
+#define RED_DIVB1 7
 #define RED_SUM0 8
+#define RED_SCRATCH 15 // reductions nobody reads
 #define RED_NSLOT 16
 #define MRM_BLOCK 256
 #define ST_SX 16

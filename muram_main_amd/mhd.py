@@ -127,6 +127,12 @@ class MhdGrid:
     def TVDlimit(self, run, dt_tvd: float):
         self._chk(self.lib.mrm_tvdlimit(self.h, C.byref(run), dt_tvd), "TVDlimit")
 
+    def Clean_divB(self, run):
+        """Clean_divB (clean_divB.C:35-369) -> max |div B| before and after"""
+        err = (C.c_double * 2)()
+        self._chk(self.lib.mrm_clean_divB(self.h, C.byref(run), err), "Clean_divB")
+        return np.array(err[:])
+
     def exchange_grid(self):
         self._chk(self.lib.mrm_exchange_grid(self.h), "exchange_grid")
 

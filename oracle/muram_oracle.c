@@ -64,6 +64,8 @@ struct orc_ctx {
   int tvd_ini;
   /* stage-loop scalars */
   double last_cfl;
+  /* Clean_divB's adaptive iteration count (clean_divB.C:50) */
+  int divb_next_iter;
 };
 
 void orc_set_threads(int n) {
@@ -115,6 +117,7 @@ orc_ctx* orc_create(const mrm_grid_desc* g, const mrm_phys_desc* p) {
   c->s_bc = -1.0;
   c->bnd_ini = 1;
   c->tvd_ini = 1;
+  c->divb_next_iter = 3;
   return c;
 }
 
@@ -972,6 +975,130 @@ void orc_exchange_grid(orc_ctx* c) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* exchange_B_acc / exchange_single_acc (exchange.C:785-1120) on one rank:   */
+/* the periodic fill of exchange_grid_acc for arrays a[0..na) only.          */
+static void exchange_arrays(orc_ctx* c, double* const* a, int na) {
+  static const int perm[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+  int gs[3][2];
+  for (int d = 0; d < 3; d++) { gs[d][0] = c->lbeg[d]; gs[d][1] = c->lend[d]; }
+  for (int dim = 0; dim < 3; dim++) {
+    const int d1 = perm[dim][0], d2 = perm[dim][1], d3 = perm[dim][2];
+    if (!c->periods[d1]) continue;
+    const int gh = c->ghosts[d1], lbeg = c->lbeg[d1], lend = c->lend[d1];
+    for (int igh = 0; igh < gh; igh++)
+      for (int k = gs[d3][0]; k <= gs[d3][1]; k++)
+        for (int j = gs[d2][0]; j <= gs[d2][1]; j++) {
+          long base = j * c->stride[d2] + k * c->stride[d3];
+          long src_r = base + (long)(lbeg + igh) * c->stride[d1];
+          long dst_r = base + (long)(lend + 1 + igh) * c->stride[d1];
+          long src_l = base + (long)(lend - gh + 1 + igh) * c->stride[d1];
+          long dst_l = base + (long)(lbeg - gh + igh) * c->stride[d1];
+          for (int v = 0; v < na; v++) {
+            a[v][dst_r] = a[v][src_r];
+            a[v][dst_l] = a[v][src_l];
+          }
+        }
+    gs[d1][1] += gh;
+    gs[d1][0] -= gh;
+  }
+}
+
+/* 4th-order div B of the interior into divB; returns max |div B|            */
+/* (clean_divB.C:125-162)                                                    */
+static double divb_interior(orc_ctx* c) {
+  const long bs = c->bufsize;
+  const double* U = c->U;
+  double* divB = c->fld[MRM_DIVB];
+  double err = 0.0;
+  for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
+    for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
+      const long off0 = j * c->stride[1] + k * c->stride[2];
+      for (int i = c->lbeg[0]; i <= c->lend[0]; i++) divB[off0 + i] = 0.0;
+      for (int v = 0; v < 3; v++) {
+        const double ww1 = 8. / (12. * c->dx[v]), ww2 = -1. / (12. * c->dx[v]);
+        const long s = c->stride[v];
+        const double* B = U + (5 + v) * bs;
+        for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+          const long n = off0 + i;
+          divB[n] += (ww1 * (B[n + s] - B[n - s]) + ww2 * (B[n + 2 * s] - B[n - 2 * s]));
+        }
+      }
+      for (int i = c->lbeg[0]; i <= c->lend[0]; i++) err = smax(err, fabs(divB[off0 + i]));
+    }
+  return err;
+}
+
+/* Clean_divB (clean_divB.C:35-369) on one rank: a damped potential phi,     */
+/* kept from call to call, is driven by the 4th-order div B and its gradient  */
+/* is added to B (its work phi div B to e); the number of passes adapts to   */
+/* how much each call reduces max |div B|.  err = {before, after} in the     */
+/* reference's units.                                                        */
+void orc_clean_divb(orc_ctx* c, const mrm_run_desc* run, double err_out[2]) {
+  (void)run;
+  const long bs = c->bufsize;
+  const int it_max = (int)c->ph.divB[1];
+  const double divB_err = c->ph.divB[2];
+  const double divB_coeff = 1.0, divB_damp = 0.65;
+  double err_unit = 0.0;
+  for (int v = 0; v < 3; v++) err_unit += 1.0 / c->dx[v];
+  err_unit = 0.5 / err_unit * 3.5449;
+  double dxmin = smin(c->dx[0], c->dx[1]);
+  double dxmax = smax(c->dx[0], c->dx[1]);
+  dxmin = smin(dxmin, c->dx[2]);
+  dxmax = smax(dxmin, c->dx[2]);   /* clean_divB.C:95-98 compares dx[2] with dxmin */
+  const double divB_fac = divB_coeff * smin(dxmin, 0.4 * dxmax) * dxmin;
+  double* phi = c->fld[MRM_PHI];
+  const double* divB = c->fld[MRM_DIVB];
+  double* Bv[3] = {c->U + 5 * bs, c->U + 6 * bs, c->U + 7 * bs};
+
+  exchange_arrays(c, Bv, 3);
+  const double err0 = divb_interior(c);
+  double err = err0;
+  int iter;
+  for (iter = 0; iter < c->divb_next_iter; iter++) {
+    for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
+      for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
+        const long off0 = j * c->stride[1] + k * c->stride[2];
+        for (int i = c->lbeg[0]; i <= c->lend[0]; i++)
+          phi[off0 + i] += divB_fac * divB[off0 + i] - divB_damp * phi[off0 + i];
+      }
+    exchange_arrays(c, &phi, 1);
+    /* vertical boundaries: phi antisymmetric (clean_divB.C:186-217) */
+    for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
+      for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
+        const long off0 = j * c->stride[1] + k * c->stride[2];
+        const int lb = c->lbeg[0], le = c->lend[0];
+        phi[off0 + lb - 1] = -phi[off0 + lb];
+        phi[off0 + lb - 2] = -phi[off0 + lb + 1];
+        phi[off0 + le + 1] = -phi[off0 + le];
+        phi[off0 + le + 2] = -phi[off0 + le - 1];
+      }
+    for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
+      for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
+        const long off0 = j * c->stride[1] + k * c->stride[2];
+        for (int v = 0; v < 3; v++) {
+          const double ww1 = 8. / (12. * c->dx[v]), ww2 = -1. / (12. * c->dx[v]);
+          const long s = c->stride[v];
+          for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+            const long n = off0 + i;
+            Bv[v][n] += (ww1 * (phi[n + s] - phi[n - s]) + ww2 * (phi[n + 2 * s] - phi[n - 2 * s]));
+          }
+        }
+        for (int i = c->lbeg[0]; i <= c->lend[0]; i++) c->U[4 * bs + off0 + i] += phi[off0 + i] * divB[off0 + i];
+      }
+    exchange_arrays(c, Bv, 3);
+    err = divb_interior(c);
+  }
+  const double e0 = err0 * err_unit, e1 = err * err_unit;
+  const double ratio = e1 / e0;
+  if (ratio > divB_err) c->divb_next_iter += 1;
+  else c->divb_next_iter -= 1;
+  c->divb_next_iter = imin(c->divb_next_iter, it_max);
+  c->divb_next_iter = imax(1, c->divb_next_iter);
+  if (err_out) { err_out[0] = e0; err_out[1] = e1; }
+}
+
+/* ------------------------------------------------------------------------ */
 /* set_p_s_bc (boundary_pdmp_1_fftw3.C:496-565) on one rank                  */
 static void set_p_s_bc(orc_ctx* c) {
   const long bs = c->bufsize;
@@ -1102,7 +1229,7 @@ void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage)
 }
 
 /* ------------------------------------------------------------------------ */
-/* One ComputeSolution iteration (solver.C:115-365), RT/divB/IO off          */
+/* One ComputeSolution iteration (solver.C:115-365), RT and IO off          */
 int orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
   for (int it = 0; it < nsteps; it++) {
     double cfl = 0.0;
@@ -1126,6 +1253,7 @@ int orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
       orc_exchange_grid(c);
       orc_tvdlimit(c, run, 0.5 * run->dt);
     }
+    if (c->ph.divB[0] > 0) orc_clean_divb(c, run, NULL);
     orc_tcheck(c, run);
     orc_exchange_grid(c);
     orc_set_boundary_conditions(c, run, MAXSTAGE + 1);

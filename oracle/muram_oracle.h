@@ -46,6 +46,7 @@ void   orc_tcheck(orc_ctx* c, const mrm_run_desc* run);
 void   orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd);
 void   orc_exchange_grid(orc_ctx* c);
 void   orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage);
+void   orc_clean_divb(orc_ctx* c, const mrm_run_desc* run, double err[2]);
 int    orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps);
 
 /* scalar EOS interpolations of eos_init.C:216-416 */

@@ -34,6 +34,7 @@
 #include "exchange.H"
 #include "icbc.H"
 #include "comm_split.H"
+#include "divB.H"
 
 #include "../include/muram_step.h"
 
@@ -42,7 +43,7 @@ extern double p_bc, s_bc;
 enum {
   OP_STARTUP = 0, OP_SAVECONS = 1, OP_C2P = 2, OP_VALF = 3, OP_RESID = 4,
   OP_SYNC = 5, OP_SRCINT = 6, OP_EXCHANGE = 7, OP_BC = 8, OP_TVD = 9,
-  OP_TCHECK = 10, OP_STEP = 11, OP_ADVANCE = 12
+  OP_TCHECK = 10, OP_STEP = 11, OP_ADVANCE = 12, OP_DIVB = 13
 };
 #define OP_DUMP 0x100
 
@@ -99,6 +100,7 @@ static void full_step(RunData& Run, GridData& Grid, const PhysicsData& Physics) 
     exchange_grid_acc(Grid, Physics, 0);
     TVDlimit(Run, Grid, Physics, 0.5 * Run.dt);
   }
+  if (Physics.divB[i_divB_switch] > 0) Clean_divB(Run, Grid, Physics);
   TCheck(Run, Grid, Physics);
   exchange_grid_acc(Grid, Physics, 0);
   bc(Run, Grid, Physics, maxstage + 1);
@@ -146,7 +148,7 @@ int main(int argc, char** argv) {
   memcpy(Physics.dmp, pd.dmp, sizeof pd.dmp);
   memcpy(Physics.rt, pd.rt, sizeof pd.rt);
   Physics.rt_ext[i_ext_cor] = pd.rt_ext_cor;
-  Physics.divB[i_divB_switch] = 0;
+  memcpy(Physics.divB, pd.divB, sizeof pd.divB);
   Physics.Init();
 
   GridData Grid;
@@ -163,7 +165,7 @@ int main(int argc, char** argv) {
   memset(Grid.U0, 0, sizeof(cState) * bs);
   memset(Grid.Qtot, 0, sizeof(double) * bs);
   memset(Grid.Tau, 0, sizeof(double) * bs);
-  for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb})
+  for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb, Grid.divB, Grid.phi})
     memset(a, 0, sizeof(double) * bs);
 
   std::vector<double> soa(8L * bs);
@@ -198,6 +200,7 @@ int main(int argc, char** argv) {
       case OP_TCHECK: TCheck(Run, Grid, Physics); break;
       case OP_STEP: for (int s = 0; s < arg; s++) full_step(Run, Grid, Physics); break;
       case OP_ADVANCE: Run.Advance(); break;
+      case OP_DIVB: Clean_divB(Run, Grid, Physics); break;
       default: fprintf(stderr, "harness: bad op %d\n", op); MPI_Abort(MPI_COMM_WORLD, 2);
     }
     if (ops[2 * o] & OP_DUMP) {
@@ -212,7 +215,8 @@ int main(int argc, char** argv) {
       fwrite(soa.data(), 8, 8L * bs, out);
       for (int v = 0; v < 8; v++) for (long n = 0; n < bs; n++) soa[(long)v * bs + n] = Rd[8 * n + v];
       fwrite(soa.data(), 8, 8L * bs, out);
-      for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb}) fwrite(a, 8, bs, out);
+      for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb, Grid.divB, Grid.phi})
+        fwrite(a, 8, bs, out);
     }
   }
   fclose(out);

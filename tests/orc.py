@@ -61,6 +61,7 @@ class Oracle:
             L.orc_exchange_grid.argtypes = [P]
             L.orc_set_boundary_conditions.argtypes = [P, C.POINTER(RunDesc), C.c_int]
             L.orc_step.argtypes = [P, C.POINTER(RunDesc), C.c_double, C.c_int]
+            L.orc_clean_divb.argtypes = [P, C.POINTER(RunDesc), dp]
             for n in ("orc_p_interp", "orc_s_interp", "orc_T_interp", "orc_d3_interp", "orc_eps3_interp"):
                 getattr(L, n).argtypes = [P, C.c_double, C.c_double]
                 getattr(L, n).restype = C.c_double
@@ -112,6 +113,11 @@ class Oracle:
     def set_bc(self, p, s):
         self.L.orc_set_bc(self.h, p, s)
 
+    def clean_divb(self, run):
+        err = np.zeros(2)
+        self.L.orc_clean_divb(self.h, C.byref(run), _as_dp(err))
+        return err
+
     def __getattr__(self, name):
         fn = getattr(self.L, "orc_" + name)
         return lambda *a: fn(self.h, *a)
@@ -119,7 +125,7 @@ class Oracle:
 
 # --------------------------------------------------------------------------
 OP = dict(STARTUP=0, SAVECONS=1, C2P=2, VALF=3, RESID=4, SYNC=5, SRCINT=6, EXCHANGE=7,
-          BC=8, TVD=9, TCHECK=10, STEP=11, ADVANCE=12)
+          BC=8, TVD=9, TCHECK=10, STEP=11, ADVANCE=12, DIVB=13)
 DUMP = 0x100
 
 
@@ -167,7 +173,9 @@ def run_reference(grid: GridDesc, phys: PhysDesc, run: RunDesc, table: bytes, U:
         off += 64 * bs
         Fo = np.frombuffer(raw, np.float64, 5 * bs, off).reshape((5,) + shape[1:]).copy()
         off += 40 * bs
-        dumps.append(dict(op=o, sc=sc, U=Uo, Res=Ro, fields=Fo))
+        Do = np.frombuffer(raw, np.float64, 2 * bs, off).reshape((2,) + shape[1:]).copy()
+        off += 16 * bs
+        dumps.append(dict(op=o, sc=sc, U=Uo, Res=Ro, fields=Fo, divB=Do[0], phi=Do[1]))
     return dumps
 
 

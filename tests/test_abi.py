@@ -29,5 +29,5 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_sizes_match_c_layout():
     assert C.sizeof(abi.GridDesc) == 9 * 4 + 4 + 6 * 8   # 9 ints, pad to 8, 6 doubles
-    assert C.sizeof(abi.PhysDesc) == (9 + 5 + 4 + 4 + 10 + 2 + 2 + 2 + 4 + 4 + 4 + 10) * 8 + 8
+    assert C.sizeof(abi.PhysDesc) == (9 + 5 + 4 + 4 + 10 + 2 + 2 + 2 + 4 + 4 + 4 + 10 + 3) * 8 + 8
     assert C.sizeof(abi.RunDesc) == 5 * 8 + 4 * 4

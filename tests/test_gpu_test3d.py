@@ -1,8 +1,8 @@
 """The reference's Test_3D initial state (a 192 x 10 x 10 crop of
 TEST/Test_3D/ini, solar-convection stratification from rho 9e-16 to 5e-6
 g/cm^3) restored through the restart-file path and stepped on the GPU:
-FP64 (fused and drop-in sequence, with and without va_adjust) against the
-oracle at 1e-12, a BackupSolution/RestoreSolution round trip, and the
+FP64 (fused and drop-in sequence, with and without va_adjust and divB
+cleaning) against the oracle at 1e-12, a BackupSolution/RestoreSolution round trip, and the
 FP32-mixed form's error against FP64 on real data."""
 import os
 
@@ -38,11 +38,11 @@ def restart_dir(tmp_path, crop):
     return d, config.grid_desc((n0, n1, n2), tuple(crop["dx"]))
 
 
-def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused"):
+def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused", divb=True):
     saved = os.environ.get("MRM_STEP_PATH")
     os.environ["MRM_STEP_PATH"] = path
     try:
-        m = MhdGrid(g, config.test3d_phys(va_adjust), precision)
+        m = MhdGrid(g, config.test3d_phys(va_adjust, divb), precision)
     finally:
         if saved is None:
             os.environ.pop("MRM_STEP_PATH", None)
@@ -56,15 +56,16 @@ def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused"):
     return m, r
 
 
-@pytest.mark.parametrize("va_adjust,path", [(1.0, "fused"), (1.0, "sequence"), (0.0, "fused")])
-def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, path):
-    # va_adjust 1 is the case's own setting
-    print(f"case: Test_3D crop 192x10x10, restore + 4 FP64 {path} steps vs oracle, va_adjust {va_adjust}",
-          flush=True)
+@pytest.mark.parametrize("va_adjust,divb,path", [(1.0, True, "fused"), (1.0, True, "sequence"),
+                                                (1.0, False, "fused"), (0.0, False, "fused")])
+def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, divb, path):
+    # va_adjust 1 with divB cleaning is the case's own setting
+    print(f"case: Test_3D crop 192x10x10, restore + 4 FP64 {path} steps vs oracle, va_adjust {va_adjust},"
+          f" divB {divb}", flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m, r = restored(d, g, table, va_adjust=va_adjust, path=path)
+    m, r = restored(d, g, table, va_adjust=va_adjust, path=path, divb=divb)
     assert (r.globiter, r.time) == (0, 0.0)
-    o = Oracle(g, config.test3d_phys(va_adjust), table)
+    o = Oracle(g, config.test3d_phys(va_adjust, divb), table)
     ro = config.run_desc()
     o.set_state(mhd.read_prim(d, "result", 0, g, m.shape))
     o.set_bc(-1.0, -1.0)
@@ -109,10 +110,11 @@ def test_backup_restore_round_trip(tmp_path, crop, table):
 
 @pytest.mark.parametrize("va_adjust", [0.0, 1.0])
 def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust):
-    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}", flush=True)
+    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}, no divB cleaning",
+          flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m64, r64 = restored(d, g, table, va_adjust=va_adjust)
-    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust)
+    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=False)
+    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=False)
     m64.step(r64, config.DT_RAD_OFF, 20)
     mx.step(rx, config.DT_RAD_OFF, 20)
     errs = field_errors(mx.get_state(), m64.get_state(), interior=(slice(2, -2),) * 3)

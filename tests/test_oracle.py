@@ -278,3 +278,97 @@ def test_eos_lookup_exact_on_bilinear_table():
         want = ((1 - te) * (1 - tr) * f(i, j) + te * (1 - tr) * f(i + 1, j)
                 + (1 - te) * tr * f(i, j + 1) + te * tr * f(i + 1, j + 1))
         assert got == pytest.approx(want, abs=1e-5)
+
+
+# ---------------------------------------------------------------- Clean_divB
+DIVB_GSIZE, DIVB_DX = (12, 10, 8), (3.2e6, 3.2e6, 4.8e6)
+
+
+@pytest.fixture(scope="module")
+def gold_divb():
+    return np.load(os.path.join(GOLD, "ref_divb.npz"))
+
+
+def divb_phys(va_max=3.0e8, va_adjust=0.0):
+    p = config.stratified_phys(va_max=va_max, va_adjust=va_adjust)
+    p.divB[:] = [1, 5, 0.2]
+    return p
+
+
+def test_golden_clean_divb_calls(gold_divb, table):
+    """four successive calls: phi persists, the pass count adapts (3, 4, ...)"""
+    o = Oracle(config.grid_desc(DIVB_GSIZE, DIVB_DX), divb_phys(), table)
+    r = config.run_desc()
+    o.set_state(gold_divb["op_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    for n in range(4):
+        err = o.clean_divb(r)
+        assert np.array_equal(o.field(config.DIVB_FIELD), gold_divb[f"op{n}_divB"]), n
+        assert np.array_equal(o.field(config.PHI_FIELD), gold_divb[f"op{n}_phi"]), n
+        if f"op{n}_U" in gold_divb.files:
+            assert np.array_equal(o.state(0), gold_divb[f"op{n}_U"]), n
+        assert err[1] < err[0]
+
+
+@pytest.mark.parametrize("case,va_max,vadj", [("steps", 3.0e7, 0.0), ("steps_vadj", 1.0e6, 1.0)])
+def test_golden_full_steps_with_clean_divb(gold_divb, table, case, va_max, vadj):
+    o = Oracle(config.grid_desc(DIVB_GSIZE, DIVB_DX), divb_phys(va_max, vadj), table)
+    r = config.run_desc(CFL=0.8)
+    o.set_state(gold_divb[f"{case}_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    assert o.step(r, 1e30, 1) == 0
+    assert np.array_equal(o.state(0), gold_divb[f"{case}_step1_U"])
+    assert r.dt == gold_divb[f"{case}_step1_sc"][0]
+    assert o.step(r, 1e30, 4) == 0
+    assert np.array_equal(o.state(0), gold_divb[f"{case}_step5_U"])
+    assert np.array_equal(o.field(config.PHI_FIELD), gold_divb[f"{case}_step5_phi"])
+    assert r.time == gold_divb[f"{case}_step5_sc"][1]
+
+
+def test_clean_divb_leaves_a_solenoidal_field_alone(table):
+    """B = const has zero discrete divergence: phi stays 0, U is unchanged"""
+    g = config.grid_desc(DIVB_GSIZE, DIVB_DX)
+    o = Oracle(g, divb_phys(), table)
+    tab = eos_table.load_tables(table)
+    U = initial.make_state(tab, initial.BoxSpec(gsize=DIVB_GSIZE, dx=DIVB_DX, seed=5))
+    U[5], U[6], U[7] = 40.0, -15.0, 7.0
+    o.set_state(U)
+    r = config.run_desc()
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    before = o.state(0).copy()
+    err = o.clean_divb(r)
+    assert err[0] == 0.0 and err[1] == 0.0
+    assert np.array_equal(o.state(0), before)
+    assert not o.field(config.PHI_FIELD).any()
+
+
+def test_clean_divb_removes_a_gradient_field(table):
+    """a curl-free perturbation grad(chi) of a uniform field is driven back:
+    max |div B| falls call after call, and the field is pulled towards the
+    uniform one (a sign error in the correction would grow it)"""
+    g = config.grid_desc(DIVB_GSIZE, DIVB_DX)
+    o = Oracle(g, divb_phys(), table)
+    tab = eos_table.load_tables(table)
+    U = initial.make_state(tab, initial.BoxSpec(gsize=DIVB_GSIZE, dx=DIVB_DX, seed=6))
+    U[5], U[6], U[7] = 40.0, 0.0, 0.0
+    rng = np.random.default_rng(9)
+    chi = rng.standard_normal(U.shape[1:]) * 2.0 * DIVB_DX[0]
+    gr = np.gradient(chi, DIVB_DX[2], DIVB_DX[1], DIVB_DX[0])
+    U[5] += gr[2]
+    U[6] += gr[1]
+    U[7] += gr[0]
+    o.set_state(U)
+    r = config.run_desc()
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    inner = (slice(2, -2),) * 3
+    dev0 = np.abs(o.state(0)[6][inner]).max()
+    prev = np.inf
+    for _ in range(4):
+        err = o.clean_divb(r)
+        assert err[1] < err[0] <= prev * (1 + 1e-12)
+        prev = err[1]
+    assert np.abs(o.state(0)[6][inner]).max() < dev0
