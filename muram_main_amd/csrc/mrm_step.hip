@@ -798,12 +798,12 @@ static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi) {
   double* divB = c->dF + MRM_DIVB * G.bs;
   double* phi = c->dF + MRM_PHI * G.bs;
   unsigned long long* red = c->d_red;
-  const unsigned nb = nblk(ninterior(c));
+  const unsigned nb = divb_blocks(G);
   const int npass = c->divb_next_iter;
   TScope ts(c, T_DIVB);
   HIPCHK(c, hipMemsetAsync(red + RED_DIVB0, 0, 2 * sizeof(unsigned long long), c->stream));
   if ((rc = exchange_arrays(c, U + 5 * G.bs, 3))) return rc;
-  k_divb<true><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, divB, phi, red + RED_DIVB0);
+  k_divb<true><<<nb, DIVB_TX * DIVB_TY, 0, c->stream>>>(A, U, divB, phi, red + RED_DIVB0);
   LAUNCHCHK(c, "k_divb");
   for (int it = 0; it < npass; it++) {
     const bool last = it == npass - 1;
@@ -811,12 +811,12 @@ static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi) {
     const long ncol = (long)c->lsize[1] * c->lsize[2];
     k_phi_vertical<<<nblk(ncol), MRM_BLOCK, 0, c->stream>>>(G, phi);
     LAUNCHCHK(c, "k_phi_vertical");
-    if (last && epi) k_divb_update<true><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, phi, divB, *epi);
-    else k_divb_update<false><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, phi, divB, TvdEpilogue{});
+    if (last && epi) k_divb_update<true><<<nb, DIVB_TX * DIVB_TY, 0, c->stream>>>(A, U, phi, divB, *epi);
+    else k_divb_update<false><<<nb, DIVB_TX * DIVB_TY, 0, c->stream>>>(A, U, phi, divB, TvdEpilogue{});
     LAUNCHCHK(c, "k_divb_update");
     if ((rc = exchange_arrays(c, U + 5 * G.bs, 3))) return rc;
-    if (last) k_divb<false><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, divB, phi, red + RED_DIVB1);
-    else k_divb<true><<<nb, MRM_BLOCK, 0, c->stream>>>(A, U, divB, phi, red + RED_SCRATCH);
+    if (last) k_divb<false><<<nb, DIVB_TX * DIVB_TY, 0, c->stream>>>(A, U, divB, phi, red + RED_DIVB1);
+    else k_divb<true><<<nb, DIVB_TX * DIVB_TY, 0, c->stream>>>(A, U, divB, phi, red + RED_SCRATCH);
     LAUNCHCHK(c, "k_divb");
   }
   c->divb_pending = true;
