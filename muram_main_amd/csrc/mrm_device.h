@@ -299,17 +299,38 @@ MRM_HD void sweep_residual(const double v[5][MRM_NV], const double p[5], double 
   res[4] += v[2][1] * lrt0 + v[2][2] * lrt1 + v[2][3] * lrt2;
 }
 
+// Resistive part of one sweep along D (mhdres_SR.C:422-462, 742-766): the
+// sweep's share of J = curl B, with the sweep signs -1, +1, -1 of
+// mhdres_SR.C:94 (dB2, dB3: the 4th-order derivatives along D of B_D2, B_D3),
+// and eta times the 4th-order second derivative of every B component along
+// D; s1[q] = B_q(+1) + B_q(-1), s2[q] = B_q(+2) + B_q(-2), c = B at the point.
+template <int D>
+MRM_HD void eta_sweep(double dB2, double dB3, const double c[3], const double s1[3], const double s2[3],
+                      double eta, double ww0, double ww1, double ww2, double J[3], double res[MRM_NV]) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  constexpr double sgn = D == 1 ? 1.0 : -1.0;
+  J[D2] += sgn * dB3;
+  J[D3] += -sgn * dB2;
+#pragma unroll
+  for (int q = 0; q < 3; q++) res[5 + q] += eta * (ww0 * c[q] + ww1 * s1[q] + ww2 * s2[q]);
+}
+
 // The same residual as sweep_residual with a lower register peak: the four
 // neighbour points are fetched through get(s, w[9]) (s = 0,1,3,4 for offsets
 // -2,-1,+1,+2) one pair at a time, and only the differences
 // F(+1)-F(-1) and F(+2)-F(-2) stay live.  Every arithmetic operation is the
 // one sweep_residual (and mhdres_SR.C:361-412) performs.
-template <int D, class GET>
+// With ETA the resistive part of the sweep is added too (eta_sweep; J
+// accumulates the sweep's share of curl B).
+template <int D, bool ETA = false, class GET>
 MRM_HD void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
-                                                  double lf, double res[MRM_NV]) {
+                                                  double lf, double res[MRM_NV], const ResidParams* rp = nullptr,
+                                                  double* J = nullptr) {
   constexpr int D2 = D == 0 ? 1 : 0;
   constexpr int D3 = D == 2 ? 1 : 2;
   double d1[13], d2[13];   // 8 fluxes, 3 stresses, B_D2, B_D3
+  double bs1[3], bs2[3];   // B(+1) + B(-1), B(+2) + B(-2) (ETA only)
   {
     double pa[9], pb[9];
     PtFlux fa, fb;
@@ -323,6 +344,10 @@ MRM_HD void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
     for (int q = 0; q < 3; q++) d1[8 + q] = fb.ms[q] - fa.ms[q];
     d1[11] = pb[5 + D2] - pa[5 + D2];
     d1[12] = pb[5 + D3] - pa[5 + D3];
+    if constexpr (ETA) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) bs1[q] = pb[5 + q] + pa[5 + q];
+    }
   }
   {
     double pa[9], pb[9];
@@ -337,6 +362,10 @@ MRM_HD void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
     for (int q = 0; q < 3; q++) d2[8 + q] = fb.ms[q] - fa.ms[q];
     d2[11] = pb[5 + D2] - pa[5 + D2];
     d2[12] = pb[5 + D3] - pa[5 + D3];
+    if constexpr (ETA) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) bs2[q] = pb[5 + q] + pa[5 + q];
+    }
   }
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
@@ -359,6 +388,8 @@ MRM_HD void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
   res[2] += lrt1;
   res[3] += lrt2;
   res[4] += c[1] * lrt0 + c[2] * lrt1 + c[3] * lrt2;
+  if constexpr (ETA)
+    eta_sweep<D>(dB2, dB3, c + 5, bs1, bs2, rp->eta, rp->ww0[D], rp->ww1[D], rp->ww2[D], J, res);
 }
 
 // ------------------------------------------------------------------ sources

@@ -72,7 +72,9 @@ __device__ __forceinline__ long xcd_remap(long b, long nb) {
 
 // ZWIN = true: dim-2 neighbours from a register window (one wave per SIMD);
 // ZWIN = false: dim-2 neighbours re-read through the caches each plane.
-template <bool ZWIN>
+// ETA: resistive terms (param_eta > 0) in the three sweeps and eta |J|^2 in
+// the energy residual (mhdres_SR.C:422-462).
+template <bool ZWIN, bool ETA = false>
 __global__ void __launch_bounds__(ST_SX* ST_SY, ZWIN ? 1 : 2)
 k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ Pin, const double* __restrict__ U0,
         double* __restrict__ Uout, double* __restrict__ Pout, const double* __restrict__ Qtot) {
@@ -173,14 +175,15 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + 2][tx + s];
       };
-      sweep_residual_lr<0>(getx, w[2], A.RP.w1[0], A.RP.w2[0], lfc, res);
+      double J[3] = {0.0, 0.0, 0.0};
+      sweep_residual_lr<0, ETA>(getx, w[2], A.RP.w1[0], A.RP.w2[0], lfc, res, &A.RP, J);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
       auto gety = [&](int s, double o[9]) {
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + s][tx + 2];
       };
-      sweep_residual_lr<1>(gety, w[2], A.RP.w1[1], A.RP.w2[1], lfc, res);
+      sweep_residual_lr<1, ETA>(gety, w[2], A.RP.w1[1], A.RP.w2[1], lfc, res, &A.RP, J);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
       auto getz = [&](int s, double o[9]) {
@@ -193,9 +196,10 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
           prim_of_raw(r, o);
         }
       };
-      sweep_residual_lr<2>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res);
+      sweep_residual_lr<2, ETA>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res, &A.RP, J);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      if (ETA) acc[4] += A.RP.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
       // sources, Boris, RK update, Tcheck
       const unsigned node = col + (unsigned)k * s2u;
       double u0[MRM_NV], out[MRM_NV];

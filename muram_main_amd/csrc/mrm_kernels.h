@@ -189,7 +189,7 @@ __device__ __forceinline__ void load_pt(const double* __restrict__ U, const doub
 template <int D>
 __device__ __forceinline__ void resid_dir(const double* __restrict__ U, const double* __restrict__ Pr, long bs,
                                           long node, long str, const double vc[MRM_NV], double pc,
-                                          const ResidParams& RP, double acc[MRM_NV]) {
+                                          const ResidParams& RP, double acc[MRM_NV], double J[3]) {
   double v[5][MRM_NV], p[5];
   load_pt(U, Pr, bs, node - 2 * str, v[0], p[0]);
   load_pt(U, Pr, bs, node - str, v[1], p[1]);
@@ -200,6 +200,19 @@ __device__ __forceinline__ void resid_dir(const double* __restrict__ U, const do
   p[2] = pc;
   double res[MRM_NV];
   sweep_residual<D>(v, p, RP.w1[D], RP.w2[D], RP.inv_va2max, res);
+  if (RP.eta > 0.0) {
+    constexpr int D2 = D == 0 ? 1 : 0;
+    constexpr int D3 = D == 2 ? 1 : 2;
+    const double dB2 = RP.w1[D] * (v[3][5 + D2] - v[1][5 + D2]) + RP.w2[D] * (v[4][5 + D2] - v[0][5 + D2]);
+    const double dB3 = RP.w1[D] * (v[3][5 + D3] - v[1][5 + D3]) + RP.w2[D] * (v[4][5 + D3] - v[0][5 + D3]);
+    double s1[3], s2[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      s1[q] = v[3][5 + q] + v[1][5 + q];
+      s2[q] = v[4][5 + q] + v[0][5 + q];
+    }
+    eta_sweep<D>(dB2, dB3, v[2] + 5, s1, s2, RP.eta, RP.ww0[D], RP.ww1[D], RP.ww2[D], J, res);
+  }
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
 }
@@ -220,9 +233,11 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_residual(Geom G, ResidParams RP, 
     double acc[MRM_NV];
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) acc[q] = R[q * bs + node];
-    resid_dir<0>(U, Pr, bs, node, 1, vc, pc, RP, acc);
-    resid_dir<1>(U, Pr, bs, node, G.s1, vc, pc, RP, acc);
-    resid_dir<2>(U, Pr, bs, node, G.s2, vc, pc, RP, acc);
+    double J[3] = {0.0, 0.0, 0.0};
+    resid_dir<0>(U, Pr, bs, node, 1, vc, pc, RP, acc, J);
+    resid_dir<1>(U, Pr, bs, node, G.s1, vc, pc, RP, acc, J);
+    resid_dir<2>(U, Pr, bs, node, G.s2, vc, pc, RP, acc, J);
+    if (RP.eta > 0.0) acc[4] += RP.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) R[q * bs + node] = acc[q];
     cm[0] = smax(0.0, point_cmax(vc, pc, RP.inv_va2max));

@@ -9,8 +9,14 @@ namespace MRM_NS {
 inline hipError_t launch_stage(hipStream_t st, unsigned nb, bool zwin, const StageArgs& A, const double* Uin,
                                const double* Pin, const double* U0, double* Uout, double* Pout,
                                const double* Q) {
-  if (zwin) k_stage<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-  else k_stage<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  if (A.RP.eta > 0.0) {
+    if (zwin) k_stage<true, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+    else k_stage<false, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  } else if (zwin) {
+    k_stage<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  } else {
+    k_stage<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  }
   return hipGetLastError();
 }
 

@@ -197,7 +197,8 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   *out = nullptr;
   if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
-  if (phys->params[3] > 0.0 || phys->params[4] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
+  if (phys->params[3] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
+  if (phys->params[4] > 0.0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
   if (phys->divB[0] > 0.0 && (precision != MRM_PREC_FP64 || phys->divB[1] < 1.0)) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
@@ -586,6 +587,29 @@ static void valf_apply(mrm_ctx* c, double stats[5]) {
 }
 
 // ------------------------------------------------------------------ MHD_Residual
+// the resistive members of ResidParams (mhdres_SR.C:57, 205-207)
+static void eta_params(const mrm_ctx* c, ResidParams& RP) {
+  RP.eta = c->phys.params[4];
+  for (int d = 0; d < 3; d++) {
+    const double dx = c->dx[d];
+    RP.ww0[d] = -30. / (12. * dx * dx);
+    RP.ww1[d] = 16. / (12. * dx * dx);
+    RP.ww2[d] = -1. / (12. * dx * dx);
+  }
+}
+
+// dt_cfl with the resistive limit dt_diff_fac / eta, halved for CFL > 1
+// (mhdres_SR.C:147-160, 815-819)
+static double eta_dt_limit(const mrm_ctx* c, const mrm_run_desc* run, double dt_cfl) {
+  const double eta = c->phys.params[4];
+  if (!(eta > 0.0)) return dt_cfl;
+  const double dt_diff_fac =
+      0.5 / (1. / (c->dx[0] * c->dx[0]) + 1. / (c->dx[1] * c->dx[1]) + 1. / (c->dx[2] * c->dx[2]));
+  double dt_res = dt_diff_fac / eta;
+  if (run && run->CFL > 1) dt_res *= 0.5;
+  return std::min(dt_cfl, dt_res);
+}
+
 static int launch_residual(mrm_ctx* c) {
   ResidParams RP;
   for (int d = 0; d < 3; d++) {
@@ -593,6 +617,7 @@ static int launch_residual(mrm_ctx* c) {
     RP.w2[d] = -1. / (12. * c->dx[d]);
   }
   RP.inv_va2max = c->inv_va2max;
+  eta_params(c, RP);
   HIPCHK(c, hipMemsetAsync(c->d_red + RED_CMAX, 0, sizeof(unsigned long long), c->stream));
   const double* P = c->dF + MRM_PRES * c->G.bs;
   { TScope ts(c, T_RESID);
@@ -626,7 +651,7 @@ extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_
   double cmax;
   rc = read_cmax(c, &cmax);
   if (rc) return rc;
-  if (dt_cfl) *dt_cfl = c->dxmin / cmax;   // mhdres_SR.C:795
+  if (dt_cfl) *dt_cfl = eta_dt_limit(c, run, c->dxmin / cmax);   // mhdres_SR.C:795-819
   return MRM_OK;
 }
 
@@ -1119,6 +1144,7 @@ static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const do
     A.RP.w2[d] = -1. / (12. * c->dx[d]);
   }
   A.RP.inv_va2max = c->inv_va2max;
+  eta_params(c, A.RP);
   A.SP = src_params(c, run, stage);
   A.ntx = (c->lsize[0] + ST_SX - 1) / ST_SX;
   A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
@@ -1178,7 +1204,7 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     if ((rc = read_cmax(c, &cm))) return rc;
   }
   double cfl = 0.0;
-  if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
+  if ((rc = mrm_sync_timestep(c, run, dt_rad, eta_dt_limit(c, run, c->dxmin / cm), &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
   double* U0 = c->S[i0];
   struct { double *in, *pin, *out, *pout; } st[4] = {
@@ -1564,9 +1590,10 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   NEED_FP64(c, "mrm_adjoint_forward");
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
-  if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0)
+  if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0 || c->phys.params[4] > 0.0)
     return fail(c, MRM_ERR_UNSUPPORTED,
-                "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0 and no Clean_divB");
+                "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0, eta = 0 and no "
+                "Clean_divB");
   for (int d = 0; d < 3; d++)
     if (c->lsize[d] < 4) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: interior extents below 4");
   for (double* p : c->adj_ckpt) hipFree(p);

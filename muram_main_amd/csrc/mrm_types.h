@@ -68,6 +68,10 @@ struct TvdParams {
 struct ResidParams {
   double w1[3], w2[3];
   double inv_va2max;
+  // resistivity (param_eta > 0): eta and the 4th-order second-derivative
+  // weights -30, 16, -1 over 12 dx^2 per dimension (mhdres_SR.C:205-207)
+  double eta = 0.0;
+  double ww0[3] = {0, 0, 0}, ww1[3] = {0, 0, 0}, ww2[3] = {0, 0, 0};
 };
 
 struct BcParams {

@@ -66,6 +66,8 @@ struct orc_ctx {
   double last_cfl;
   /* Clean_divB's adaptive iteration count (clean_divB.C:50) */
   int divb_next_iter;
+  /* GridData::curlB (grid.C:307-309), allocated with eta > 0 */
+  double* curlB;
 };
 
 void orc_set_threads(int n) {
@@ -129,6 +131,7 @@ void orc_destroy(orc_ctx* c) {
   for (int t = 0; t < 6; t++) free(c->tab[t]);
   for (int t = 0; t < 2; t++) free(c->itab[t]);
   free(c->hfb); free(c->hft); free(c->qft);
+  free(c->curlB);
   free(c);
 }
 
@@ -411,26 +414,41 @@ void orc_adjust_valf_max(orc_ctx* c, const mrm_run_desc* run, double stats[5]) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* MHD_Residual (mhdres_SR.C:27-854) with eta = Spitzer = ambipolar =        */
+/* MHD_Residual (mhdres_SR.C:27-854) with Spitzer = ambipolar =             */
 /* diagnostics = 0: three sweeps d = 0,1,2 (loop_order :86), each over the   */
-/* interior transverse lines and the full line along d1 incl. ghosts.        */
+/* interior transverse lines and the full line along d1 incl. ghosts.  With  */
+/* eta > 0 each sweep adds its part of J = curl B to curlB (:742-749), the   */
+/* last one adds eta |J|^2 to R.e (:440-451), every sweep adds eta times the */
+/* 4th-order second derivative of B along d1 (:452-462, :757-766), and dt is */
+/* limited by the diffusion time (:815-819).                                 */
 double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
-  (void)run;
   static const int loop_order[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
   static const double del[3][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
+  static const double sign[3] = {-1., 1., -1.};
   const long bs = c->bufsize;
   const double* U = c->U;
   const double* Pg = c->fld[MRM_PRES];
   double* R = c->Res;
   const double inv_va2max = c->inv_va2max;
+  const double eta = c->ph.params[4];
+  const int needs_curlB = eta > 0.0;
   double dxmin = c->dx[0];
   for (int d = 1; d < 3; d++) dxmin = smin(dxmin, c->dx[d]);
+  if (needs_curlB) {
+    if (!c->curlB) c->curlB = (double*)malloc(sizeof(double) * 3 * (size_t)bs);
+    memset(c->curlB, 0, sizeof(double) * 3 * (size_t)bs);
+  }
+  double* curlB = c->curlB;
   double cmax = 0.0;
   for (int d = 0; d < 3; d++) {
     const int d1 = loop_order[d][0], d2 = loop_order[d][1], d3 = loop_order[d][2];
     const double dx = c->dx[d1];
     const double w1 = 8. / (12. * dx);
     const double w2 = -1. / (12. * dx);
+    const double ww0 = -30. / (12. * dx * dx);
+    const double ww1 = 16. / (12. * dx * dx);
+    const double ww2 = -1. / (12. * dx * dx);
+    const double signd1 = sign[d1];
     const double p1 = del[d1][0], p2 = del[d1][1], p3 = del[d1][2];
     const int gh = c->ghosts[d1];
     const int sz = c->lend[d1] - c->lbeg[d1] + 1 + 2 * gh;
@@ -517,6 +535,21 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
             res[2] += lrt1;
             res[3] += lrt2;
             res[4] += var[sz + i] * lrt0 + var[2 * sz + i] * lrt1 + var[3 * sz + i] * lrt2;
+            if (needs_curlB) {
+              double cb[3];
+              cb[d1] = 0;
+              cb[d2] = signd1 * dB3;
+              cb[d3] = -signd1 * dB2;
+              for (int q = 0; q < 3; q++) curlB[q * bs + node] += cb[q];
+              if (d == 2) {
+                const double jx = curlB[node], jy = curlB[bs + node], jz = curlB[2 * bs + node];
+                R[4 * bs + node] += eta * (jx * jx + jy * jy + jz * jz);
+              }
+              for (int v = 5; v <= 7; v++) {
+                const double* X = var + v * sz;
+                res[v] += eta * (ww0 * X[i] + ww1 * (X[i + 1] + X[i - 1]) + ww2 * (X[i + 2] + X[i - 2]));
+              }
+            }
             for (int v = 0; v < NV; v++) R[v * bs + node] += res[v];
           }
         }
@@ -525,7 +558,15 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
     }
     cmax = smax(cmax, cmax_d);
   }
-  return dxmin / cmax;
+  double dt_cfl = dxmin / cmax;
+  if (needs_curlB) {
+    const double dt_diff_fac =
+        0.5 / (1. / (c->dx[0] * c->dx[0]) + 1. / (c->dx[1] * c->dx[1]) + 1. / (c->dx[2] * c->dx[2]));
+    double dt_res = dt_diff_fac / eta;
+    if (run && run->CFL > 1) dt_res *= 0.5;
+    dt_cfl = smin(dt_cfl, dt_res);
+  }
+  return dt_cfl;
 }
 
 /* Sync_timestep (solver.C:476-548) on one rank */

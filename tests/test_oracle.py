@@ -372,3 +372,86 @@ def test_clean_divb_removes_a_gradient_field(table):
         assert err[1] < err[0] <= prev * (1 + 1e-12)
         prev = err[1]
     assert np.abs(o.state(0)[6][inner]).max() < dev0
+
+
+# ---------------------------------------------------------------- resistivity
+@pytest.fixture(scope="module")
+def gold_eta():
+    return np.load(os.path.join(GOLD, "ref_eta.npz"))
+
+
+def eta_phys(eta):
+    p = config.stratified_phys()
+    p.params[config.I_PARAM_ETA] = eta
+    return p
+
+
+@pytest.mark.parametrize("case", ["eta_cfl", "eta_diff"])
+def test_golden_resistive_residual_and_steps(gold_eta, table, case):
+    g = config.grid_desc(DIVB_GSIZE, DIVB_DX)
+    p = eta_phys(float(gold_eta[f"{case}_eta"]))
+    o = Oracle(g, p, table)
+    r = config.run_desc(CFL=0.8)
+    o.set_state(gold_eta[f"{case}_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.cons_to_prim()
+    o.adjust_valf_max(r, None)
+    dt = o.mhd_residual(r)
+    assert np.array_equal(o.state(2), gold_eta[f"{case}_resid_R"])
+    assert dt == gold_eta[f"{case}_resid_sc"][2]
+    o = Oracle(g, p, table)
+    r = config.run_desc(CFL=0.8)
+    o.set_state(gold_eta[f"{case}_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.step(r, 1e30, 1)
+    assert np.array_equal(o.state(0), gold_eta[f"{case}_step1_U"])
+    o.step(r, 1e30, 4)
+    assert np.array_equal(o.state(0), gold_eta[f"{case}_step5_U"])
+    assert r.dt == gold_eta[f"{case}_step5_sc"][0]
+
+
+def test_resistive_residual_on_a_sine_field(table):
+    """B = (B0, 0, A sin(k y)) on a uniform gas at rest: the resistive terms of
+    the residual have closed forms through the symbols of the 4th-order
+    stencils, R(Bz) = eta D2(k) A sin(ky) and R(e) = eta (D1(k) A cos(ky))^2,
+    with D1(k) = (8 sin(k dy) - sin(2 k dy)) / (6 dy) and
+    D2(k) = (-30 + 32 cos(k dy) - 2 cos(2 k dy)) / (12 dy^2); the diffusion
+    limit sets dt_cfl = dt_diff_fac / eta (halved for CFL > 1)"""
+    n1, dy, amp, eta = 16, 3.2e6, 20.0, 3.0e11
+    g = config.grid_desc((12, n1, 8), (3.2e6, dy, 3.2e6))
+    p = eta_phys(eta)
+    p.params[config.I_PARAM_GRAV] = 0.0
+    o = Oracle(g, p, table)
+    tab = eos_table.load_tables(table)
+    U = initial.make_state(tab, initial.BoxSpec(gsize=(12, n1, 8), dx=(3.2e6, dy, 3.2e6), seed=1))
+    inner = (slice(2, -2),) * 3
+    rho, e = U[0][inner].mean(), U[4][inner].mean()
+    U[0], U[4] = rho, e
+    U[1:4] = 0.0
+    k = 2 * np.pi / (n1 * dy)
+    y = (np.arange(n1 + 4) - 2 + 0.5) * dy
+    U[5], U[6] = 50.0, 0.0
+    U[7] = amp * np.sin(k * y)[None, :, None]
+    o.set_state(U)
+    r = config.run_desc(CFL=0.8)
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.cons_to_prim()
+    o.adjust_valf_max(r, None)
+    dt = o.mhd_residual(r)
+    R = o.state(2)
+    D1 = (8 * np.sin(k * dy) - np.sin(2 * k * dy)) / (6 * dy)
+    D2 = (-30 + 32 * np.cos(k * dy) - 2 * np.cos(2 * k * dy)) / (12 * dy * dy)
+    mid = (slice(2, -2), slice(2, -2), slice(4, -4))     # away from the vertical boundaries
+    Bz = U[7][mid]
+    Jx = D1 * amp * np.cos(k * y)[None, :, None] * np.ones(U[7].shape)
+    np.testing.assert_allclose(R[7][mid], eta * D2 * Bz, rtol=1e-9, atol=1e-12 * eta * abs(D2) * amp)
+    np.testing.assert_allclose(R[4][mid], eta * Jx[mid] ** 2, rtol=1e-9, atol=1e-12 * eta * (D1 * amp) ** 2)
+    assert np.abs(R[5][mid]).max() <= 1e-12 * eta * abs(D2) * amp
+    fac = 0.5 / (2 / 3.2e6 ** 2 + 1 / dy ** 2)
+    assert dt == pytest.approx(min(dt, fac / eta), rel=1e-15)
+    r2 = config.run_desc(CFL=1.2)
+    dt2 = o.mhd_residual(r2)
+    assert dt2 <= 0.5 * fac / eta * (1 + 1e-15)
