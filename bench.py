@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--path", default="fused", choices=["fused", "sequence"])
     ap.add_argument("--va-adjust", type=float, default=None,
                     help="override param_va_adjust of the workload (1: Adjust_Valf_Max's adaptive limiter)")
+    ap.add_argument("--eta", type=float, default=None,
+                    help="param_eta (resistivity, cm^2/s); default: the workload's (0)")
     ap.add_argument("--divb", action="store_true",
                     help="Clean_divB on (divB_switch 1, at most 5 passes, error target 0.2)")
     ap.add_argument("--stage-zwin", default=None, choices=["0", "1"])
@@ -364,6 +366,8 @@ def main():
         wl.phys.params[config.I_PARAM_VA_ADJUST] = args.va_adjust
     if args.divb:
         wl.phys.divB[:] = [1, 5, 0.2]
+    if args.eta is not None:
+        wl.phys.params[config.I_PARAM_ETA] = args.eta
     box = wl.box
     t = time.perf_counter()
     spec = eos_table.TableSpec(n_eps=args.table_n, n_lr=args.table_n, n_lp3=200, n_s3=150)
@@ -404,7 +408,7 @@ def main():
                    "precision": "fp64 plain" if main_form == "fp64" else "fp32-mixed",
                    "steps_timed": args.steps, "dt_s": R["dt_s"],
                    "path": args.path, "va_adjust": wl.phys.params[config.I_PARAM_VA_ADJUST],
-                   "divb": bool(wl.phys.divB[0] > 0),
+                   "divb": bool(wl.phys.divB[0] > 0), "eta": wl.phys.params[config.I_PARAM_ETA],
                    "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "1"),
                    "fma": os.environ.get("MRM_FMA", "1")},
         "hbm_frac_model": R["hbm_frac_model"],

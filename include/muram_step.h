@@ -51,7 +51,8 @@ extern "C" {
 #define MRM_TAU  6
 #define MRM_DIVB 7   /* Clean_divB's divergence and potential (grid.H:78-80) */
 #define MRM_PHI  8
-#define MRM_NFIELD 9
+#define MRM_SFLX 9   /* Spitzer heat flux along B (grid.H:71, result_prim_8) */
+#define MRM_NFIELD 10
 
 /* host array layouts for mrm_set_state / mrm_get_state */
 #define MRM_LAYOUT_AOS 0   /* cState[bufsize]: 8 doubles per cell (mhd3d.H:24-27) */
@@ -175,8 +176,10 @@ int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage
  * RT and IO off: SaveCons, 4 x (ConsToPrim, Adjust_Valf_Max at stage 1,
  * MHD_Residual, Sync_timestep at stage 1, Source_Integrate_Tcheck, exchange,
  * BC), TVDlimit (split in two halves when cfl > CFL_tvd), Clean_divB when
- * phys.divB[0] > 0, TCheck, exchange, BC, Run.Advance.  dt_rad is the radiative time step the caller
- * supplies (RT is outside the path). */
+ * phys.divB[0] > 0, TCheck, exchange, BC, Run.Advance.  dt_rad is the
+ * radiative time step the caller supplies (RT is outside the path).  With
+ * resistivity the stage kernels add the resistive terms; with Spitzer
+ * conduction the step runs the per-operation kernels. */
 int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
 
 /* Restart files of one rank holding the whole grid, host side (no context):
@@ -187,6 +190,11 @@ int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
  * conservative, 8 x bufsize with ghosts; reading zeroes the ghosts. */
 int mrm_io_read_prim(const char* dir, const char* resfile, int iter, const mrm_grid_desc* grid, double* U);
 int mrm_io_write_prim(const char* dir, const char* resfile, int iter, const mrm_grid_desc* grid, const double* U);
+/* One variable file result_prim_<v>.<iter> to or from a field of bufsize
+ * doubles (v = 8: Spitzer's heat flux sflx, io_xysl.C:322-336, 389-392). */
+int mrm_io_read_var(const char* dir, const char* resfile, int v, int iter, const mrm_grid_desc* grid, double* f);
+int mrm_io_write_var(const char* dir, const char* resfile, int v, int iter, const mrm_grid_desc* grid,
+                     const double* f);
 int mrm_io_write_header(const char* dir, int iter, const mrm_grid_desc* grid, double time, double dt,
                         double va_max);
 int mrm_io_read_header(const char* dir, int iter, int gsize[3], double dx[3], double* time, double* dt,

@@ -17,7 +17,7 @@ ERRORS = {-1: "MRM_ERR_ARG", -2: "MRM_ERR_HIP", -3: "MRM_ERR_EOS", -4: "MRM_ERR_
           -5: "MRM_ERR_CFL", -6: "MRM_ERR_STATE", -7: "MRM_ERR_UNSUPPORTED"}
 PREC_FP64, PREC_FP32_MIXED = 0, 1
 U, U0, RES = 0, 1, 2
-PRES, TEMP, NE, RHOI, AMB, QTOT, TAU, DIVB, PHI = range(9)
+PRES, TEMP, NE, RHOI, AMB, QTOT, TAU, DIVB, PHI, SFLX = range(10)
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
 
 # every symbol include/muram_step.h declares
@@ -29,7 +29,7 @@ EXPORTS = [
     "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit", "mrm_clean_divB",
     "mrm_exchange_grid", "mrm_set_boundary_conditions", "mrm_step",
     "mrm_io_read_prim", "mrm_io_write_prim", "mrm_io_write_header", "mrm_io_read_header",
-    "mrm_io_write_backup", "mrm_io_read_backup",
+    "mrm_io_write_backup", "mrm_io_read_backup", "mrm_io_read_var", "mrm_io_write_var",
     "mrm_adjoint_forward", "mrm_adjoint_backward",
     "mrm_timers", "mrm_get_timers", "mrm_timer_name",
 ]
@@ -108,6 +108,8 @@ def load(path: str = LIB_PATH):
         "mrm_io_read_header": ([C.c_char_p, C.c_int, C.POINTER(C.c_int), dp, dp, dp, dp], C.c_int),
         "mrm_io_write_backup": ([C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
         "mrm_io_read_backup": ([C.c_char_p, C.POINTER(C.c_int), dp, dp, dp], C.c_int),
+        "mrm_io_read_var": ([C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(GridDesc), dp], C.c_int),
+        "mrm_io_write_var": ([C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(GridDesc), dp], C.c_int),
         "mrm_adjoint_forward": ([P, C.POINTER(RunDesc), C.c_double, C.c_int, C.c_int], C.c_int),
         "mrm_adjoint_backward": ([P, dp, dp, C.c_int], C.c_int),
         "mrm_timers": ([P, C.c_int], C.c_int),

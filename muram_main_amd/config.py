@@ -25,7 +25,7 @@ I_TVD_RHOLEV, I_TVD_RHOLOG, I_TVD_QRHO, I_TVD_BPAR, I_TVD_VHYP, I_TVD_QDIFF_BND,
 I_TCHK_EPS_MIN, I_TCHK_RHO_MIN, I_TCHK_EPS_MAX, I_TCHK_VMAX = range(4)
 I_RT_TSTEP, I_RT_CFL = 3, 4
 I_DIVB_SWITCH, I_DIVB_ITMAX, I_DIVB_ERR = range(3)
-DIVB_FIELD, PHI_FIELD = 7, 8     # mrm field ids of div B and phi
+DIVB_FIELD, PHI_FIELD, SFLX_FIELD = 7, 8, 9     # mrm field ids of div B, phi, Spitzer heat flux
 
 
 def default_phys() -> PhysDesc:
@@ -68,16 +68,18 @@ def stratified_phys(va_max: float = 3.0e8, va_adjust: float = 0.0) -> PhysDesc:
     return p
 
 
-def test3d_phys(va_adjust: float = 1.0, divb: bool = True) -> PhysDesc:
-    """TEST/Test_3D/parameters.dat with Spitzer conduction, damping and the
-    potential-field top boundary off (outside the path): va_max 1e7 with
-    va_adjust 1 (:51-52), half-open top (:60), B_crit 2.5e30 (:63), hot plate
-    eps_top 2e14 (:64), divB cleaning on with 5 passes at most and a 0.2 error
-    reduction target (:94-96), tchk eps_min 1.75e12, rho_min 1e-18, eps_max
-    2e14, vmax 5e7 (:99-102)."""
+def test3d_phys(va_adjust: float = 1.0, divb: bool = True, spitzer: bool = True) -> PhysDesc:
+    """TEST/Test_3D/parameters.dat with damping, RT and the potential-field
+    top boundary off (outside the path): va_max 1e7 with va_adjust 1
+    (:51-52), Spitzer conduction 1e-6 (:54), half-open top (:60), B_crit
+    2.5e30 (:63), hot plate eps_top 2e14 (:64), divB cleaning on with 5
+    passes at most and a 0.2 error reduction target (:94-96), tchk eps_min
+    1.75e12, rho_min 1e-18, eps_max 2e14, vmax 5e7 (:99-102)."""
     p = stratified_phys(va_max=1.0e7, va_adjust=va_adjust)
     if divb:
         p.divB[:] = [1, 5, 0.2]
+    if spitzer:
+        p.params[I_PARAM_SPITZER] = 1.0e-6
     p.bnd[I_BND_BCRIT] = 2.5e30
     p.bnd[I_BND_EPS_TOP] = 2.0e14
     p.tchk[:] = [1.75e12, 1.0e-18, 2.0e14, 5.0e7]

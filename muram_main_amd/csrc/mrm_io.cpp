@@ -110,6 +110,44 @@ extern "C" int mrm_io_write_prim(const char* dir, const char* resfile, int iter,
   return MRM_OK;
 }
 
+// one result_prim_<v> file (v = 8: Spitzer's heat flux, io_xysl.C:322-336,
+// 389-392) to or from a field of bufsize doubles, ghosts zero on reading
+extern "C" int mrm_io_read_var(const char* dir, const char* resfile, int v, int iter, const mrm_grid_desc* grid,
+                               double* f) {
+  Dims D;
+  if (!dir || !resfile || !f || v < 0 || !dims_of(grid, D)) return MRM_ERR_ARG;
+  const long n_int = (long)D.gs[0] * D.gs[1] * D.gs[2];
+  std::vector<float> buf(n_int);
+  FILE* fh = std::fopen(prim_name(dir, resfile, v, iter).c_str(), "rb");
+  if (!fh) return MRM_ERR_ARG;
+  const size_t got = std::fread(buf.data(), sizeof(float), n_int, fh);
+  std::fclose(fh);
+  if ((long)got != n_int) return MRM_ERR_ARG;
+  std::memset(f, 0, sizeof(double) * D.bs);
+  for (int k = 0; k < D.gs[2]; k++)
+    for (int j = 0; j < D.gs[1]; j++)
+      for (int i = 0; i < D.gs[0]; i++)
+        f[node(D, i, j, k)] = (double)buf[i + (long)D.gs[0] * (j + (long)D.gs[1] * k)];
+  return MRM_OK;
+}
+
+extern "C" int mrm_io_write_var(const char* dir, const char* resfile, int v, int iter, const mrm_grid_desc* grid,
+                                const double* f) {
+  Dims D;
+  if (!dir || !resfile || !f || v < 0 || !dims_of(grid, D)) return MRM_ERR_ARG;
+  const long n_int = (long)D.gs[0] * D.gs[1] * D.gs[2];
+  std::vector<float> buf(n_int);
+  for (int k = 0; k < D.gs[2]; k++)
+    for (int j = 0; j < D.gs[1]; j++)
+      for (int i = 0; i < D.gs[0]; i++)
+        buf[i + (long)D.gs[0] * (j + (long)D.gs[1] * k)] = (float)f[node(D, i, j, k)];
+  FILE* fh = std::fopen(prim_name(dir, resfile, v, iter).c_str(), "wb");
+  if (!fh) return MRM_ERR_ARG;
+  const size_t put = std::fwrite(buf.data(), sizeof(float), n_int, fh);
+  std::fclose(fh);
+  return (long)put == n_int ? MRM_OK : MRM_ERR_ARG;
+}
+
 extern "C" int mrm_io_write_header(const char* dir, int iter, const mrm_grid_desc* grid, double time, double dt,
                                    double va_max) {
   if (!dir || !grid) return MRM_ERR_ARG;

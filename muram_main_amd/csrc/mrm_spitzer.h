@@ -1,0 +1,106 @@
+// mrm_spitzer.h -- Spitzer heat conduction in hyperbolic form (param_spitzer
+// > 0) on the device, FP64, for the per-operation entry points: the heat flux
+// along B, sflx, is a cell field with its own residual Rflx and RK update.
+//   residual (mhdres_SR.C:350-359, 486-521, 680-690): the energy flux carries
+//     sflx B_d along every sweep; each sweep damps sflx with a 4th difference
+//     and adds B_d dT/dx_d to B.grad T; then sflx relaxes towards the
+//     saturation-limited Spitzer flux -kappa B.grad T / |B|^2 at the rate
+//     nu_hyp, bounded by the characteristic-speed margin and by avr_max/dt;
+//   integrate / SaveCons (Add_Sources_Integrate_SR.C:410-419, 504-517);
+//   ghosts (boundary_pdmp_1_fftw3.C:306-310, 455-466; exchange.C:111-120).
+#pragma once
+#include "mrm_kernels.h"
+
+namespace MRM_NS {
+
+struct SptParams {
+  double w1[3], w2[3];
+  double k0;          // param_spitzer
+  double sat_flx;     // (1 + X_H) Rgas^1.5 sqrt(1836) / 8 (mhdres_SR.C:46)
+  double hyp_diff, cmax_hyp, nu_hyp_max;   // from the call's dt and CFL (mhdres_SR.C:136-145)
+};
+
+// one interior cell per thread; U primitive (after ConsToPrim), T its
+// temperature over the whole buffer
+__global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_resid(Geom G, SptParams S, const double* __restrict__ U,
+                                                             const double* __restrict__ T,
+                                                             const double* __restrict__ sflx, double* __restrict__ R,
+                                                             double* __restrict__ Rflx) {
+  const long bs = G.bs;
+  int i, j, k;
+  if (!interior_ijk(G, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const long n = i + j * G.s1 + (long)k * G.s2;
+  const long st[3] = {1, G.s1, G.s2};
+  double re = 0.0, rf = Rflx[n], bgt = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const long s = st[d];
+    const double* Bd = U + (5 + d) * bs;
+    const double w1 = S.w1[d], w2 = S.w2[d];
+    re -= w1 * (sflx[n + s] * Bd[n + s] - sflx[n - s] * Bd[n - s]) +
+          w2 * (sflx[n + 2 * s] * Bd[n + 2 * s] - sflx[n - 2 * s] * Bd[n - 2 * s]);
+    rf -= S.hyp_diff * ((sflx[n + 2 * s] + sflx[n - 2 * s]) - 4. * (sflx[n + s] + sflx[n - s]) + 6. * sflx[n]);
+    bgt += Bd[n] * (w1 * (T[n + s] - T[n - s]) + w2 * (T[n + 2 * s] - T[n - 2 * s]));
+  }
+  R[4 * bs + n] += re;
+  const double v0 = U[bs + n], v1 = U[2 * bs + n], v2 = U[3 * bs + n];
+  const double b0 = U[5 * bs + n], b1 = U[6 * bs + n], b2 = U[7 * bs + n];
+  const double vv = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+  const double bb = b0 * b0 + b1 * b1 + b2 * b2;
+  const double tempi = T[n];
+  const double c_lim = S.cmax_hyp - vv;
+  const double tt32 = tempi * sqrt(tempi);
+  double kap = S.k0 * tempi * tt32;
+  const double F_sat = S.sat_flx * U[n] * tt32;
+  const double invBB = 1.0 / ((1e-100 < bb) ? bb : 1e-100);
+  double F = -kap * bgt;
+  const double f_lim = F_sat / (F_sat + fabs(F * sqrt(invBB)));
+  F *= invBB * f_lim;
+  kap *= f_lim;
+  const double nu_c = c_lim * c_lim * U[4 * bs + n] / (tempi * kap);
+  const double nu = (S.nu_hyp_max < nu_c) ? S.nu_hyp_max : nu_c;
+  Rflx[n] = rf + nu * (F - sflx[n]);
+}
+
+// sflx = sflx0 + wdt Rflx, Rflx = 0 (interior)
+__global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_integrate(Geom G, double wdt, double* __restrict__ sflx,
+                                                                 const double* __restrict__ sflx0,
+                                                                 double* __restrict__ Rflx) {
+  int i, j, k;
+  if (!interior_ijk(G, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const long n = i + j * G.s1 + (long)k * G.s2;
+  sflx[n] = sflx0[n] + wdt * Rflx[n];
+  Rflx[n] = 0.0;
+}
+
+// sflx0 = sflx (interior)
+__global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_save(Geom G, const double* __restrict__ sflx,
+                                                            double* __restrict__ sflx0) {
+  int i, j, k;
+  if (!interior_ijk(G, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const long n = i + j * G.s1 + (long)k * G.s2;
+  sflx0[n] = sflx[n];
+}
+
+// vertical ghosts of sflx over every (j,k): zero at the bottom; at the top
+// symmetric under a hot plate, else the top two interior layers and the
+// ghosts zero (zero flux through the boundary)
+__global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_bc(Geom G, int hot_plate, double* __restrict__ sflx) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)G.n1 * G.n2) return;
+  const long base = (t % G.n1) * G.s1 + (t / G.n1) * G.s2;
+  sflx[base + G.lb0 - 1] = 0.0;
+  sflx[base + G.lb0 - 2] = 0.0;
+  const long n1 = base + G.le0, n2 = n1 - 1;
+  if (hot_plate) {
+    sflx[n1 + 1] = sflx[n1];
+    sflx[n1 + 2] = sflx[n2];
+  } else {
+    sflx[n2] = 0.0;
+    sflx[n1] = 0.0;
+    sflx[n1 + 1] = 0.0;
+    sflx[n1 + 2] = 0.0;
+  }
+}
+
+}  // namespace MRM_NS

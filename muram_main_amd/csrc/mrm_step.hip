@@ -18,6 +18,7 @@
 #include "../../include/muram_step.h"
 #include "mrm_launch.h"
 #include "mrm_divb.h"
+#include "mrm_spitzer.h"
 #include "mrm_mixed_api.h"
 #include "mrm_adjoint_api.h"
 
@@ -89,6 +90,9 @@ struct mrm_ctx {
   // errors wait in d_red[RED_DIVB0/1] until the next host round trip
   int divb_next_iter = 3;
   bool divb_pending = false;
+
+  // Spitzer conduction (param_spitzer > 0): sflx0 and Rflx; sflx is field MRM_SFLX
+  double* d_spt = nullptr;
 
   unsigned long long* d_red = nullptr;   // RED_NSLOT slots
   double* d_part = nullptr;
@@ -197,8 +201,8 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   *out = nullptr;
   if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
-  if (phys->params[3] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
-  if (phys->params[4] > 0.0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
+  if (phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
+  if ((phys->params[3] > 0.0 || phys->params[4] > 0.0) && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
   if (phys->divB[0] > 0.0 && (precision != MRM_PREC_FP64 || phys->divB[1] < 1.0)) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
@@ -288,6 +292,11 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   c->fma = !(fm && fm[0] == '0');
   if (!rc && (e = hipMemsetAsync(c->dR, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipMemsetAsync(c->dF, 0, MRM_NFIELD * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
+  if (!rc && phys->params[3] > 0.0) {   // Spitzer: sflx0, Rflx
+    if ((e = hipMalloc(&c->d_spt, 2 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc sflx", e);
+    else if ((e = hipMemsetAsync(c->d_spt, 0, 2 * bs * sizeof(double), c->stream)) != hipSuccess)
+      rc = fail(c, MRM_ERR_HIP, "memset sflx", e);
+  }
   if (!rc && (e = hipStreamSynchronize(c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "sync", e);
   if (rc) {
     fprintf(stderr, "mrm_create: %s\n", c->err.c_str());
@@ -305,7 +314,7 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   for (int q = 0; q < 3; q++) { hipFree(c->S[q]); hipFree(c->mxmem[q]); }
   for (int q = 0; q < 2; q++) hipFree(c->Pm[q]);
-  hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2);
+  hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2); hipFree(c->d_spt);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
   for (double* p : c->adj_ckpt) hipFree(p);
   hipFree(c->adj_mem);
@@ -610,7 +619,9 @@ static double eta_dt_limit(const mrm_ctx* c, const mrm_run_desc* run, double dt_
   return std::min(dt_cfl, dt_res);
 }
 
-static int launch_residual(mrm_ctx* c) {
+static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run);
+
+static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
   ResidParams RP;
   for (int d = 0; d < 3; d++) {
     RP.w1[d] = 8. / (12. * c->dx[d]);
@@ -632,6 +643,36 @@ static int launch_residual(mrm_ctx* c) {
   k_cmax_faces<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->inv_va2max, c->dU, P, c->d_red);
   }
   LAUNCHCHK(c, "k_cmax_faces");
+  if (c->d_spt) return launch_spitzer_resid(c, run);
+  return MRM_OK;
+}
+
+// Spitzer terms of MHD_Residual on the primitive state: constants from the
+// dt and CFL of the call (mhdres_SR.C:37-49, 136-145; Run.dt is still the
+// previous step's at stage 1)
+static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run) {
+  SptParams S;
+  for (int d = 0; d < 3; d++) {
+    S.w1[d] = 8. / (12. * c->dx[d]);
+    S.w2[d] = -1. / (12. * c->dx[d]);
+  }
+  const double CFL_hyp = 1.8, avr_max = 0.25, eps_hyp = 0.02, X_H = 0.7, Rgas = 8.314e7;
+  S.k0 = c->phys.params[3];
+  S.sat_flx = (1.0 + X_H) * pow(Rgas, 1.5) * sqrt(1836.) / 8.0;
+  const double rdt = run ? run->dt : 0.0, rcfl = run ? run->CFL : 0.0;
+  if (rdt > 0.0) {
+    S.cmax_hyp = std::min(rcfl, CFL_hyp) * c->dxmin / rdt;
+    S.nu_hyp_max = avr_max / rdt;
+    S.hyp_diff = eps_hyp / rdt;
+  } else {
+    S.cmax_hyp = 0.0;
+    S.nu_hyp_max = avr_max * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
+    S.hyp_diff = eps_hyp * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
+  }
+  const long bs = c->G.bs;
+  k_spitzer_resid<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, S, c->dU, c->dF + MRM_TEMP * bs,
+                                                                   c->dF + MRM_SFLX * bs, c->dR, c->d_spt + bs);
+  LAUNCHCHK(c, "k_spitzer_resid");
   return MRM_OK;
 }
 
@@ -646,7 +687,7 @@ extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_mhd_residual");
   if (run && run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
-  int rc = launch_residual(c);
+  int rc = launch_residual(c, run);
   if (rc) return rc;
   double cmax;
   rc = read_cmax(c, &cmax);
@@ -703,6 +744,12 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
   TScope ts(c, T_SRCINT);
   k_source_integrate<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q);
   LAUNCHCHK(c, "k_source_integrate");
+  if (c->d_spt) {
+    const long bs = c->G.bs;
+    k_spitzer_integrate<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP.wdt, c->dF + MRM_SFLX * bs,
+                                                                         c->d_spt, c->d_spt + bs);
+    LAUNCHCHK(c, "k_spitzer_integrate");
+  }
   return MRM_OK;
 }
 
@@ -713,6 +760,10 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
   TScope ts(c, T_SAVECONS);
   k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
   LAUNCHCHK(c, "k_save_cons");
+  if (c->d_spt) {
+    k_spitzer_save<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dF + MRM_SFLX * c->G.bs, c->d_spt);
+    LAUNCHCHK(c, "k_spitzer_save");
+  }
   return MRM_OK;
 }
 
@@ -750,11 +801,15 @@ static int exchange_mx(mrm_ctx* c, MxState M, float* P) {
   return MRM_OK;
 }
 
+static int exchange_arrays(mrm_ctx* c, double* base, int na);
+
 extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   c->fast_valid = false;
   if (c->precision == MRM_PREC_FP32_MIXED) return exchange_mx(c, c->M[c->cur], nullptr);
-  return exchange_buf(c, c->dU, nullptr);
+  int rc = exchange_buf(c, c->dU, nullptr);
+  if (rc || !c->d_spt) return rc;
+  return exchange_arrays(c, c->dF + MRM_SFLX * c->G.bs, 1);
 }
 
 // ------------------------------------------------------------------ Clean_divB
@@ -971,7 +1026,12 @@ extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, 
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
   if (c->precision == MRM_PREC_FP32_MIXED) return bc_mx(c, c->M[c->cur], nullptr, nullptr);
-  return bc_buf(c, c->dU, nullptr, nullptr);
+  int rc = bc_buf(c, c->dU, nullptr, nullptr);
+  if (rc || !c->d_spt) return rc;
+  const long nyz = (long)c->G.n1 * c->G.n2;
+  k_spitzer_bc<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.bnd[3] > 0.0, c->dF + MRM_SFLX * c->G.bs);
+  LAUNCHCHK(c, "k_spitzer_bc");
+  return MRM_OK;
 }
 
 // ------------------------------------------------------------------ TVD
@@ -1103,7 +1163,7 @@ static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
       STEPCHK(mrm_mhd_residual(c, run, &dt_mhd));
       STEPCHK(mrm_sync_timestep(c, run, dt_rad, dt_mhd, &cfl));
     } else {
-      STEPCHK(launch_residual(c));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
+      STEPCHK(launch_residual(c, run));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
     }
     STEPCHK(mrm_source_integrate_tcheck(c, run, stage));
     STEPCHK(mrm_exchange_grid(c));
@@ -1416,7 +1476,8 @@ extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
-  const bool fused = !c->force_sequence;
+  // Spitzer conduction runs through the per-operation kernels
+  const bool fused = !c->force_sequence && !c->d_spt;
   for (int it = 0; it < nsteps; it++) {
     int rc = mx ? step_mixed(c, run, dt_rad) : fused ? step_fused(c, run, dt_rad) : step_sequence(c, run, dt_rad);
     if (rc) return rc;
@@ -1590,10 +1651,11 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   NEED_FP64(c, "mrm_adjoint_forward");
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
-  if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0 || c->phys.params[4] > 0.0)
+  if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0 || c->phys.params[4] > 0.0 ||
+      c->d_spt)
     return fail(c, MRM_ERR_UNSUPPORTED,
                 "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0, eta = 0 and no "
-                "Clean_divB");
+                "Clean_divB or Spitzer conduction");
   for (int d = 0; d < 3; d++)
     if (c->lsize[d] < 4) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: interior extents below 4");
   for (double* p : c->adj_ckpt) hipFree(p);

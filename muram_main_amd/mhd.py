@@ -146,17 +146,23 @@ class MhdGrid:
     # ------------------------------------------------------------ restart files
     def RestoreSolution(self, dir_3d: str, resfile: str, backfile: str, run):
         """RestoreSolution (io_xysl.C:272-412): read backup.dat (globiter, time,
-        p_bc, s_bc), then result_prim_<v>.<globiter>; the state goes to the device."""
+        p_bc, s_bc), then result_prim_<v>.<globiter> (and v = 8, the heat
+        flux, with Spitzer conduction); the state goes to the device."""
         it, t, p_bc, s_bc = read_backup(backfile)
         run.globiter = it
         run.time = t
         self.set_bc(p_bc, s_bc)
         self.set_state(read_prim(dir_3d, resfile, it, self.grid, self.shape))
+        if self.phys.params[3] > 0.0:     # Spitzer's heat flux is variable 8 (io_xysl.C:389-392)
+            self.set_field(abi.SFLX, read_var(dir_3d, resfile, 8, it, self.grid, self.shape))
 
     def BackupSolution(self, dir_3d: str, resfile: str, backfile: str, run):
-        """BackupSolution (io_xysl.C:224-270): result_prim_<v>.<globiter>,
-        backup.dat and Header.<globiter> from the device state."""
+        """BackupSolution (io_xysl.C:224-270): result_prim_<v>.<globiter>
+        (v = 8 with Spitzer conduction), backup.dat and Header.<globiter> from
+        the device state."""
         write_prim(dir_3d, resfile, run.globiter, self.grid, self.get_state())
+        if self.phys.params[3] > 0.0:
+            write_var(dir_3d, resfile, 8, run.globiter, self.grid, self.get_field(abi.SFLX))
         p_bc, s_bc = self.bc()
         write_backup(backfile, run.globiter, run.time, p_bc, s_bc)
         inv_va2max = self.limits()[0]
@@ -198,6 +204,20 @@ def write_prim(dir_3d: str, resfile: str, it: int, grid: abi.GridDesc, U: np.nda
     U = np.ascontiguousarray(U, dtype=np.float64)
     _chk_io(lib.mrm_io_write_prim(dir_3d.encode(), resfile.encode(), it, C.byref(grid),
                                   U.ctypes.data_as(C.POINTER(C.c_double))), "write_prim")
+
+
+def read_var(dir_3d: str, resfile: str, v: int, it: int, grid: abi.GridDesc, shape) -> np.ndarray:
+    """result_prim_<v>.<it> -> field [n2, n1, n0] (ghosts zero)"""
+    f = np.empty(tuple(shape), dtype=np.float64)
+    _chk_io(abi.load().mrm_io_read_var(dir_3d.encode(), resfile.encode(), v, it, C.byref(grid),
+                                       f.ctypes.data_as(C.POINTER(C.c_double))), "read_var")
+    return f
+
+
+def write_var(dir_3d: str, resfile: str, v: int, it: int, grid: abi.GridDesc, f: np.ndarray):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    _chk_io(abi.load().mrm_io_write_var(dir_3d.encode(), resfile.encode(), v, it, C.byref(grid),
+                                        f.ctypes.data_as(C.POINTER(C.c_double))), "write_var")
 
 
 def write_header(dir_3d: str, it: int, grid: abi.GridDesc, time: float, dt: float, va_max: float):

@@ -68,6 +68,8 @@ struct orc_ctx {
   int divb_next_iter;
   /* GridData::curlB (grid.C:307-309), allocated with eta > 0 */
   double* curlB;
+  /* Spitzer conduction (grid.C:311-318): sflx lives in fld[MRM_SFLX] */
+  double *sflx0, *Rflx, *BgradT;
 };
 
 void orc_set_threads(int n) {
@@ -120,6 +122,11 @@ orc_ctx* orc_create(const mrm_grid_desc* g, const mrm_phys_desc* p) {
   c->bnd_ini = 1;
   c->tvd_ini = 1;
   c->divb_next_iter = 3;
+  if (p->params[3] > 0.0) {
+    c->sflx0 = (double*)calloc(bs, sizeof(double));
+    c->Rflx = (double*)calloc(bs, sizeof(double));
+    c->BgradT = (double*)calloc(bs, sizeof(double));
+  }
   return c;
 }
 
@@ -132,6 +139,7 @@ void orc_destroy(orc_ctx* c) {
   for (int t = 0; t < 2; t++) free(c->itab[t]);
   free(c->hfb); free(c->hft); free(c->qft);
   free(c->curlB);
+  free(c->sflx0); free(c->Rflx); free(c->BgradT);
   free(c);
 }
 
@@ -420,7 +428,10 @@ void orc_adjust_valf_max(orc_ctx* c, const mrm_run_desc* run, double stats[5]) {
 /* eta > 0 each sweep adds its part of J = curl B to curlB (:742-749), the   */
 /* last one adds eta |J|^2 to R.e (:440-451), every sweep adds eta times the */
 /* 4th-order second derivative of B along d1 (:452-462, :757-766), and dt is */
-/* limited by the diffusion time (:815-819).                                 */
+/* limited by the diffusion time (:815-819).  With Spitzer conduction the    */
+/* energy flux carries sflx B (:350-359), each sweep damps sflx with a 4th   */
+/* difference and adds its part of B.grad T (:680-690), and the last sweep   */
+/* relaxes sflx towards the saturation-limited Spitzer flux (:492-521).      */
 double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
   static const int loop_order[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
   static const double del[3][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
@@ -439,6 +450,25 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
     memset(c->curlB, 0, sizeof(double) * 3 * (size_t)bs);
   }
   double* curlB = c->curlB;
+  /* Spitzer constants (mhdres_SR.C:37-49, 136-145): from the dt of the call */
+  const double k0_spt = c->ph.params[3];
+  const int spitzer = k0_spt > 0.0;
+  const double CFL_hyp = 1.8, avr_max = 0.25, eps_hyp = 0.02, X_H = 0.7, Rgas = 8.314e7;
+  const double sat_flx = (1.0 + X_H) * pow(Rgas, 1.5) * sqrt(1836.) / 8.0;
+  const double rdt = run ? run->dt : 0.0, rcfl = run ? run->CFL : 0.0;
+  double cmax_hyp, nu_hyp_max, hyp_diff;
+  if (rdt > 0.0) {
+    cmax_hyp = smin(rcfl, CFL_hyp) * dxmin / rdt;
+    nu_hyp_max = avr_max / rdt;
+    hyp_diff = eps_hyp / rdt;
+  } else {
+    cmax_hyp = 0.0;
+    nu_hyp_max = avr_max * cmax_hyp / (smin(rcfl, CFL_hyp) * dxmin);
+    hyp_diff = eps_hyp * cmax_hyp / (smin(rcfl, CFL_hyp) * dxmin);
+  }
+  const double* sflxg = c->fld[MRM_SFLX];
+  const double* temp = c->fld[MRM_TEMP];
+  if (spitzer) memset(c->BgradT, 0, sizeof(double) * (size_t)bs);
   double cmax = 0.0;
   for (int d = 0; d < 3; d++) {
     const int d1 = loop_order[d][0], d2 = loop_order[d][1], d3 = loop_order[d][2];
@@ -457,13 +487,16 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
     double cmax_d = 0.0;
 #pragma omp parallel reduction(max:cmax_d)
     {
-      double* buf = (double*)malloc(sizeof(double) * (size_t)sz * (NV + NV + 4));
+      double* buf = (double*)malloc(sizeof(double) * (size_t)sz * (NV + NV + 7));
       double* var = buf;                 /* [NV][sz] */
       double* flx = var + NV * sz;       /* [NV][sz] */
       double* msx = flx + NV * sz;
       double* msy = msx + sz;
       double* msz = msy + sz;
       double* lf1 = msz + sz;
+      double* sfl = lf1 + sz;            /* sflx along the line */
+      double* vva = sfl + sz;            /* |v| */
+      double* bba = vva + sz;            /* |B|^2 */
       double lcmax = 0.0;
 #pragma omp for collapse(2) schedule(static)
       for (int k = kbeg; k <= kend; k++)
@@ -472,6 +505,7 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
           for (int i = 0; i < sz; i++) {
             long node = offset + i * str;
             for (int v = 0; v < NV; v++) var[v * sz + i] = U[v * bs + node];
+            sfl[i] = spitzer ? sflxg[node] : 0.0;
           }
           for (int i = 0; i < sz; i++) {
             long node = offset + i * str;
@@ -493,6 +527,8 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
             double pres = Pg[node];
             double cs2 = 5.0 / 3.0 * pres * dn;
             vv = sqrt(vv);
+            vva[i] = vv;
+            bba[i] = bb;
             lcmax = smax(lcmax, vv + 0.0 + sqrt(smax(cs2, lf1[i] * (cs2 + va2))));
             double mflx = vel * var0;
             flx[0 * sz + i] = mflx;
@@ -506,6 +542,7 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
             msx[i] = mag * var5 - p1 * pmag;
             msy[i] = mag * var6 - p2 * pmag;
             msz[i] = mag * var7 - p3 * pmag;
+            if (spitzer) flx[4 * sz + i] += sfl[i] * var[(5 + d1) * sz + i];
           }
           for (int i = gh; i < sz - gh; i++) {
             long node = offset + i * str;
@@ -514,6 +551,12 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
               const double* F = flx + v * sz;
               res[v] = 0.0;
               res[v] -= w1 * (F[i + 1] - F[i - 1]) + w2 * (F[i + 2] - F[i - 2]);
+            }
+            if (spitzer && d < 2) {
+              const double* Bd = var + (5 + d1) * sz;
+              c->Rflx[node] -= hyp_diff * ((sfl[i + 2] + sfl[i - 2]) - 4. * (sfl[i + 1] + sfl[i - 1]) + 6. * sfl[i]);
+              c->BgradT[node] += (Bd[i] * (w1 * (temp[node + str] - temp[node - str]) +
+                                           w2 * (temp[node + 2 * str] - temp[node - 2 * str])));
             }
             const double* B2 = var + (5 + d2) * sz;
             const double* B3 = var + (5 + d3) * sz;
@@ -551,6 +594,24 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
               }
             }
             for (int v = 0; v < NV; v++) R[v * bs + node] += res[v];
+            if (spitzer && d == 2) {
+              const double* Bd = var + (5 + d1) * sz;
+              const double tempi = temp[node];
+              c->BgradT[node] += (Bd[i] * (w1 * (temp[node + str] - temp[node - str]) +
+                                           w2 * (temp[node + 2 * str] - temp[node - 2 * str])));
+              c->Rflx[node] -= hyp_diff * ((sfl[i + 2] + sfl[i - 2]) - 4. * (sfl[i + 1] + sfl[i - 1]) + 6. * sfl[i]);
+              const double c_lim = cmax_hyp - vva[i];
+              const double tt32 = tempi * sqrt(tempi);
+              double kap_spt = k0_spt * tempi * tt32;
+              const double F_sat = sat_flx * var[i] * tt32;
+              const double invBB = 1.0 / smax(1e-100, bba[i]);
+              double F_spt = -kap_spt * c->BgradT[node];
+              const double f_lim = F_sat / (F_sat + fabs(F_spt * sqrt(invBB)));
+              F_spt *= invBB * f_lim;
+              kap_spt *= f_lim;
+              const double nu_hyp = smin(c_lim * c_lim * var[4 * sz + i] / (tempi * kap_spt), nu_hyp_max);
+              c->Rflx[node] += nu_hyp * (F_spt - sfl[i]);
+            }
           }
         }
       cmax_d = smax(cmax_d, lcmax);
@@ -670,6 +731,14 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
       }
       for (int i = c->lbeg[0]; i <= c->lend[0]; i++)
         tcheck_cell(U, bs, off0 + i, rho_min, eps_min, eps_max, vmax);
+      if (c->Rflx) {   /* hyperbolic conduction (Add_Sources_Integrate_SR.C:410-419) */
+        double* sflx = c->fld[MRM_SFLX];
+        for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+          const long node = off0 + i;
+          sflx[node] = c->sflx0[node] + wdt * c->Rflx[node];
+          c->Rflx[node] = 0;
+        }
+      }
     }
 }
 
@@ -683,6 +752,8 @@ void orc_save_cons(orc_ctx* c) {
       for (int v = 0; v < NV; v++)
         for (int i = c->lbeg[0]; i <= c->lend[0]; i++)
           c->U0[v * bs + off0 + i] = c->U[v * bs + off0 + i];
+      if (c->sflx0)   /* Add_Sources_Integrate_SR.C:504-517 */
+        for (int i = c->lbeg[0]; i <= c->lend[0]; i++) c->sflx0[off0 + i] = c->fld[MRM_SFLX][off0 + i];
     }
 }
 
@@ -1009,6 +1080,11 @@ void orc_exchange_grid(orc_ctx* c) {
             c->U[v * bs + dst_r] = c->U[v * bs + src_r];
             c->U[v * bs + dst_l] = c->U[v * bs + src_l];
           }
+          if (c->Rflx) {   /* sflx travels with U (exchange.C:111-120) */
+            double* sf = c->fld[MRM_SFLX];
+            sf[dst_r] = sf[src_r];
+            sf[dst_l] = sf[src_l];
+          }
         }
     gs[d1][1] += gh;
     gs[d1][0] -= gh;
@@ -1245,6 +1321,7 @@ void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage)
       u2[4] = eps2 * rho2 + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
       long g1 = c->lbeg[0] - 1 + base, g2 = c->lbeg[0] - 2 + base;
       for (int v = 0; v < NV; v++) { U[v * bs + g1] = u1[v]; U[v * bs + g2] = u2[v]; }
+      if (c->Rflx) { c->fld[MRM_SFLX][g1] = 0.0; c->fld[MRM_SFLX][g2] = 0.0; }   /* :306-310 */
     }
   /* top ghost layers (:402-474), vertical field (bnd_pot = 0) */
 #pragma omp parallel for collapse(2) schedule(static)
@@ -1266,6 +1343,18 @@ void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage)
       u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
       long g1 = c->lend[0] + 1 + base, g2 = c->lend[0] + 2 + base;
       for (int v = 0; v < NV; v++) { U[v * bs + g1] = u1[v]; U[v * bs + g2] = u2[v]; }
+      if (c->Rflx) {   /* heat flux (:455-466): symmetric under a hot plate, else zero */
+        double* sf = c->fld[MRM_SFLX];
+        if (eps_top > 0.0) {
+          sf[g1] = sf[n1];
+          sf[g2] = sf[n2];
+        } else {
+          sf[n2] = 0.0;
+          sf[n1] = 0.0;
+          sf[g1] = 0.0;
+          sf[g2] = 0.0;
+        }
+      }
     }
 }
 

@@ -167,6 +167,10 @@ int main(int argc, char** argv) {
   memset(Grid.Tau, 0, sizeof(double) * bs);
   for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb, Grid.divB, Grid.phi})
     memset(a, 0, sizeof(double) * bs);
+  // Spitzer conduction's arrays exist when param_spitzer > 0 (grid.C:311-318)
+  for (double* a : {Grid.sflx, Grid.sflx0, Grid.Rflx, Grid.BgradT})
+    if (a) memset(a, 0, sizeof(double) * bs);
+  std::vector<double> zeros(bs, 0.0);
 
   std::vector<double> soa(8L * bs);
   rd(soa.data(), 8, 8L * bs, f);
@@ -217,6 +221,7 @@ int main(int argc, char** argv) {
       fwrite(soa.data(), 8, 8L * bs, out);
       for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb, Grid.divB, Grid.phi})
         fwrite(a, 8, bs, out);
+      fwrite(Grid.sflx ? Grid.sflx : zeros.data(), 8, bs, out);
     }
   }
   fclose(out);

@@ -4,10 +4,10 @@
     python tests/golden/make_test3d_crop.py
 
 Contents (data only):
-  prim          float32 [8, NZ, NY, 192]: result_prim_0..7.000000 of
-                TEST/Test_3D/ini (rho, v0, v1, v2, e_int, B0, B1, B2; the
-                192-cell vertical direction fastest), cropped to the first
-                NY x NZ horizontal cells
+  prim          float32 [9, NZ, NY, 192]: result_prim_0..8.000000 of
+                TEST/Test_3D/ini (rho, v0, v1, v2, e_int, B0, B1, B2 and
+                Spitzer's heat flux sflx; the 192-cell vertical direction
+                fastest), cropped to the first NY x NZ horizontal cells
   gsize, dx     the full Test_3D grid (ini/Header.000000)
   header_ini    text of ini/Header.000000
   backup_ini    text of ini/backup.dat
@@ -39,8 +39,8 @@ def main():
     gsize = np.array([int(v) for v in hdr[:3]], dtype=np.int64)
     dx = np.array([float(v) for v in hdr[3:6]], dtype=np.float64)
     n0, n1, n2 = (int(v) for v in gsize)
-    prim = np.empty((8, NZ, NY, n0), dtype=np.float32)
-    for v in range(8):
+    prim = np.empty((9, NZ, NY, n0), dtype=np.float32)
+    for v in range(9):
         a = np.fromfile(os.path.join(SRC, "ini", f"result_prim_{v}.000000"), dtype=np.float32)
         prim[v] = a.reshape(n2, n1, n0)[:NZ, :NY, :]
 

@@ -173,9 +173,9 @@ def run_reference(grid: GridDesc, phys: PhysDesc, run: RunDesc, table: bytes, U:
         off += 64 * bs
         Fo = np.frombuffer(raw, np.float64, 5 * bs, off).reshape((5,) + shape[1:]).copy()
         off += 40 * bs
-        Do = np.frombuffer(raw, np.float64, 2 * bs, off).reshape((2,) + shape[1:]).copy()
-        off += 16 * bs
-        dumps.append(dict(op=o, sc=sc, U=Uo, Res=Ro, fields=Fo, divB=Do[0], phi=Do[1]))
+        Do = np.frombuffer(raw, np.float64, 3 * bs, off).reshape((3,) + shape[1:]).copy()
+        off += 24 * bs
+        dumps.append(dict(op=o, sc=sc, U=Uo, Res=Ro, fields=Fo, divB=Do[0], phi=Do[1], sflx=Do[2]))
     return dumps
 
 

@@ -31,18 +31,18 @@ def crop():
 def restart_dir(tmp_path, crop):
     """the crop as the reference lays out a restart: result_prim_<v>, backup.dat"""
     d = str(tmp_path) + "/"
-    for v in range(8):
+    for v in range(9):
         crop["prim"][v].tofile(d + f"result_prim_{v}.000000")
     open(d + "backup.dat", "w").write(str(crop["backup_ini"]))
     n2, n1, n0 = crop["prim"].shape[1:]
     return d, config.grid_desc((n0, n1, n2), tuple(crop["dx"]))
 
 
-def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused", divb=True):
+def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused", divb=True, spitzer=True):
     saved = os.environ.get("MRM_STEP_PATH")
     os.environ["MRM_STEP_PATH"] = path
     try:
-        m = MhdGrid(g, config.test3d_phys(va_adjust, divb), precision)
+        m = MhdGrid(g, config.test3d_phys(va_adjust, divb, spitzer), precision)
     finally:
         if saved is None:
             os.environ.pop("MRM_STEP_PATH", None)
@@ -56,18 +56,23 @@ def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused", 
     return m, r
 
 
-@pytest.mark.parametrize("va_adjust,divb,path", [(1.0, True, "fused"), (1.0, True, "sequence"),
-                                                (1.0, False, "fused"), (0.0, False, "fused")])
-def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, divb, path):
-    # va_adjust 1 with divB cleaning is the case's own setting
+@pytest.mark.parametrize("va_adjust,divb,spitzer,path", [(1.0, True, True, "fused"), (1.0, True, False, "fused"),
+                                                        (1.0, True, False, "sequence"), (1.0, False, False, "fused"),
+                                                        (0.0, False, False, "fused")])
+def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, divb, spitzer, path):
+    # va_adjust 1 with divB cleaning and Spitzer conduction is the case's own
+    # setting (conduction runs the per-operation kernels)
     print(f"case: Test_3D crop 192x10x10, restore + 4 FP64 {path} steps vs oracle, va_adjust {va_adjust},"
-          f" divB {divb}", flush=True)
+          f" divB {divb}, Spitzer {spitzer}", flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m, r = restored(d, g, table, va_adjust=va_adjust, path=path, divb=divb)
+    m, r = restored(d, g, table, va_adjust=va_adjust, path=path, divb=divb, spitzer=spitzer)
     assert (r.globiter, r.time) == (0, 0.0)
-    o = Oracle(g, config.test3d_phys(va_adjust, divb), table)
+    o = Oracle(g, config.test3d_phys(va_adjust, divb, spitzer), table)
     ro = config.run_desc()
     o.set_state(mhd.read_prim(d, "result", 0, g, m.shape))
+    if spitzer:
+        o.field(abi.SFLX)[...] = mhd.read_var(d, "result", 8, 0, g, m.shape)
+        assert np.array_equal(m.get_field(abi.SFLX), o.field(abi.SFLX))
     o.set_bc(-1.0, -1.0)
     o.exchange_grid()
     o.set_boundary_conditions(ro, 0)
@@ -75,6 +80,8 @@ def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, di
     o.step(ro, config.DT_RAD_OFF, 4)
     m.step(r, config.DT_RAD_OFF, 4)
     errs = [relerr(m.get_state()[v], o.state(0)[v]) for v in range(8)]
+    if spitzer:
+        errs.append(relerr(m.get_field(abi.SFLX), o.field(abi.SFLX)))
     print("per-field rel err:", ["%.2e" % e for e in errs], "dt", r.dt, ro.dt)
     assert max(errs) <= TOL
     assert r.dt == pytest.approx(ro.dt, rel=TOL)
@@ -110,11 +117,11 @@ def test_backup_restore_round_trip(tmp_path, crop, table):
 
 @pytest.mark.parametrize("va_adjust", [0.0, 1.0])
 def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust):
-    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}, no divB cleaning",
-          flush=True)
+    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}, no divB cleaning"
+          " or conduction", flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=False)
-    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=False)
+    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=False, spitzer=False)
+    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=False, spitzer=False)
     m64.step(r64, config.DT_RAD_OFF, 20)
     mx.step(rx, config.DT_RAD_OFF, 20)
     errs = field_errors(mx.get_state(), m64.get_state(), interior=(slice(2, -2),) * 3)

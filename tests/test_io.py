@@ -124,3 +124,16 @@ def test_read_full_reference_initial_state():
     prim = np.stack([np.fromfile(f"{REF_INI}/result_prim_{v}.000000", dtype=np.float32).reshape(64, 64, 192)
                      for v in range(8)])
     np.testing.assert_array_equal(U, cons_of_prim(prim, shape_of(g)))
+
+
+def test_heat_flux_file_round_trip(tmp_path, crop):
+    """result_prim_8 (Spitzer's heat flux) through mrm_io_read_var /
+    mrm_io_write_var: the interior in the reference's order, ghosts zero"""
+    g = crop_grid(crop)
+    d = str(tmp_path) + "/"
+    crop["prim"][8].tofile(d + "result_prim_8.000000")
+    f = mhd.read_var(d, "result", 8, 0, g, shape_of(g))
+    assert np.array_equal(f[2:-2, 2:-2, 2:-2], crop["prim"][8].astype(np.float64))
+    assert not f[:2].any() and not f[:, :, :2].any()
+    mhd.write_var(d, "result", 8, 3, g, f)
+    assert open(d + "result_prim_8.000003", "rb").read() == open(d + "result_prim_8.000000", "rb").read()

@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from muram_main_amd import config, eos_table, initial
+from muram_main_amd import abi, config, eos_table, initial
 from orc import Oracle, ref_available, run_reference, relerr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -455,3 +455,60 @@ def test_resistive_residual_on_a_sine_field(table):
     r2 = config.run_desc(CFL=1.2)
     dt2 = o.mhd_residual(r2)
     assert dt2 <= 0.5 * fac / eta * (1 + 1e-15)
+
+
+# ---------------------------------------------------------------- Spitzer conduction
+@pytest.fixture(scope="module")
+def gold_spt():
+    return np.load(os.path.join(GOLD, "ref_spitzer.npz"))
+
+
+def spitzer_cases():
+    import sys
+    sys.path.insert(0, GOLD)
+    import make_golden_spitzer
+    return make_golden_spitzer.cases()
+
+
+@pytest.mark.parametrize("case", ["spt_cool", "spt_corona"])
+def test_golden_spitzer_steps(gold_spt, table, case):
+    box, phys = spitzer_cases()[case]
+    o = Oracle(config.grid_desc(box.gsize, box.dx), phys, table)
+    r = config.run_desc(CFL=0.8)
+    o.set_state(gold_spt[f"{case}_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.step(r, 1e30, 1)
+    assert np.array_equal(o.state(0), gold_spt[f"{case}_step1_U"])
+    assert np.array_equal(o.field(config.SFLX_FIELD), gold_spt[f"{case}_step1_sflx"])
+    o.step(r, 1e30, 4)
+    assert np.array_equal(o.state(0), gold_spt[f"{case}_step5_U"])
+    assert np.array_equal(o.field(config.SFLX_FIELD), gold_spt[f"{case}_step5_sflx"])
+    assert r.dt == gold_spt[f"{case}_step5_sc"][0]
+
+
+def test_spitzer_flux_runs_down_the_temperature_gradient(gold_spt):
+    """In the corona case the relaxed heat flux along B opposes B.grad T
+    (heat flows from hot to cold) wherever the gradient along the field is
+    strong; a sign error in the Spitzer flux would reverse it."""
+    box, phys = spitzer_cases()["spt_corona"]
+    table = open(os.path.join(GOLD, "eos_small.dat"), "rb").read()
+    o = Oracle(config.grid_desc(box.gsize, box.dx), phys, table)
+    r = config.run_desc(CFL=0.8)
+    o.set_state(gold_spt["spt_corona_in"])
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.step(r, 1e30, 5)
+    o.cons_to_prim()
+    T = o.field(abi.TEMP)
+    U = o.state(0)
+    # B.grad T with the 4th-order stencil along the vertical (the dominant part here)
+    dz = box.dx[0]
+    gT = (8 * (T[:, :, 3:-1] - T[:, :, 1:-3]) - (T[:, :, 4:] - T[:, :, :-4])) / (12 * dz)
+    bgt = U[5][:, :, 2:-2] * gT
+    s = o.field(config.SFLX_FIELD)[:, :, 2:-2]
+    inner = (slice(2, -2), slice(2, -2))
+    bgt, s = bgt[inner], s[inner]
+    strong = np.abs(bgt) > 0.1 * np.abs(bgt).max()
+    assert strong.sum() > 20
+    assert np.mean(np.sign(s[strong]) == -np.sign(bgt[strong])) > 0.95
