@@ -72,7 +72,9 @@ def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, di
     o.set_state(mhd.read_prim(d, "result", 0, g, m.shape))
     if spitzer:
         o.field(abi.SFLX)[...] = mhd.read_var(d, "result", 8, 0, g, m.shape)
-        assert np.array_equal(m.get_field(abi.SFLX), o.field(abi.SFLX))
+        # the GPU's ghosts are already filled; the interior came from the file
+        inner = (slice(2, -2),) * 3
+        assert np.array_equal(m.get_field(abi.SFLX)[inner], o.field(abi.SFLX)[inner])
     o.set_bc(-1.0, -1.0)
     o.exchange_grid()
     o.set_boundary_conditions(ro, 0)
