@@ -1,6 +1,7 @@
 """Hand-written adjoint of the FP64 step (mrm_adjoint_forward/backward)
 against central finite differences of the forward, through a
-torch.autograd.Function wrapper.
+torch.autograd.Function wrapper, and against the autograd tape of the FP64
+PyTorch restatement of the same steps (tests/tape_step.py).
 
 Objective J = sum of v_vertical^2 = (M0/rho)^2 over one vertical layer of the
 interior after N steps.  Inputs: the interior of the initial state (rho, M, e,
@@ -159,3 +160,38 @@ def test_adjoint_checkpoint_spacing_gives_the_same_gradient():
     # step to step, the recomputation evaluates it with the exact build: the
     # states agree to rounding, the gradients to far below the FD tolerance
     np.testing.assert_allclose(g1, g3, rtol=1e-9, atol=1e-9 * np.abs(g1).max())
+
+
+def tape_gradient(prob):
+    """dJ/dU0 from the autograd tape of the PyTorch FP64 restatement of the
+    same steps (tests/tape_step.py, matched to the oracle's forward)"""
+    from tape_step import TapeStep
+    tape = TapeStep(prob.g, prob.phys, prob.table, prob.p_bc, prob.s_bc)
+    U0 = torch.tensor(prob.U, dtype=torch.float64, requires_grad=True)
+    X = tape.fill(U0)
+    for _ in range(prob.nsteps):
+        X = tape.step(X, prob.dt)
+    L = prob.layer
+    v = X[1, 2:-2, 2:-2, L] / X[0, 2:-2, 2:-2, L]
+    Jt = (v * v).sum()
+    Jt.backward()
+    return U0.grad.numpy(), float(Jt)
+
+
+@pytest.mark.parametrize("gsize,seed,nsteps,every", [((16, 12, 12), 51, 2, 1), ((12, 10, 14), 55, 3, 2)])
+def test_adjoint_matches_fp64_tape_gradient(gsize, seed, nsteps, every):
+    print(f"case: adjoint vs FP64 tape gradient {gsize}, {nsteps} steps", flush=True)
+    prob = Problem(gsize, seed=seed, nsteps=nsteps, every=every)
+    grad, J = gradient(prob)
+    gt, Jt = tape_gradient(prob)
+    assert J == pytest.approx(Jt, rel=1e-12)
+    errs = []
+    for q in range(8):
+        a, b = grad[q][prob.inner], gt[q][prob.inner]
+        den = np.abs(b).max()
+        errs.append(float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a).max()))
+    print("per-field max|adjoint - tape| / max|tape|:", ["%.1e" % e for e in errs])
+    # the same derivative taken two ways: agreement to rounding
+    assert max(errs) <= 1e-10
+    # the initial ghosts are overwritten by exchange + BC: no sensitivity there
+    assert not np.any(gt[:, 0:2]) and not np.any(gt[:, :, :, 0:2])
