@@ -53,6 +53,12 @@ extern "C" {
 #define MRM_PHI  8
 #define MRM_SFLX 9   /* Spitzer heat flux along B (grid.H:71, result_prim_8) */
 #define MRM_NFIELD 10
+/* diagnostics arrays (grid.H:69-70, 93-100), read and written through
+ * mrm_get_field / mrm_set_field: tvar1..tvar8, Qres, Qvis */
+#define MRM_TVAR1 10
+#define MRM_TVAR8 17
+#define MRM_QRES 18
+#define MRM_QVIS 19
 
 /* host array layouts for mrm_set_state / mrm_get_state */
 #define MRM_LAYOUT_AOS 0   /* cState[bufsize]: 8 doubles per cell (mhd3d.H:24-27) */
@@ -95,7 +101,7 @@ typedef struct mrm_run_desc {
   double CFL_tvd;
   double outcad;
   int    autostep;
-  int    need_diagnostics;   /* must be 0: diagnostics are outside the path */
+  int    need_diagnostics;   /* the operations add to the diagnostics arrays */
   int    globiter;
   int    iteration;
 } mrm_run_desc;
@@ -122,7 +128,9 @@ int mrm_eos_load(mrm_ctx* ctx, const void* table_bytes, size_t nbytes);
 /* ACCH::UpdateGPU / UpdateCPU counterparts for U, U0, Res (ACCH.h:49-50). */
 int mrm_set_state(mrm_ctx* ctx, int which, const double* host, int layout);
 int mrm_get_state(mrm_ctx* ctx, int which, double* host, int layout);
-/* ... and for pres, temp, ne, rhoi, amb, Qtot, Tau (one double per cell). */
+/* ... and for pres, temp, ne, rhoi, amb, Qtot, Tau, divB, phi, sflx and the
+ * diagnostics arrays (one double per cell; diagnostics never collected read
+ * as zero). */
 int mrm_set_field(mrm_ctx* ctx, int field, const double* host);
 int mrm_get_field(mrm_ctx* ctx, int field, double* host);
 
@@ -133,6 +141,17 @@ int mrm_get_limits(const mrm_ctx* ctx, double lim[4]);
 int mrm_set_limits(mrm_ctx* ctx, const double lim[4]);
 int mrm_get_bc(const mrm_ctx* ctx, double* p_bc, double* s_bc);
 int mrm_set_bc(mrm_ctx* ctx, double p_bc, double s_bc);
+
+/* Diagnostics (FP64 contexts): every operation called with
+ * run->need_diagnostics adds its terms to tvar1..8, Qres, Qvis
+ * (mhdres_SR.C:368-383, 434-435, 688-702; Add_Sources_Integrate_SR.C:122-296,
+ * 618-620; tvdlimit_SR.C:577-666; clean_divB.C:264-270).  mrm_diag_zero is the
+ * zeroing loop solver.C:271-296 runs where the flag turns on (stage 4 of an
+ * output step); the same zeroing happens by itself on the first call that
+ * takes a run descriptor with the flag set after one with it clear, so a
+ * caller that sets the flag as the solver does needs no explicit call.  The
+ * arrays are allocated by their first use. */
+int mrm_diag_zero(mrm_ctx* ctx);
 
 /* ConsToPrim (eos.H:39, cons_to_prim.C:39-236): U -> primitive in place over
  * the whole buffer, pres/temp/ne/rhoi/amb filled. */
@@ -179,7 +198,10 @@ int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage
  * phys.divB[0] > 0, TCheck, exchange, BC, Run.Advance.  dt_rad is the
  * radiative time step the caller supplies (RT is outside the path).  With
  * resistivity the stage kernels add the resistive terms; with Spitzer
- * conduction the step runs the per-operation kernels. */
+ * conduction the step runs the per-operation kernels.  With
+ * run->need_diagnostics the last step collects the diagnostics from its
+ * stage 4 on, zeroed first (solver.C:264-296), through the per-operation
+ * kernels; run->need_diagnostics is left as it was. */
 int mrm_step(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, int nsteps);
 
 /* Restart files of one rank holding the whole grid, host side (no context):

@@ -18,13 +18,16 @@ ERRORS = {-1: "MRM_ERR_ARG", -2: "MRM_ERR_HIP", -3: "MRM_ERR_EOS", -4: "MRM_ERR_
 PREC_FP64, PREC_FP32_MIXED = 0, 1
 U, U0, RES = 0, 1, 2
 PRES, TEMP, NE, RHOI, AMB, QTOT, TAU, DIVB, PHI, SFLX = range(10)
+# diagnostics arrays tvar1..tvar8, Qres, Qvis
+TVAR1, QRES, QVIS = 10, 18, 19
+DIAG_NAMES = ("tvar1", "tvar2", "tvar3", "tvar4", "tvar5", "tvar6", "tvar7", "tvar8", "Qres", "Qvis")
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
 
 # every symbol include/muram_step.h declares
 EXPORTS = [
     "mrm_create", "mrm_destroy", "mrm_last_error", "mrm_bufsize", "mrm_synchronize",
     "mrm_eos_load", "mrm_set_state", "mrm_get_state", "mrm_set_field", "mrm_get_field",
-    "mrm_get_limits", "mrm_set_limits", "mrm_get_bc", "mrm_set_bc",
+    "mrm_get_limits", "mrm_set_limits", "mrm_get_bc", "mrm_set_bc", "mrm_diag_zero",
     "mrm_cons_to_prim", "mrm_adjust_valf_max", "mrm_mhd_residual", "mrm_sync_timestep",
     "mrm_source_integrate_tcheck", "mrm_save_cons", "mrm_tcheck", "mrm_tvdlimit", "mrm_clean_divB",
     "mrm_exchange_grid", "mrm_set_boundary_conditions", "mrm_step",
@@ -95,6 +98,7 @@ def load(path: str = LIB_PATH):
         "mrm_sync_timestep": ([P, C.POINTER(RunDesc), C.c_double, C.c_double, dp], C.c_int),
         "mrm_source_integrate_tcheck": ([P, C.POINTER(RunDesc), C.c_int], C.c_int),
         "mrm_save_cons": ([P], C.c_int),
+        "mrm_diag_zero": ([P], C.c_int),
         "mrm_tcheck": ([P, C.POINTER(RunDesc)], C.c_int),
         "mrm_tvdlimit": ([P, C.POINTER(RunDesc), C.c_double], C.c_int),
         "mrm_clean_divB": ([P, C.POINTER(RunDesc), dp], C.c_int),

@@ -265,9 +265,12 @@ MRM_HD double point_cmax_cons(const double u[MRM_NV], double pres, double inv_va
 
 // Residual contribution of sweep D at the centre point given the point data of
 // the five points c-2..c+2 along D (mhdres_SR.C:361-412).  res must hold 0.
-template <int D>
+// DIAG: dg[0] += the energy flux divergence (tvar1, :368-383) and dg[1] +=
+// v . Lorentz force (tvar3, :434-435)
+template <int D, bool DIAG = false>
 MRM_HD void sweep_residual(const double v[5][MRM_NV], const double p[5], double w1,
-                                               double w2, double inv_va2max, double res[MRM_NV]) {
+                                               double w2, double inv_va2max, double res[MRM_NV],
+                                               double* dg = nullptr) {
   constexpr int D2 = D == 0 ? 1 : 0;
   constexpr int D3 = D == 2 ? 1 : 2;
   PtFlux fm2, fm1, fp1, fp2;
@@ -280,6 +283,7 @@ MRM_HD void sweep_residual(const double v[5][MRM_NV], const double p[5], double 
     res[q] = 0.0;
     res[q] -= w1 * (fp1.f[q] - fm1.f[q]) + w2 * (fp2.f[q] - fm2.f[q]);
   }
+  if (DIAG) dg[0] += res[4];
   double dB2 = w1 * (v[3][5 + D2] - v[1][5 + D2]) + w2 * (v[4][5 + D2] - v[0][5 + D2]);
   double dB3 = w1 * (v[3][5 + D3] - v[1][5 + D3]) + w2 * (v[4][5 + D3] - v[0][5 + D3]);
   double cb[3];
@@ -296,7 +300,9 @@ MRM_HD void sweep_residual(const double v[5][MRM_NV], const double p[5], double 
   res[1] += lrt0;
   res[2] += lrt1;
   res[3] += lrt2;
-  res[4] += v[2][1] * lrt0 + v[2][2] * lrt1 + v[2][3] * lrt2;
+  const double vl = v[2][1] * lrt0 + v[2][2] * lrt1 + v[2][3] * lrt2;
+  res[4] += vl;
+  if (DIAG) dg[1] += vl;
 }
 
 // Resistive part of one sweep along D (mhdres_SR.C:422-462, 742-766): the
@@ -414,9 +420,12 @@ MRM_HD void tcheck_vals(double u[MRM_NV], double rho_min, double eps_min, double
 // Sources, Boris correction, RK update and Tcheck of one cell
 // (Add_Sources_Integrate_SR.C:331-465).  up: primitive U at the cell; R: its
 // residual (modified); u0: U0.  Returns the new conservative state in out.
+// DIAG: dg[0..3] = efaci, e_dmpi, boris3 and the energy before the clamp
+// (Add_Sources_Integrate_SR.C:168-170, 218, 292)
+template <bool DIAG = false>
 MRM_HD void source_integrate_cell(const SrcParams& P, const double up[MRM_NV],
                                                       double R[MRM_NV], const double u0[MRM_NV], double Qrad,
-                                                      double out[MRM_NV]) {
+                                                      double out[MRM_NV], double* dg = nullptr) {
   double dn = P.invRdt * up[4];
   double efaci = dn / fmax(dn, fabs(Qrad));
   dn = P.g * up[0] - 0.0;
@@ -446,6 +455,12 @@ MRM_HD void source_integrate_cell(const SrcParams& P, const double up[MRM_NV],
   R[4] += boris3;
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) out[q] = u0[q] + P.wdt * R[q];
+  if (DIAG) {
+    dg[0] = efaci;
+    dg[1] = e_dmpi;
+    dg[2] = boris3;
+    dg[3] = out[4];
+  }
   tcheck_vals(out, P.rho_min, P.eps_min, P.eps_max, P.vmax);
 }
 
@@ -547,12 +562,13 @@ MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const 
 
 // diffusive flux at interface i (between points i = b and i+1 = c), with
 // a = i-1 and d = i+2 (tvdlimit_SR.C:436-563); returns the resistive-heating
-// interface term (:609-614)
-template <int D>
+// interface term (:609-614); DIAG: *qv = the viscous-heating interface term
+// (:627-634)
+template <int D, bool DIAG = false>
 MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
                                             const TvdPt& d, const double Sb[MRM_NV], const double Sc[MRM_NV],
                                             double hfb1, double hft1, double qft1, bool hyper_on,
-                                            bool zero_bflux, double F[MRM_NV]) {
+                                            bool zero_bflux, double F[MRM_NV], double* qv = nullptr) {
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
     double sl = c.v[q] - b.v[q];
@@ -600,18 +616,23 @@ MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, cons
   flx4 += (uif4 * min_cz) + qft1 * (uif1 * flx1 + uif2 * flx2 + uif3 * flx3);
   F[0] = flx0; F[1] = flx1; F[2] = flx2; F[3] = flx3; F[4] = flx4;
   if (zero_bflux) F[5] = 0.0;
+  if (DIAG) *qv = (c.v[1] - b.v[1]) * F[1] + (c.v[2] - b.v[2]) * F[2] + (c.v[3] - b.v[3]) * F[3];
   return (c.v[5] - b.v[5]) * F[5] + (c.v[6] - b.v[6]) * F[6] + (c.v[7] - b.v[7]) * F[7];
 }
 
-// in-place update of point i from the fluxes at i-1 and i (tvdlimit_SR.C:565-657)
+// in-place update of point i from the fluxes at i-1 and i (tvdlimit_SR.C:565-657);
+// DIAG: dg[0..3] = the energy diffusion (tvar8), boris3 (tvar6), qres and the
+// energy before its floor (tvar7), :577-666
+template <bool DIAG = false>
 MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_NV],
                                            const double F[MRM_NV], double qm, double q, double qft1,
-                                           double u[MRM_NV]) {
+                                           double u[MRM_NV], double* dg = nullptr) {
   double r[MRM_NV];
 #pragma unroll
   for (int k = 0; k < MRM_NV; k++) r[k] = P.dt_tvd * (F[k] - Fm[k]);
   double v1 = b.v[1], v2 = b.v[2], v3 = b.v[3], b5 = b.v[5], b6 = b.v[6], b7 = b.v[7];
   r[4] += (1.0 - qft1) * (r[1] * v1 + r[2] * v2 + r[3] * v3);
+  if (DIAG) dg[0] = r[4];
   double dn = (r[1] * b5 + r[2] * b6 + r[3] * b7) / smax(1e-100, b.bsqr);
   double boris0 = -b.BC2 * (r[1] - dn * b5);
   double boris1 = -b.BC2 * (r[2] - dn * b6);
@@ -623,6 +644,11 @@ MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_N
 #pragma unroll
   for (int k = 0; k < MRM_NV; k++) u[k] += r[k];
   u[0] = smax(P.rho_min, u[0]);
+  if (DIAG) {
+    dg[1] = boris3;
+    dg[2] = qres;
+    dg[3] = u[4];
+  }
   u[4] = smax(u[4], P.eps_min * u[0] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]);
 }
 

@@ -14,6 +14,8 @@ struct DivbArgs {
   Geom G;
   double w1[3], w2[3];     // 8/(12 dx), -1/(12 dx) per dimension (grid.C:339-340)
   double fac, damp;        // divB_fac, divB_damp (clean_divB.C:47-48, 110)
+  double* qres = nullptr;  // diagnostics: Qres += phi divB / dt (clean_divB.C:264-270)
+  double invdt = 0.0;
 };
 
 // Tiling of the interior for the two cell kernels: a 64 x 4 (i, j) tile per
@@ -150,6 +152,7 @@ __global__ void __launch_bounds__(DIVB_TX * DIVB_TY) k_divb_update(DivbArgs A, d
       U[6 * bs + n] += (A.w1[1] * (phi[n + s1] - phi[n - s1]) + A.w2[1] * (phi[n + 2 * s1] - phi[n - 2 * s1]));
       U[7 * bs + n] += (A.w1[2] * (pw[3] - pw[1]) + A.w2[2] * (pw[4] - pw[0]));
       U[4 * bs + n] += pw[2] * divB[n];
+      if (A.qres) A.qres[n] += pw[2] * divB[n] * A.invdt;
       if (EPI) {
         double uu[MRM_NV];
 #pragma unroll

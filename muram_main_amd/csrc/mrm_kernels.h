@@ -186,10 +186,11 @@ __device__ __forceinline__ void load_pt(const double* __restrict__ U, const doub
   p = P[node];
 }
 
-template <int D>
+template <int D, bool DIAG = false>
 __device__ __forceinline__ void resid_dir(const double* __restrict__ U, const double* __restrict__ Pr, long bs,
                                           long node, long str, const double vc[MRM_NV], double pc,
-                                          const ResidParams& RP, double acc[MRM_NV], double J[3]) {
+                                          const ResidParams& RP, double acc[MRM_NV], double J[3],
+                                          double* dg = nullptr) {
   double v[5][MRM_NV], p[5];
   load_pt(U, Pr, bs, node - 2 * str, v[0], p[0]);
   load_pt(U, Pr, bs, node - str, v[1], p[1]);
@@ -199,7 +200,7 @@ __device__ __forceinline__ void resid_dir(const double* __restrict__ U, const do
   for (int q = 0; q < MRM_NV; q++) v[2][q] = vc[q];
   p[2] = pc;
   double res[MRM_NV];
-  sweep_residual<D>(v, p, RP.w1[D], RP.w2[D], RP.inv_va2max, res);
+  sweep_residual<D, DIAG>(v, p, RP.w1[D], RP.w2[D], RP.inv_va2max, res, dg);
   if (RP.eta > 0.0) {
     constexpr int D2 = D == 0 ? 1 : 0;
     constexpr int D3 = D == 2 ? 1 : 2;
@@ -217,10 +218,12 @@ __device__ __forceinline__ void resid_dir(const double* __restrict__ U, const do
   for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
 }
 
-// R += residual over the interior; cmax of the interior points into red
+// R += residual over the interior; cmax of the interior points into red.
+// DIAG: tvar1 and tvar3 of the cell (mhdres_SR.C:368-383, 434-435)
+template <bool DIAG = false>
 __global__ void __launch_bounds__(MRM_BLOCK) k_residual(Geom G, ResidParams RP, const double* __restrict__ U,
                                                         const double* __restrict__ Pr, double* __restrict__ R,
-                                                        unsigned long long* red) {
+                                                        unsigned long long* red, DiagArgs DA) {
   __shared__ double lds[MRM_BLOCK / 64];
   const long bs = G.bs;
   double cm[1] = {0.0};
@@ -234,12 +237,17 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_residual(Geom G, ResidParams RP, 
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) acc[q] = R[q * bs + node];
     double J[3] = {0.0, 0.0, 0.0};
-    resid_dir<0>(U, Pr, bs, node, 1, vc, pc, RP, acc, J);
-    resid_dir<1>(U, Pr, bs, node, G.s1, vc, pc, RP, acc, J);
-    resid_dir<2>(U, Pr, bs, node, G.s2, vc, pc, RP, acc, J);
+    double dg[2] = {0.0, 0.0};
+    resid_dir<0, DIAG>(U, Pr, bs, node, 1, vc, pc, RP, acc, J, dg);
+    resid_dir<1, DIAG>(U, Pr, bs, node, G.s1, vc, pc, RP, acc, J, dg);
+    resid_dir<2, DIAG>(U, Pr, bs, node, G.s2, vc, pc, RP, acc, J, dg);
     if (RP.eta > 0.0) acc[4] += RP.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) R[q * bs + node] = acc[q];
+    if (DIAG) {
+      DA.dg[DG_T1 * bs + node] += dg[0];
+      DA.dg[DG_T3 * bs + node] += dg[1];
+    }
     cm[0] = smax(0.0, point_cmax(vc, pc, RP.inv_va2max));
   }
   block_reduce<OpMax, 1>(cm, lds);
@@ -282,9 +290,11 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_cmax_faces(Geom G, double inv_va2
 }
 
 // ---------------------------------------------------------------- sources / integrate
+// DIAG: tvar4..6 set and tvar7 added to (Add_Sources_Integrate_SR.C:122-296)
+template <bool DIAG = false>
 __global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParams SP, double* __restrict__ U,
                                                                 const double* __restrict__ U0, double* __restrict__ R,
-                                                                const double* __restrict__ Qtot) {
+                                                                const double* __restrict__ Qtot, DiagArgs DA) {
   const long bs = G.bs;
   int i, j, k;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -298,11 +308,18 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParam
     u0[q] = U0[q * bs + node];
   }
   double Qrad = Qtot ? Qtot[node] : 0.0;
-  source_integrate_cell(SP, up, r, u0, Qrad, out);
+  double dg[4];
+  source_integrate_cell<DIAG>(SP, up, r, u0, Qrad, out, dg);
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
     U[q * bs + node] = out[q];
     R[q * bs + node] = 0.0;
+  }
+  if (DIAG) {
+    DA.dg[DG_T4 * bs + node] = dg[0];
+    DA.dg[DG_T5 * bs + node] = dg[1];
+    DA.dg[DG_T6 * bs + node] = dg[2];
+    DA.dg[DG_T7 * bs + node] += (out[4] - dg[3]) * DA.idt;
   }
 }
 
@@ -315,8 +332,9 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_save_cons(Geom G, const double* _
   for (int q = 0; q < MRM_NV; q++) U0[q * G.bs + node] = U[q * G.bs + node];
 }
 
+// with DA.dg, tvar7 += the clamp's change of e / dt (Add_Sources_Integrate_SR.C:618-620)
 __global__ void __launch_bounds__(MRM_BLOCK) k_tcheck(Geom G, double rho_min, double eps_min, double eps_max,
-                                                      double vmax, double* __restrict__ U) {
+                                                      double vmax, double* __restrict__ U, DiagArgs DA) {
   int i, j, k;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (!interior_ijk(G, t, i, j, k)) return;
@@ -324,9 +342,11 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tcheck(Geom G, double rho_min, do
   double u[MRM_NV];
 #pragma unroll
   for (int q = 0; q < 5; q++) u[q] = U[q * G.bs + node];
+  const double eei = u[4];
   tcheck_vals(u, rho_min, eps_min, eps_max, vmax);
 #pragma unroll
   for (int q = 0; q < 5; q++) U[q * G.bs + node] = u[q];
+  if (DA.dg) DA.dg[DG_T7 * G.bs + node] += (u[4] - eei) * DA.idt;
 }
 
 // ---------------------------------------------------------------- exchange (periodic, one rank)
@@ -523,10 +543,13 @@ __device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
 }
 
 // VALF: the dim-2 epilogue also reduces Adjust_Valf_Max's statistics
-// (va_adjust > 0); that variant is held to two waves per SIMD like the others
-template <int D, bool HAS_TAU, bool EPI, bool VALF = false>
+// (va_adjust > 0); that variant is held to two waves per SIMD like the others.
+// DIAG: tvar8, tvar6, Qres, tvar7 and Qvis of every updated point
+// (tvdlimit_SR.C:577-666)
+template <int D, bool HAS_TAU, bool EPI, bool VALF = false, bool DIAG = false>
 __global__ void __launch_bounds__(MRM_BLOCK) __attribute__((amdgpu_waves_per_eu(VALF ? 2 : 1)))
-k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restrict__ Pr, TvdEpilogue X) {
+k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restrict__ Pr, TvdEpilogue X,
+            DiagArgs DA) {
   __shared__ double lds[(VALF ? 5 : 1) * MRM_BLOCK / 64];
   __shared__ double vacc[VALF ? 5 * MRM_BLOCK : 1];
   if (VALF) valf_lds_init(vacc + threadIdx.x, MRM_BLOCK);
@@ -557,7 +580,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
     const int istart = 1 + (P.is_gbeg0 ? gh : 0);
     TvdPt W0, W1, W2, W3;      // points t-3, t-2, t-1, t
     double S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
-    double qm = 0.0;
+    double qm = 0.0, qv = 0.0, qvm = 0.0;
     for (int s = 0; s < sz; s++) {
       long node = base + s * str;
       double u[MRM_NV];
@@ -577,13 +600,21 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
         const double ih = D == 0 ? P.hfb[ii] : lhfb, it = D == 0 ? P.hft[ii] : lhft;
         const double iq = D == 0 ? P.qft[ii] : lqft;
         bool zb = (D == 0) && ((P.is_gend0 && ii == sz - gh - 1) || (P.is_gbeg0 && ii == gh - 1));
-        double q = tvd_iface<D>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F);
+        double q = tvd_iface<D, DIAG>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F, &qv);
         if (ii >= gh && ii <= sz - gh - 1) {
           long nd = base + ii * str;
           double uu[MRM_NV];
 #pragma unroll
           for (int k = 0; k < MRM_NV; k++) uu[k] = U[k * bs + nd];
-          tvd_update(P, W1, Fm, F, qm, q, iq, uu);
+          double dg[4];
+          tvd_update<DIAG>(P, W1, Fm, F, qm, q, iq, uu, dg);
+          if (DIAG) {
+            DA.dg[DG_T8 * bs + nd] += dg[0] * DA.idt;
+            DA.dg[DG_T6 * bs + nd] += dg[1] * DA.idt;
+            DA.dg[DG_QRES * bs + nd] += DA.dt_fac * dg[2];
+            DA.dg[DG_T7 * bs + nd] += (uu[4] - dg[3]) * DA.idt;
+            DA.dg[DG_QVIS * bs + nd] += DA.dt_fac * (0.5 * (qv + qvm) * iq);
+          }
           if (EPI) {
             tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
             const double pn = ghost_pres_(X.E, uu);
@@ -603,6 +634,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
 #pragma unroll
         for (int k = 0; k < MRM_NV; k++) Fm[k] = F[k];
         qm = q;
+        qvm = qv;
       }
       W0 = W1; W1 = W2; W2 = W3;
     }

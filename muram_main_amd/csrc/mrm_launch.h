@@ -31,20 +31,37 @@ inline hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const TvdEpi
                                    const Geom& G, const TvdParams& P, double* U, const double* Pr) {
   TvdEpilogue X{};
   if (d == 0) {
-    if (tau) k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
-    else k_tvd_sweep<0, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+    if (tau) k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
+    else k_tvd_sweep<0, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
   } else if (d == 1) {
-    if (tau) k_tvd_sweep<1, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
-    else k_tvd_sweep<1, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+    if (tau) k_tvd_sweep<1, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
+    else k_tvd_sweep<1, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
   } else if (epi && epi->valf) {
-    if (tau) k_tvd_sweep<2, true, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
-    else k_tvd_sweep<2, false, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
+    if (tau) k_tvd_sweep<2, true, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, DiagArgs{});
+    else k_tvd_sweep<2, false, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, DiagArgs{});
   } else if (epi) {
-    if (tau) k_tvd_sweep<2, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
-    else k_tvd_sweep<2, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi);
+    if (tau) k_tvd_sweep<2, true, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, DiagArgs{});
+    else k_tvd_sweep<2, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, DiagArgs{});
   } else {
-    if (tau) k_tvd_sweep<2, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
-    else k_tvd_sweep<2, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X);
+    if (tau) k_tvd_sweep<2, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
+    else k_tvd_sweep<2, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
+  }
+  return hipGetLastError();
+}
+
+// the same sweeps with the diagnostics of every updated point (no epilogue)
+inline hipError_t launch_tvd_sweep_diag(hipStream_t st, int d, bool tau, unsigned nb, const Geom& G,
+                                        const TvdParams& P, double* U, const double* Pr, const DiagArgs& DA) {
+  TvdEpilogue X{};
+  if (d == 0) {
+    if (tau) k_tvd_sweep<0, true, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DA);
+    else k_tvd_sweep<0, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DA);
+  } else if (d == 1) {
+    if (tau) k_tvd_sweep<1, true, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DA);
+    else k_tvd_sweep<1, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DA);
+  } else {
+    if (tau) k_tvd_sweep<2, true, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DA);
+    else k_tvd_sweep<2, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DA);
   }
   return hipGetLastError();
 }

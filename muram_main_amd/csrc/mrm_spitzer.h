@@ -21,11 +21,12 @@ struct SptParams {
 };
 
 // one interior cell per thread; U primitive (after ConsToPrim), T its
-// temperature over the whole buffer
+// temperature over the whole buffer.  With t2 (diagnostics), tvar2 += the
+// conductive energy flux divergence (mhdres_SR.C:368-376, 689-694)
 __global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_resid(Geom G, SptParams S, const double* __restrict__ U,
                                                              const double* __restrict__ T,
                                                              const double* __restrict__ sflx, double* __restrict__ R,
-                                                             double* __restrict__ Rflx) {
+                                                             double* __restrict__ Rflx, double* __restrict__ t2) {
   const long bs = G.bs;
   int i, j, k;
   if (!interior_ijk(G, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
@@ -43,6 +44,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_resid(Geom G, SptParams S
     bgt += Bd[n] * (w1 * (T[n + s] - T[n - s]) + w2 * (T[n + 2 * s] - T[n - 2 * s]));
   }
   R[4 * bs + n] += re;
+  if (t2) t2[n] += re;
   const double v0 = U[bs + n], v1 = U[2 * bs + n], v2 = U[3 * bs + n];
   const double b0 = U[5 * bs + n], b1 = U[6 * bs + n], b2 = U[7 * bs + n];
   const double vv = sqrt(v0 * v0 + v1 * v1 + v2 * v2);

@@ -94,6 +94,11 @@ struct mrm_ctx {
   // Spitzer conduction (param_spitzer > 0): sflx0 and Rflx; sflx is field MRM_SFLX
   double* d_spt = nullptr;
 
+  // diagnostics tvar1..8, Qres, Qvis (DG_N arrays of bs doubles, fields
+  // MRM_TVAR1..MRM_QVIS), allocated by their first use
+  double* d_diag = nullptr;
+  int diag_last = 0;   // need_diagnostics of the last call that took a run descriptor
+
   unsigned long long* d_red = nullptr;   // RED_NSLOT slots
   double* d_part = nullptr;
   long npart = 0;
@@ -314,7 +319,7 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   for (int q = 0; q < 3; q++) { hipFree(c->S[q]); hipFree(c->mxmem[q]); }
   for (int q = 0; q < 2; q++) hipFree(c->Pm[q]);
-  hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2); hipFree(c->d_spt);
+  hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2); hipFree(c->d_spt); hipFree(c->d_diag);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
   for (double* p : c->adj_ckpt) hipFree(p);
   hipFree(c->adj_mem);
@@ -464,10 +469,20 @@ extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
   return MRM_OK;
 }
 
+static int diag_ensure(mrm_ctx* c);
+
 extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
-  if (!c || !host || f < 0 || f >= MRM_NFIELD) return MRM_ERR_ARG;
-  c->fast_valid = false;
+  if (!c || !host || f < 0 || f >= MRM_TVAR1 + DG_N) return MRM_ERR_ARG;
   const size_t bs = c->G.bs;
+  if (f >= MRM_TVAR1) {
+    int rc = diag_ensure(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_diag + (f - MRM_TVAR1) * bs, host, bs * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRM_OK;
+  }
+  c->fast_valid = false;
   HIPCHK(c, hipMemcpyAsync(c->dF + f * bs, host, bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (f == MRM_QTOT) c->qtot_set = true;
@@ -476,8 +491,19 @@ extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
 }
 
 extern "C" int mrm_get_field(mrm_ctx* c, int f, double* host) {
-  if (!c || !host || f < 0 || f >= MRM_NFIELD) return MRM_ERR_ARG;
+  if (!c || !host || f < 0 || f >= MRM_TVAR1 + DG_N) return MRM_ERR_ARG;
   const size_t bs = c->G.bs;
+  if (f >= MRM_TVAR1) {
+    // diagnostics never collected read as zero
+    if (!c->d_diag) {
+      std::fill(host, host + bs, 0.0);
+      return MRM_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(host, c->d_diag + (f - MRM_TVAR1) * bs, bs * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRM_OK;
+  }
   if (c->precision == MRM_PREC_FP32_MIXED && f == MRM_PRES) {
     // the FP32 pressure carried by the mixed step, widened
     std::vector<float> pf(bs);
@@ -532,11 +558,14 @@ extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
 static void valf_apply(mrm_ctx* c, double stats[5]);
 
 // device reductions + the host-side update of Adjust_Valf_Max.C:97-144
+static int diag_track(mrm_ctx* c, const mrm_run_desc* run);
+
 extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
-  (void)run;
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_adjust_valf_max");
   c->fast_valid = false;
+  int rc = diag_track(c, run);
+  if (rc) return rc;
   HIPCHK(c, hipMemsetAsync(c->d_red, 0, RED_NSLOT * sizeof(unsigned long long), c->stream));
   long n = ninterior(c);
   unsigned nb = nblk(n);
@@ -619,6 +648,56 @@ static double eta_dt_limit(const mrm_ctx* c, const mrm_run_desc* run, double dt_
   return std::min(dt_cfl, dt_res);
 }
 
+// ------------------------------------------------------------------ diagnostics
+static int diag_ensure(mrm_ctx* c) {
+  if (c->d_diag) return MRM_OK;
+  const size_t n = (size_t)DG_N * c->G.bs;
+  hipError_t e = hipMalloc(&c->d_diag, n * sizeof(double));
+  if (e != hipSuccess) {
+    c->d_diag = nullptr;
+    return fail(c, MRM_ERR_HIP, "hipMalloc diagnostics", e);
+  }
+  HIPCHK(c, hipMemsetAsync(c->d_diag, 0, n * sizeof(double), c->stream));
+  return MRM_OK;
+}
+
+// solver.C:264-296 zeroes the arrays where need_diagnostics turns on (stage 4
+// of an output step) and turns it off at the next step's stage 1.  A caller
+// that sets the flag as the solver does gets the same zeroing: the first
+// call that sees it set after calls that did not zeroes the arrays.
+static int diag_track(mrm_ctx* c, const mrm_run_desc* run) {
+  const int on = run && run->need_diagnostics;
+  int rc = MRM_OK;
+  if (on && !c->diag_last && c->precision == MRM_PREC_FP64) rc = mrm_diag_zero(c);
+  c->diag_last = on;
+  return rc;
+}
+
+// the arrays a call adds its diagnostics to: none without need_diagnostics;
+// idt = 1 / Run.dt, dt_fac = dt_tvd / Run.dt
+static int diag_args(mrm_ctx* c, const mrm_run_desc* run, DiagArgs& DA, double dt_tvd = 0.0) {
+  DA = DiagArgs{};
+  int rc = diag_track(c, run);
+  if (rc) return rc;
+  if (!run || !run->need_diagnostics) return MRM_OK;
+  rc = diag_ensure(c);
+  if (rc) return rc;
+  DA.dg = c->d_diag;
+  DA.idt = 1.0 / run->dt;
+  DA.dt_fac = dt_tvd / run->dt;
+  return MRM_OK;
+}
+
+// the zeroing of solver.C:271-296 at the start of the stage that collects them
+extern "C" int mrm_diag_zero(mrm_ctx* c) {
+  if (!c) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_diag_zero");
+  c->diag_last = 1;
+  if (!c->d_diag) return diag_ensure(c);
+  HIPCHK(c, hipMemsetAsync(c->d_diag, 0, (size_t)DG_N * c->G.bs * sizeof(double), c->stream));
+  return MRM_OK;
+}
+
 static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run);
 
 static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
@@ -629,10 +708,14 @@ static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
   }
   RP.inv_va2max = c->inv_va2max;
   eta_params(c, RP);
+  DiagArgs DA;
+  int rc = diag_args(c, run, DA);
+  if (rc) return rc;
   HIPCHK(c, hipMemsetAsync(c->d_red + RED_CMAX, 0, sizeof(unsigned long long), c->stream));
   const double* P = c->dF + MRM_PRES * c->G.bs;
   { TScope ts(c, T_RESID);
-  k_residual<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red);
+  if (DA.dg) k_residual<true><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red, DA);
+  else k_residual<false><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red, DA);
   }
   LAUNCHCHK(c, "k_residual");
   long nf = 2L * c->G.g0 * c->lsize[1] * c->lsize[2] + 2L * c->G.g1 * c->lsize[0] * c->lsize[2] +
@@ -670,8 +753,12 @@ static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run) {
     S.hyp_diff = eps_hyp * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
   }
   const long bs = c->G.bs;
+  DiagArgs DA;
+  int rc = diag_args(c, run, DA);
+  if (rc) return rc;
   k_spitzer_resid<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, S, c->dU, c->dF + MRM_TEMP * bs,
-                                                                   c->dF + MRM_SFLX * bs, c->dR, c->d_spt + bs);
+                                                                   c->dF + MRM_SFLX * bs, c->dR, c->d_spt + bs,
+                                                                   DA.dg ? DA.dg + DG_T2 * bs : nullptr);
   LAUNCHCHK(c, "k_spitzer_resid");
   return MRM_OK;
 }
@@ -686,7 +773,6 @@ static int read_cmax(mrm_ctx* c, double* cmax) {
 extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_cfl) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_mhd_residual");
-  if (run && run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   int rc = launch_residual(c, run);
   if (rc) return rc;
   double cmax;
@@ -701,6 +787,8 @@ static int invalid_dt(double a) { return (a != a) || (a <= 0.) || (a == INFINITY
 
 extern "C" int mrm_sync_timestep(mrm_ctx* c, mrm_run_desc* run, double dt_rad, double dt_mhd, double* cfl) {
   if (!c || !run) return MRM_ERR_ARG;
+  int rc = diag_track(c, run);
+  if (rc) return rc;
   double r0 = dt_rad, r1 = dt_mhd;
   if (invalid_dt(r0)) return fail(c, MRM_ERR_INVALID_DT, "Invalid time step: dt_rad");
   if (invalid_dt(r1)) return fail(c, MRM_ERR_INVALID_DT, "Invalid time step: dt_mhd");
@@ -741,8 +829,14 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
   c->fast_valid = false;
   SrcParams SP = src_params(c, run, stage);
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
+  DiagArgs DA;
+  int rc = diag_args(c, run, DA);
+  if (rc) return rc;
   TScope ts(c, T_SRCINT);
-  k_source_integrate<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q);
+  if (DA.dg)
+    k_source_integrate<true><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q, DA);
+  else
+    k_source_integrate<false><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q, DA);
   LAUNCHCHK(c, "k_source_integrate");
   if (c->d_spt) {
     const long bs = c->G.bs;
@@ -768,15 +862,17 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
 }
 
 extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
-  (void)run;
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_tcheck");
   c->fast_valid = false;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  DiagArgs DA;
+  int rc = diag_args(c, run, DA);
+  if (rc) return rc;
   TScope ts(c, T_TCHECK);
   k_tcheck<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax,
-                                                          c->dU);
+                                                          c->dU, DA);
   LAUNCHCHK(c, "k_tcheck");
   return MRM_OK;
 }
@@ -870,11 +966,15 @@ static int exchange_arrays(mrm_ctx* c, double* base, int na) {
 // kernel also makes the first potential update, each later one the next
 // pass's; with epi the last correction pass finishes every cell.  The two
 // maxima land in d_red[RED_DIVB0/1] and c->divb_pending is set.
-static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi) {
+static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi, const DiagArgs* DA = nullptr) {
   int rc;
   if ((rc = divb_settle(c, nullptr))) return rc;
   const Geom& G = c->G;
-  const DivbArgs A = divb_args(c);
+  DivbArgs A = divb_args(c);
+  if (DA && DA->dg) {
+    A.qres = DA->dg + DG_QRES * G.bs;
+    A.invdt = DA->idt;
+  }
   double* divB = c->dF + MRM_DIVB * G.bs;
   double* phi = c->dF + MRM_PHI * G.bs;
   unsigned long long* red = c->d_red;
@@ -906,10 +1006,12 @@ static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi) {
 extern "C" int mrm_clean_divB(mrm_ctx* c, const mrm_run_desc* run, double err[2]) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_clean_divB");
-  if (run && run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   if (c->phys.divB[1] < 1.0) return fail(c, MRM_ERR_ARG, "mrm_clean_divB: divB_itmax < 1");
   c->fast_valid = false;
-  int rc = clean_divb_dev(c, c->dU, nullptr);
+  DiagArgs DA;
+  int rc = diag_args(c, run, DA);
+  if (rc) return rc;
+  rc = clean_divb_dev(c, c->dU, nullptr, &DA);
   if (rc) return rc;
   return divb_settle(c, err);
 }
@@ -1021,10 +1123,11 @@ static int bc_mx(mrm_ctx* c, MxState M, float* P, unsigned long long* cmax) {
 }
 
 extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, int stage) {
-  (void)run; (void)stage;
+  (void)stage;
   if (!c) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
+  if (int rc = diag_track(c, run)) return rc;
   if (c->precision == MRM_PREC_FP32_MIXED) return bc_mx(c, c->M[c->cur], nullptr, nullptr);
   int rc = bc_buf(c, c->dU, nullptr, nullptr);
   if (rc || !c->d_spt) return rc;
@@ -1141,16 +1244,41 @@ static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const
   return MRM_OK;
 }
 
+// the three sweeps with the diagnostics of every updated point: the marching
+// kernels in every dimension, exact build
+static int tvd_buf_diag(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const DiagArgs& DA) {
+  TvdParams P;
+  int rc = tvd_params(c, dt_tvd, P);
+  if (rc) return rc;
+  const Geom& G = c->G;
+  const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
+  for (int d = 0; d < 3; d++) {
+    P.cmax = 0.975 * c->dx[d] / dt_tvd;
+    P.idxd1 = idx[d];
+    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
+    TScope ts(c, T_TVD0 + d);
+    hipError_t e = exact::launch_tvd_sweep_diag(c->stream, d, P.tau != nullptr, nblk(nl), G, P, U, Pr, DA);
+    if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep", e);
+  }
+  return MRM_OK;
+}
+
 extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
   if (!c || !run) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_tvdlimit");
   c->fast_valid = false;
+  DiagArgs DA;
+  int rc = diag_args(c, run, DA, dt_tvd);
+  if (rc) return rc;
+  if (DA.dg) return tvd_buf_diag(c, c->dU, c->dF + MRM_PRES * c->G.bs, dt_tvd, DA);
   return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, dt_tvd, nullptr);
 }
 
 // ------------------------------------------------------------------ full step
 // the reference's call sequence, one drop-in operation after the other
-static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
+// with diag, stage 4 and everything after it collect the diagnostics, zeroed
+// first (solver.C:264-296); run->need_diagnostics is left set
+static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad, bool diag = false) {
   int rc;
 #define STEPCHK(x) do { rc = (x); if (rc) return rc; } while (0)
   double cfl = 0.0;
@@ -1158,6 +1286,10 @@ static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   for (int stage = 1; stage <= 4; stage++) {
     STEPCHK(mrm_cons_to_prim(c));
     if (stage == 1) STEPCHK(mrm_adjust_valf_max(c, run, nullptr));
+    if (diag && stage == 4) {
+      run->need_diagnostics = 1;
+      STEPCHK(mrm_diag_zero(c));
+    }
     if (stage == 1) {
       double dt_mhd;
       STEPCHK(mrm_mhd_residual(c, run, &dt_mhd));
@@ -1474,18 +1606,26 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
 extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
   if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
-  if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
-  // Spitzer conduction runs through the per-operation kernels
+  const int want_diag = run->need_diagnostics;
+  if (want_diag && mx) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_step: diagnostics need an FP64 context");
+  // Spitzer conduction, and the step that collects diagnostics, run through
+  // the per-operation kernels
   const bool fused = !c->force_sequence && !c->d_spt;
+  run->need_diagnostics = 0;
+  int rc = MRM_OK;
   for (int it = 0; it < nsteps; it++) {
-    int rc = mx ? step_mixed(c, run, dt_rad) : fused ? step_fused(c, run, dt_rad) : step_sequence(c, run, dt_rad);
-    if (rc) return rc;
+    const bool diag = want_diag && it == nsteps - 1;
+    if (!diag) c->diag_last = 0;
+    rc = mx ? step_mixed(c, run, dt_rad) : (fused && !diag) ? step_fused(c, run, dt_rad)
+                                                            : step_sequence(c, run, dt_rad, diag);
+    if (rc) break;
     run->iteration++;
     run->globiter++;
     run->time += run->dt;
   }
-  return MRM_OK;
+  run->need_diagnostics = want_diag;
+  return rc;
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -1585,7 +1725,8 @@ static int adj_recompute(mrm_ctx* c, int n, const double* Un, AdjBufs& B, double
   HIPCHK(c, hipMemcpyAsync(B.Z, B.W, f8b, hipMemcpyDeviceToDevice, c->stream));
   const double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   const double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
-  k_tcheck<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax, B.Z);
+  k_tcheck<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, c->phys.tchk[1], c->phys.tchk[0], eps_max, vmax, B.Z,
+                                                          DiagArgs{});
   LAUNCHCHK(c, "k_tcheck");
   if ((rc = exchange_buf(c, B.Z, nullptr))) return rc;
   if ((rc = bottom_sums(c, 0, B.Z, B.small + 8))) return rc;

@@ -74,6 +74,24 @@ struct ResidParams {
   double ww0[3] = {0, 0, 0}, ww1[3] = {0, 0, 0}, ww2[3] = {0, 0, 0};
 };
 
+// diagnostics arrays tvar1..8, Qres, Qvis (grid.H:69-100): array t at
+// dg + t * bs.  idt = 1 / Run.dt, dt_fac = dt_tvd / Run.dt (tvdlimit_SR.C:115-116)
+#define DG_T1 0
+#define DG_T2 1
+#define DG_T3 2
+#define DG_T4 3
+#define DG_T5 4
+#define DG_T6 5
+#define DG_T7 6
+#define DG_T8 7
+#define DG_QRES 8
+#define DG_QVIS 9
+#define DG_N 10
+struct DiagArgs {
+  double* dg = nullptr;
+  double idt = 0.0, dt_fac = 0.0;
+};
+
 struct BcParams {
   double p_bc, s_bc, B_crit, eps_top, cc;
   int open_top;

@@ -136,6 +136,16 @@ class MhdGrid:
     def exchange_grid(self):
         self._chk(self.lib.mrm_exchange_grid(self.h), "exchange_grid")
 
+    # ------------------------------------------------------------ diagnostics
+    def zero_diagnostics(self):
+        """the zeroing of tvar1..8, Qres, Qvis at the start of the stage that
+        collects them (solver.C:271-296)"""
+        self._chk(self.lib.mrm_diag_zero(self.h), "zero_diagnostics")
+
+    def diagnostics(self) -> dict:
+        """tvar1..tvar8, Qres, Qvis (grid.H:69-100) by name, each [nz, ny, nx]"""
+        return {n: self.get_field(abi.TVAR1 + t) for t, n in enumerate(abi.DIAG_NAMES)}
+
     def SetBoundaryConditions(self, run, stage: int):
         self._chk(self.lib.mrm_set_boundary_conditions(self.h, C.byref(run), stage),
                   "SetBoundaryConditions")

@@ -8,8 +8,10 @@
  * std::max / std::min semantics (NaN handling included) are kept where the
  * reference uses them, fmax / fmin where it uses those.
  *
- * Out of scope and rejected at creation: Spitzer conduction, resistivity,
- * ambipolar diffusion, damping, potential-field top boundary, diagnostics.
+ * Covered besides the ideal-MHD step: Adjust_Valf_Max, Clean_divB,
+ * resistivity, Spitzer conduction, restart of the state, and the
+ * diagnostics arrays (tvar1..8, Qres, Qvis).  Out of scope: ambipolar
+ * diffusion, damping, the potential-field top boundary.
  */
 #include "muram_oracle.h"
 
@@ -70,6 +72,9 @@ struct orc_ctx {
   double* curlB;
   /* Spitzer conduction (grid.C:311-318): sflx lives in fld[MRM_SFLX] */
   double *sflx0, *Rflx, *BgradT;
+  /* diagnostics tvar1..8, Qres, Qvis (grid.C:319-329, grid.H:69-100), allocated at the
+     first orc_diag_zero or first call with need_diagnostics */
+  double* diag[10];
 };
 
 void orc_set_threads(int n) {
@@ -140,6 +145,7 @@ void orc_destroy(orc_ctx* c) {
   free(c->hfb); free(c->hft); free(c->qft);
   free(c->curlB);
   free(c->sflx0); free(c->Rflx); free(c->BgradT);
+  for (int t = 0; t < 10; t++) free(c->diag[t]);
   free(c);
 }
 
@@ -422,6 +428,23 @@ void orc_adjust_valf_max(orc_ctx* c, const mrm_run_desc* run, double stats[5]) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* diagnostics arrays tvar1..8, Qres, Qvis: every operation called with     */
+/* need_diagnostics adds its terms to them; the caller zeroes them at the    */
+/* start of the stage that collects them, as solver.C:271-296 does.          */
+enum { DG_T1, DG_T2, DG_T3, DG_T4, DG_T5, DG_T6, DG_T7, DG_T8, DG_QRES, DG_QVIS };
+void orc_diag_zero(orc_ctx* c) {
+  for (int t = 0; t < 10; t++) {
+    if (!c->diag[t]) c->diag[t] = (double*)malloc(sizeof(double) * (size_t)c->bufsize);
+    memset(c->diag[t], 0, sizeof(double) * (size_t)c->bufsize);
+  }
+}
+static double** diag_of(orc_ctx* c, const mrm_run_desc* run) {
+  if (!run || !run->need_diagnostics) return NULL;
+  if (!c->diag[0]) orc_diag_zero(c);
+  return c->diag;
+}
+double* orc_diag(orc_ctx* c, int t) { return (t >= 0 && t < 10) ? c->diag[t] : NULL; }
+
 /* MHD_Residual (mhdres_SR.C:27-854) with Spitzer = ambipolar =             */
 /* diagnostics = 0: three sweeps d = 0,1,2 (loop_order :86), each over the   */
 /* interior transverse lines and the full line along d1 incl. ghosts.  With  */
@@ -468,6 +491,7 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
   }
   const double* sflxg = c->fld[MRM_SFLX];
   const double* temp = c->fld[MRM_TEMP];
+  double** dg = diag_of(c, run);
   if (spitzer) memset(c->BgradT, 0, sizeof(double) * (size_t)bs);
   double cmax = 0.0;
   for (int d = 0; d < 3; d++) {
@@ -552,6 +576,17 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
               res[v] = 0.0;
               res[v] -= w1 * (F[i + 1] - F[i - 1]) + w2 * (F[i + 2] - F[i - 2]);
             }
+            if (dg) {   /* mhdres_SR.C:368-383, 688-697 */
+              if (spitzer) {
+                const double* Bd = var + (5 + d1) * sz;
+                double dn = -(w1 * ((sfl[i + 1] * Bd[i + 1]) - (sfl[i - 1] * Bd[i - 1])) +
+                              w2 * ((sfl[i + 2] * Bd[i + 2]) - (sfl[i - 2] * Bd[i - 2])));
+                dg[DG_T1][node] += res[4] - dn;
+                dg[DG_T2][node] += dn;
+              } else {
+                dg[DG_T1][node] += res[4];
+              }
+            }
             if (spitzer && d < 2) {
               const double* Bd = var + (5 + d1) * sz;
               c->Rflx[node] -= hyp_diff * ((sfl[i + 2] + sfl[i - 2]) - 4. * (sfl[i + 1] + sfl[i - 1]) + 6. * sfl[i]);
@@ -578,6 +613,7 @@ double orc_mhd_residual(orc_ctx* c, const mrm_run_desc* run) {
             res[2] += lrt1;
             res[3] += lrt2;
             res[4] += var[sz + i] * lrt0 + var[2 * sz + i] * lrt1 + var[3 * sz + i] * lrt2;
+            if (dg) dg[DG_T3][node] += var[sz + i] * lrt0 + var[2 * sz + i] * lrt1 + var[3 * sz + i] * lrt2;
             if (needs_curlB) {
               double cb[3];
               cb[d1] = 0;
@@ -684,6 +720,8 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
   const double inv_va2max = c->inv_va2max;
   double *U = c->U, *U0 = c->U0, *R = c->Res;
   const double* Qtot = c->fld[MRM_QTOT];
+  double** dg = diag_of(c, run);   /* the diagnostic branch, :122-296 */
+  const double invdt = 1.0 / run->dt;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
     for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
@@ -697,6 +735,10 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
         double e_dmpi = dn * U[bs + node];
         R[bs + node] += dn;
         R[4 * bs + node] += e_dmpi + Qrad * efaci;
+        if (dg) {
+          dg[DG_T4][node] = efaci;
+          dg[DG_T5][node] = e_dmpi;
+        }
       }
       for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
         long node = off0 + i;
@@ -721,6 +763,7 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
         R[2 * bs + node] += boris1;
         R[3 * bs + node] += boris2;
         R[4 * bs + node] += boris3;
+        if (dg) dg[DG_T6][node] = boris3;
       }
       for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
         long node = off0 + i;
@@ -729,8 +772,11 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
           R[v * bs + node] = 0;
         }
       }
-      for (int i = c->lbeg[0]; i <= c->lend[0]; i++)
+      for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+        const double ee = U[4 * bs + off0 + i];
         tcheck_cell(U, bs, off0 + i, rho_min, eps_min, eps_max, vmax);
+        if (dg) dg[DG_T7][off0 + i] += (U[4 * bs + off0 + i] - ee) * invdt;
+      }
       if (c->Rflx) {   /* hyperbolic conduction (Add_Sources_Integrate_SR.C:410-419) */
         double* sflx = c->fld[MRM_SFLX];
         for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
@@ -757,10 +803,11 @@ void orc_save_cons(orc_ctx* c) {
     }
 }
 
-/* TCheck (Add_Sources_Integrate_SR.C:534-625), diagnostics off */
+/* TCheck (Add_Sources_Integrate_SR.C:534-625) */
 void orc_tcheck(orc_ctx* c, const mrm_run_desc* run) {
-  (void)run;
   const long bs = c->bufsize;
+  double** dg = diag_of(c, run);
+  const double invdt = dg ? 1.0 / run->dt : 0.0;
   const double eps_min = c->ph.tchk[0], rho_min = c->ph.tchk[1];
   const double vmax = (c->v_lim > 0.0) ? c->v_lim : c->ph.tchk[3];
   const double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->ph.tchk[2];
@@ -768,15 +815,22 @@ void orc_tcheck(orc_ctx* c, const mrm_run_desc* run) {
   for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
     for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
       long off0 = j * c->stride[1] + k * c->stride[2];
-      for (int i = c->lbeg[0]; i <= c->lend[0]; i++)
+      for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+        const double eei = c->U[4 * bs + off0 + i];
         tcheck_cell(c->U, bs, off0 + i, rho_min, eps_min, eps_max, vmax);
+        if (dg) dg[DG_T7][off0 + i] += (c->U[4 * bs + off0 + i] - eei) * invdt;
+      }
     }
 }
 
 /* ------------------------------------------------------------------------ */
-/* TVDlimit (tvdlimit_SR.C:33-695), diagnostics and ambipolar off.           */
+/* TVDlimit (tvdlimit_SR.C:33-695), ambipolar off.  With need_diagnostics   */
+/* it adds tvar8 (energy diffusion), tvar6 (its Boris projection), Qres,     */
+/* tvar7 (the energy floor) and Qvis (viscous heating), :577-666.            */
 void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
-  (void)run;
+  double** dg = diag_of(c, run);
+  const double idt_full = dg ? 1.0 / run->dt : 0.0;
+  const double dt_fac = dg ? dt_tvd / run->dt : 0.0;
   const mrm_phys_desc* P = &c->ph;
   const long bs = c->bufsize;
   const double rho_min = P->tchk[1], eps_min = P->tchk[0];
@@ -1027,12 +1081,21 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             qrho[i] = (var[5 * sz + i + 1] - var[5 * sz + i]) * flx[5 * sz + i] +
                       (var[6 * sz + i + 1] - var[6 * sz + i]) * flx[6 * sz + i] +
                       (var[7 * sz + i + 1] - var[7 * sz + i]) * flx[7 * sz + i];
+          /* viscous-heating interface term for Qvis (:627-634), kept in hyp_e
+             which is no longer needed */
+          if (dg)
+            for (int i = 1; i < sz - 2; i++)
+              hyp_e[i] = (var[1 * sz + i + 1] - var[1 * sz + i]) * flx[1 * sz + i] +
+                         (var[2 * sz + i + 1] - var[2 * sz + i]) * flx[2 * sz + i] +
+                         (var[3 * sz + i + 1] - var[3 * sz + i]) * flx[3 * sz + i];
           for (int i = gh; i < sz - gh; i++) {
+            long node = offset + i * str;
             double r[NV];
             for (int v = 0; v < NV; v++) r[v] = dt_tvd * (flx[v * sz + i] - flx[v * sz + i - 1]);
             double v1 = var[sz + i], v2 = var[2 * sz + i], v3 = var[3 * sz + i];
             double b5 = var[5 * sz + i], b6 = var[6 * sz + i], b7 = var[7 * sz + i];
             r[4] += (1.0 - qft1[i]) * (r[1] * v1 + r[2] * v2 + r[3] * v3);
+            if (dg) dg[DG_T8][node] += r[4] * idt_full;
             double dn = (r[1] * b5 + r[2] * b6 + r[3] * b7) / smax(1e-100, bsqr[i]);
             double boris0 = -BC2[i] * (r[1] - dn * b5);
             double boris1 = -BC2[i] * (r[2] - dn * b6);
@@ -1041,11 +1104,17 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             r[1] += boris0; r[2] += boris1; r[3] += boris2; r[4] += boris3;
             double qres = 0.5 * (qrho[i] + qrho[i - 1]) * qft1[i];
             r[4] += dt_tvd * qres;
-            long node = offset + i * str;
             for (int v = 0; v < NV; v++) U[v * bs + node] += r[v];
             U[node] = smax(rho_min, U[node]);
             double mx = U[bs + node], my = U[2 * bs + node], mz = U[3 * bs + node];
+            const double e_pre = U[4 * bs + node];
             U[4 * bs + node] = smax(U[4 * bs + node], eps_min * U[node] + 0.5 * (mx * mx + my * my + mz * mz) / U[node]);
+            if (dg) {
+              dg[DG_T6][node] += boris3 * idt_full;
+              dg[DG_QRES][node] += dt_fac * qres;
+              dg[DG_T7][node] += (U[4 * bs + node] - e_pre) * idt_full;
+              dg[DG_QVIS][node] += dt_fac * (0.5 * (hyp_e[i] + hyp_e[i - 1]) * qft1[i]);
+            }
           }
         }
       free(mem);
@@ -1151,8 +1220,9 @@ static double divb_interior(orc_ctx* c) {
 /* how much each call reduces max |div B|.  err = {before, after} in the     */
 /* reference's units.                                                        */
 void orc_clean_divb(orc_ctx* c, const mrm_run_desc* run, double err_out[2]) {
-  (void)run;
   const long bs = c->bufsize;
+  double** dg = diag_of(c, run);   /* Qres += phi divB / dt (clean_divB.C:264-270) */
+  const double invdt = dg ? 1.0 / run->dt : 0.0;
   const int it_max = (int)c->ph.divB[1];
   const double divB_err = c->ph.divB[2];
   const double divB_coeff = 1.0, divB_damp = 0.65;
@@ -1202,6 +1272,8 @@ void orc_clean_divb(orc_ctx* c, const mrm_run_desc* run, double err_out[2]) {
           }
         }
         for (int i = c->lbeg[0]; i <= c->lend[0]; i++) c->U[4 * bs + off0 + i] += phi[off0 + i] * divB[off0 + i];
+        if (dg)
+          for (int i = c->lbeg[0]; i <= c->lend[0]; i++) dg[DG_QRES][off0 + i] += phi[off0 + i] * divB[off0 + i] * invdt;
       }
     exchange_arrays(c, Bv, 3);
     err = divb_interior(c);
@@ -1360,17 +1432,25 @@ void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage)
 
 /* ------------------------------------------------------------------------ */
 /* One ComputeSolution iteration (solver.C:115-365), RT and IO off          */
+/* solver.C:102-380; with run->need_diagnostics the last step collects the  */
+/* diagnostics from its stage 4 on, zeroed first (solver.C:264-296).         */
 int orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
+  const int want_diag = run->need_diagnostics;
   for (int it = 0; it < nsteps; it++) {
     double cfl = 0.0;
     orc_save_cons(c);
     for (int stage = 1; stage <= MAXSTAGE; stage++) {
       orc_cons_to_prim(c);
       if (stage == 1) orc_adjust_valf_max(c, run, NULL);
+      run->need_diagnostics = want_diag && (it == nsteps - 1) && (stage == MAXSTAGE);
+      if (run->need_diagnostics) orc_diag_zero(c);
       double dt_mhd = orc_mhd_residual(c, run);
       if (stage == 1) {
         int rc = orc_sync_timestep(c, run, dt_rad, dt_mhd, &cfl);
-        if (rc) return rc;
+        if (rc) {
+          run->need_diagnostics = want_diag;
+          return rc;
+        }
       }
       orc_source_integrate_tcheck(c, run, stage);
       orc_exchange_grid(c);
@@ -1391,5 +1471,6 @@ int orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
     run->globiter++;
     run->time += run->dt;
   }
+  run->need_diagnostics = want_diag;
   return MRM_OK;
 }

@@ -48,6 +48,10 @@ void   orc_exchange_grid(orc_ctx* c);
 void   orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage);
 void   orc_clean_divb(orc_ctx* c, const mrm_run_desc* run, double err[2]);
 int    orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps);
+/* diagnostics tvar1..8, Qres, Qvis (t = 0..9): zeroed by orc_diag_zero,
+   added to by every operation called with run->need_diagnostics */
+void   orc_diag_zero(orc_ctx* c);
+double* orc_diag(orc_ctx* c, int t);
 
 /* scalar EOS interpolations of eos_init.C:216-416 */
 double orc_p_interp(const orc_ctx* c, double ee, double dd);

@@ -43,7 +43,8 @@ extern double p_bc, s_bc;
 enum {
   OP_STARTUP = 0, OP_SAVECONS = 1, OP_C2P = 2, OP_VALF = 3, OP_RESID = 4,
   OP_SYNC = 5, OP_SRCINT = 6, OP_EXCHANGE = 7, OP_BC = 8, OP_TVD = 9,
-  OP_TCHECK = 10, OP_STEP = 11, OP_ADVANCE = 12, OP_DIVB = 13
+  OP_TCHECK = 10, OP_STEP = 11, OP_ADVANCE = 12, OP_DIVB = 13, OP_DIAG = 14,
+  OP_STEPDIAG = 15
 };
 #define OP_DUMP 0x100
 
@@ -81,12 +82,23 @@ static void bc(const RunData& Run, GridData& Grid, const PhysicsData& Physics, i
   SetBoundaryConditions(Run, Grid, Physics, stage == 1 ? 2 : stage, 1, nullptr);
 }
 
-// one iteration of the ComputeSolution loop body (solver.C:115-365)
-static void full_step(RunData& Run, GridData& Grid, const PhysicsData& Physics) {
+// the zeroing of the diagnostics arrays at the start of the stage that
+// collects them (solver.C:271-296)
+static void diag_zero(GridData& Grid) {
+  for (double* a : {Grid.tvar1, Grid.tvar2, Grid.tvar3, Grid.tvar4, Grid.tvar5, Grid.tvar6, Grid.tvar7,
+                    Grid.tvar8, Grid.Qres, Grid.Qvis})
+    memset(a, 0, sizeof(double) * Grid.bufsize);
+}
+
+// one iteration of the ComputeSolution loop body (solver.C:115-365); with
+// diag, stage 4 and what follows it collect the diagnostics (solver.C:264-267)
+static void full_step(RunData& Run, GridData& Grid, const PhysicsData& Physics, bool diag = false) {
   SaveCons(Grid, Physics);
   for (int stage = 1; stage <= maxstage; stage++) {
     ConsToPrim(Grid, Physics, Run);
     if (stage == 1) Adjust_Valf_Max(Run, Grid, Physics);
+    Run.need_diagnostics = diag && stage == maxstage;
+    if (Run.need_diagnostics) diag_zero(Grid);
     g_dt_mhd = MHD_Residual(Run, Grid, Physics);
     if (stage == 1) sync_timestep(Run, Physics, g_dt_rad, g_dt_mhd, &g_cfl);
     Source_Integrate_Tcheck(Run, Grid, Physics, stage);
@@ -105,6 +117,7 @@ static void full_step(RunData& Run, GridData& Grid, const PhysicsData& Physics) 
   exchange_grid_acc(Grid, Physics, 0);
   bc(Run, Grid, Physics, maxstage + 1);
   Run.Advance();
+  Run.need_diagnostics = 0;
 }
 
 static void rd(void* p, size_t sz, size_t n, FILE* f) {
@@ -130,7 +143,9 @@ int main(int argc, char** argv) {
   Run.rank = 0; Run.zrank = 0;
   Run.dt = rdsc.dt; Run.time = rdsc.time; Run.CFL = rdsc.CFL; Run.CFL_tvd = rdsc.CFL_tvd;
   Run.outcad = rdsc.outcad; Run.autostep = rdsc.autostep;
-  Run.need_diagnostics = 0; Run.diagnostics = 0; Run.verbose = 0;
+  // diagnostics = 1 only allocates tvar1..8, Qres, Qvis (grid.C:319-335);
+  // need_diagnostics is set by OP_DIAG and OP_STEPDIAG
+  Run.need_diagnostics = 0; Run.diagnostics = 1; Run.verbose = 0;
   Run.globiter = rdsc.globiter; Run.iteration = rdsc.iteration;
   strncpy(Run.eos_name, eos_path, 127); Run.eos_name[127] = 0;
 
@@ -171,6 +186,7 @@ int main(int argc, char** argv) {
   for (double* a : {Grid.sflx, Grid.sflx0, Grid.Rflx, Grid.BgradT})
     if (a) memset(a, 0, sizeof(double) * bs);
   std::vector<double> zeros(bs, 0.0);
+  diag_zero(Grid);
 
   std::vector<double> soa(8L * bs);
   rd(soa.data(), 8, 8L * bs, f);
@@ -205,6 +221,13 @@ int main(int argc, char** argv) {
       case OP_STEP: for (int s = 0; s < arg; s++) full_step(Run, Grid, Physics); break;
       case OP_ADVANCE: Run.Advance(); break;
       case OP_DIVB: Clean_divB(Run, Grid, Physics); break;
+      case OP_DIAG:
+        Run.need_diagnostics = arg;
+        if (arg) diag_zero(Grid);
+        break;
+      case OP_STEPDIAG:
+        for (int s = 0; s < arg; s++) full_step(Run, Grid, Physics, s == arg - 1);
+        break;
       default: fprintf(stderr, "harness: bad op %d\n", op); MPI_Abort(MPI_COMM_WORLD, 2);
     }
     if (ops[2 * o] & OP_DUMP) {
@@ -222,6 +245,9 @@ int main(int argc, char** argv) {
       for (double* a : {Grid.pres, Grid.temp, Grid.ne, Grid.rhoi, Grid.amb, Grid.divB, Grid.phi})
         fwrite(a, 8, bs, out);
       fwrite(Grid.sflx ? Grid.sflx : zeros.data(), 8, bs, out);
+      for (double* a : {Grid.tvar1, Grid.tvar2, Grid.tvar3, Grid.tvar4, Grid.tvar5, Grid.tvar6, Grid.tvar7,
+                        Grid.tvar8, Grid.Qres, Grid.Qvis})
+        fwrite(a, 8, bs, out);
     }
   }
   fclose(out);

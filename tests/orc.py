@@ -62,6 +62,9 @@ class Oracle:
             L.orc_set_boundary_conditions.argtypes = [P, C.POINTER(RunDesc), C.c_int]
             L.orc_step.argtypes = [P, C.POINTER(RunDesc), C.c_double, C.c_int]
             L.orc_clean_divb.argtypes = [P, C.POINTER(RunDesc), dp]
+            L.orc_diag_zero.argtypes = [P]
+            L.orc_diag.argtypes = [P, C.c_int]
+            L.orc_diag.restype = dp
             for n in ("orc_p_interp", "orc_s_interp", "orc_T_interp", "orc_d3_interp", "orc_eps3_interp"):
                 getattr(L, n).argtypes = [P, C.c_double, C.c_double]
                 getattr(L, n).restype = C.c_double
@@ -113,6 +116,16 @@ class Oracle:
     def set_bc(self, p, s):
         self.L.orc_set_bc(self.h, p, s)
 
+    def diag(self):
+        """tvar1..8, Qres, Qvis as a [10, nz, ny, nx] copy (zeros before any
+        diagnostics were collected)"""
+        out = np.zeros((10,) + self.shape)
+        for t in range(10):
+            p = self.L.orc_diag(self.h, t)
+            if p:
+                out[t] = np.ctypeslib.as_array(p, shape=(self.bs,)).reshape(self.shape)
+        return out
+
     def clean_divb(self, run):
         err = np.zeros(2)
         self.L.orc_clean_divb(self.h, C.byref(run), _as_dp(err))
@@ -125,7 +138,7 @@ class Oracle:
 
 # --------------------------------------------------------------------------
 OP = dict(STARTUP=0, SAVECONS=1, C2P=2, VALF=3, RESID=4, SYNC=5, SRCINT=6, EXCHANGE=7,
-          BC=8, TVD=9, TCHECK=10, STEP=11, ADVANCE=12, DIVB=13)
+          BC=8, TVD=9, TCHECK=10, STEP=11, ADVANCE=12, DIVB=13, DIAG=14, STEPDIAG=15)
 DUMP = 0x100
 
 
@@ -175,7 +188,10 @@ def run_reference(grid: GridDesc, phys: PhysDesc, run: RunDesc, table: bytes, U:
         off += 40 * bs
         Do = np.frombuffer(raw, np.float64, 3 * bs, off).reshape((3,) + shape[1:]).copy()
         off += 24 * bs
-        dumps.append(dict(op=o, sc=sc, U=Uo, Res=Ro, fields=Fo, divB=Do[0], phi=Do[1], sflx=Do[2]))
+        Go = np.frombuffer(raw, np.float64, 10 * bs, off).reshape((10,) + shape[1:]).copy()
+        off += 80 * bs
+        dumps.append(dict(op=o, sc=sc, U=Uo, Res=Ro, fields=Fo, divB=Do[0], phi=Do[1], sflx=Do[2],
+                          diag=Go))
     return dumps
 
 

@@ -512,3 +512,74 @@ def test_spitzer_flux_runs_down_the_temperature_gradient(gold_spt):
     strong = np.abs(bgt) > 0.1 * np.abs(bgt).max()
     assert strong.sum() > 20
     assert np.mean(np.sign(s[strong]) == -np.sign(bgt[strong])) > 0.95
+
+
+# ---------------------------------------------------------------- diagnostics
+@pytest.fixture(scope="module")
+def gold_diag():
+    return np.load(os.path.join(GOLD, "ref_diag.npz"))
+
+
+def diag_cases():
+    import sys
+    sys.path.insert(0, GOLD)
+    import make_golden_diag
+    return make_golden_diag.cases()
+
+
+def run_diag_case(table, case, U, phys=None, plain_steps=2):
+    box, phys0, cfl_tvd = diag_cases()[case]
+    o = Oracle(config.grid_desc(box.gsize, box.dx), phys or phys0, table)
+    r = config.run_desc(CFL=0.8, CFL_tvd=cfl_tvd)
+    o.set_state(U)
+    o.exchange_grid()
+    o.set_boundary_conditions(r, 0)
+    o.step(r, 1e30, plain_steps)
+    return o, r
+
+
+@pytest.mark.parametrize("case", ["diag_plain", "diag_full"])
+def test_golden_diagnostics(gold_diag, table, case):
+    """tvar1..8, Qres, Qvis of the last of three steps, bit for bit the
+    compiled reference's, and need_diagnostics handed back unchanged"""
+    o, r = run_diag_case(table, case, gold_diag[f"{case}_in"])
+    r.need_diagnostics = 1
+    o.step(r, 1e30, 1)
+    assert r.need_diagnostics == 1
+    assert r.dt == gold_diag[f"{case}_sc"][0]
+    G = o.diag()[(slice(None),) + (slice(2, -2),) * 3]
+    for t, name in enumerate(abi.DIAG_NAMES):
+        assert np.array_equal(G[t], gold_diag[f"{case}_diag"][t]), name
+
+
+@pytest.mark.parametrize("case", ["diag_plain", "diag_full"])
+def test_diagnostics_close_the_energy_budget(gold_diag, table, case):
+    """Over the collecting step the change of the total energy of every
+    interior cell is dt times tvar1 + tvar2 + tvar3 (MHD_Residual at stage 4:
+    flux divergence, conduction, v.Lorentz force), + tvar5 + tvar6 (gravity,
+    Boris terms of the integration and of TVDlimit), + tvar7 (every clamp of
+    e), + tvar8 (TVD energy diffusion), + Qres (resistive heating of TVDlimit
+    and Clean_divB's phi divB); Qtot = 0 and resistivity is switched off here
+    (its eta J^2 has no diagnostic).  A dropped term, a wrong sign or a wrong
+    1/dt or dt_tvd/dt factor breaks the balance."""
+    box, phys, _ = diag_cases()[case]
+    phys.params[config.I_PARAM_ETA] = 0.0
+    o, r = run_diag_case(table, case, gold_diag[f"{case}_in"], phys)
+    e0 = o.state(0)[4].copy()
+    r.need_diagnostics = 1
+    o.step(r, 1e30, 1)
+    inner = (slice(2, -2),) * 3
+    de = (o.state(0)[4] - e0)[inner]
+    G = o.diag()[(slice(None),) + inner]
+    terms = (0, 1, 2, 4, 5, 6, 7, 8)
+
+    def imbalance(skip=None):
+        rate = sum(G[t] for t in terms if t != skip)
+        return np.abs(de - r.dt * rate).max() / np.abs(de).max()
+
+    assert imbalance() < 1e-12, imbalance()
+    # each term is needed: leaving one out breaks the balance far above that
+    # (conduction, the Boris terms and the clamps act in the corona case only)
+    needed = terms if case == "diag_full" else (0, 2, 4, 7, 8)
+    for t in needed:
+        assert imbalance(t) > 1e-4, abi.DIAG_NAMES[t]
