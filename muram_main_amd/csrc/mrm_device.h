@@ -7,9 +7,13 @@
 // that every wavefront load is a contiguous 512-byte segment.
 //
 // Arithmetic keeps the reference's operation order; the library is built with
-// -ffp-contract=off so no a*b+c is fused.  smax/smin reproduce std::max /
-// std::min (NaN handling included); fmax/fmin are used where the reference
-// calls those.
+// -ffp-contract=off so no a*b+c is fused.  smax/smin stand for the
+// reference's std::max / std::min and compile to one v_max_f64 / v_min_f64
+// (the compare-and-select form costs three instructions): for ordered
+// operands the value is the same, only the sign of a zero result may differ
+// (std::max(-0., +0.) is -0.), and a NaN operand gives the other operand.
+// The states these kernels see hold no NaN (ConsToPrim's fallbacks replace
+// invalid EOS values before any limiter reads them).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,8 +34,8 @@ namespace MRM_NS {
 
 
 
-MRM_HD double smax(double a, double b) { return (a < b) ? b : a; }
-MRM_HD double smin(double a, double b) { return (b < a) ? b : a; }
+MRM_HD double smax(double a, double b) { return __builtin_fmax(a, b); }
+MRM_HD double smin(double a, double b) { return __builtin_fmin(a, b); }
 MRM_HD int imax_(int a, int b) { return (a < b) ? b : a; }
 MRM_HD int imin_(int a, int b) { return (b < a) ? b : a; }
 
