@@ -18,8 +18,8 @@ namespace MRM_NS {
 
 __device__ __forceinline__ int mx_slot(int q) { return q < 4 ? q : q - 1; }
 
-__device__ __forceinline__ float fsmax(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float fsmin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fsmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float fsmin(float a, float b) { return __builtin_fminf(a, b); }
 
 // FP32 reciprocal, quotient and square root from the hardware estimates
 // (v_rcp_f32, v_sqrt_f32: about 1 ulp), adequate for the FP32 stencils
