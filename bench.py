@@ -334,6 +334,8 @@ def main():
                     help="override param_va_adjust of the workload (1: Adjust_Valf_Max's adaptive limiter)")
     ap.add_argument("--eta", type=float, default=None,
                     help="param_eta (resistivity, cm^2/s); default: the workload's (0)")
+    ap.add_argument("--spitzer", type=float, default=None,
+                    help="param_spitzer (Spitzer heat conduction; FP64 only, so the FP32-mixed leg is skipped)")
     ap.add_argument("--divb", action="store_true",
                     help="Clean_divB on (divB_switch 1, at most 5 passes, error target 0.2)")
     ap.add_argument("--stage-zwin", default=None, choices=["0", "1"])
@@ -368,6 +370,11 @@ def main():
         wl.phys.divB[:] = [1, 5, 0.2]
     if args.eta is not None:
         wl.phys.params[config.I_PARAM_ETA] = args.eta
+    if args.spitzer is not None:
+        wl.phys.params[config.I_PARAM_SPITZER] = args.spitzer
+        if args.precision == "both":
+            args.precision = "fp64"
+            log("Spitzer conduction runs in FP64 contexts only: FP32-mixed leg skipped")
     box = wl.box
     t = time.perf_counter()
     spec = eos_table.TableSpec(n_eps=args.table_n, n_lr=args.table_n, n_lp3=200, n_s3=150)
@@ -409,6 +416,7 @@ def main():
                    "steps_timed": args.steps, "dt_s": R["dt_s"],
                    "path": args.path, "va_adjust": wl.phys.params[config.I_PARAM_VA_ADJUST],
                    "divb": bool(wl.phys.divB[0] > 0), "eta": wl.phys.params[config.I_PARAM_ETA],
+                   "spitzer": wl.phys.params[config.I_PARAM_SPITZER],
                    "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "1"),
                    "fma": os.environ.get("MRM_FMA", "1")},
         "hbm_frac_model": R["hbm_frac_model"],

@@ -198,7 +198,8 @@ int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage
  * phys.divB[0] > 0, TCheck, exchange, BC, Run.Advance.  dt_rad is the
  * radiative time step the caller supplies (RT is outside the path).  With
  * resistivity the stage kernels add the resistive terms; with Spitzer
- * conduction the step runs the per-operation kernels.  With
+ * conduction two kernels per stage form the temperature and the conduction
+ * terms, which the stage kernel adds.  With
  * run->need_diagnostics the last step collects the diagnostics from its
  * stage 4 on, zeroed first (solver.C:264-296), through the per-operation
  * kernels; run->need_diagnostics is left as it was. */

@@ -99,7 +99,8 @@ MRM_HD double eps3_interp(const EosDev& E, double pp, double ss) {
 // Out: v (3), e_int, and p/T/ne/rhoi/amb in out[0..4].  With NOUT = 2 only
 // p and T are produced (T decides the validity fallback of p); NOUT = 5
 // produces all five.
-template <int NOUT>
+// WANT_T with NOUT = 2: the temperature is always formed (out[1])
+template <int NOUT, bool WANT_T = false>
 MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
                                                   double& m3, double& e, double out[5]) {
   const double eos_gamma = 1.65, eos_mu = 0.62, eos_mu_n = 1.3;
@@ -155,7 +156,7 @@ MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
     // exp(tt) is finite and positive for |tt| < 700, so it is evaluated only
     // when a fallback branch uses it or tt lies outside that range.
     const bool plain = !(lgr < E.xlr[0]) && !(eps > 0.8 * E.eps_max);
-    if (plain && tt > -700.0 && tt < 700.0) {
+    if (!WANT_T && plain && tt > -700.0 && tt < 700.0) {
       temp = 1.0;
     } else {
       temp = exp(tt);

@@ -73,8 +73,9 @@ __device__ __forceinline__ long xcd_remap(long b, long nb) {
 // ZWIN = true: dim-2 neighbours from a register window (one wave per SIMD);
 // ZWIN = false: dim-2 neighbours re-read through the caches each plane.
 // ETA: resistive terms (param_eta > 0) in the three sweeps and eta |J|^2 in
-// the energy residual (mhdres_SR.C:422-462).
-template <bool ZWIN, bool ETA = false>
+// the energy residual (mhdres_SR.C:422-462).  ERES: A.eres holds a further
+// energy residual per cell (Spitzer conduction's), added after the sweeps.
+template <bool ZWIN, bool ETA = false, bool ERES = false>
 __global__ void __launch_bounds__(ST_SX* ST_SY, ZWIN ? 1 : 2)
 k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ Pin, const double* __restrict__ U0,
         double* __restrict__ Uout, double* __restrict__ Pout, const double* __restrict__ Qtot) {
@@ -202,6 +203,7 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
       if (ETA) acc[4] += A.RP.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
       // sources, Boris, RK update, Tcheck
       const unsigned node = col + (unsigned)k * s2u;
+      if (ERES) acc[4] += ld32(A.eres, node);
       double u0[MRM_NV], out[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) u0[q] = ld32(U0 + q * bs, node);

@@ -6,17 +6,21 @@
 
 namespace MRM_NS {
 
+template <bool ZWIN>
+inline void launch_stage_z(hipStream_t st, unsigned nb, const StageArgs& A, const double* Uin, const double* Pin,
+                           const double* U0, double* Uout, double* Pout, const double* Q) {
+  const bool eta = A.RP.eta > 0.0, eres = A.eres != nullptr;
+  if (eta && eres) k_stage<ZWIN, true, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  else if (eta) k_stage<ZWIN, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  else if (eres) k_stage<ZWIN, false, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  else k_stage<ZWIN><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+}
+
 inline hipError_t launch_stage(hipStream_t st, unsigned nb, bool zwin, const StageArgs& A, const double* Uin,
                                const double* Pin, const double* U0, double* Uout, double* Pout,
                                const double* Q) {
-  if (A.RP.eta > 0.0) {
-    if (zwin) k_stage<true, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-    else k_stage<false, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-  } else if (zwin) {
-    k_stage<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-  } else {
-    k_stage<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
-  }
+  if (zwin) launch_stage_z<true>(st, nb, A, Uin, Pin, U0, Uout, Pout, Q);
+  else launch_stage_z<false>(st, nb, A, Uin, Pin, U0, Uout, Pout, Q);
   return hipGetLastError();
 }
 

@@ -105,4 +105,75 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_bc(Geom G, int hot_plate,
   }
 }
 
+// ---------------------------------------------------------------- fused step
+// The fused step keeps conservative states and carries only the pressure, so
+// conduction gets two kernels per stage around the stage kernel:
+// k_eos_temp forms ConsToPrim's temperature of the stage's input state over
+// the whole buffer (cons_to_prim.C:108-198), and k_spitzer_stage does what
+// k_spitzer_resid and k_spitzer_integrate do for one stage -- the conductive
+// energy term goes to eres, which the stage kernel adds to the energy
+// residual after its three sweeps, and sflx_out = sflx0 + wdt Rflx with Rflx
+// formed from sflx_in (the integrator leaves Rflx zero between stages, so it
+// never needs storing).
+
+__global__ void __launch_bounds__(MRM_BLOCK) k_eos_temp(Geom G, EosDev E, const double* __restrict__ U,
+                                                        double* __restrict__ T) {
+  const long bs = G.bs;
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    double m1 = U[bs + n], m2 = U[2 * bs + n], m3 = U[3 * bs + n], e = U[4 * bs + n], pr[5];
+    cons_to_prim_cell<2, true>(E, U[n], m1, m2, m3, e, pr);
+    T[n] = pr[1];
+  }
+}
+
+// one interior cell per thread; U conservative (the stage's input), T its
+// temperature, sflx_in with its ghosts filled
+__global__ void __launch_bounds__(MRM_BLOCK) k_spitzer_stage(Geom G, SptParams S, double wdt,
+                                                             const double* __restrict__ U,
+                                                             const double* __restrict__ T,
+                                                             const double* __restrict__ sflx,
+                                                             const double* __restrict__ sflx0,
+                                                             double* __restrict__ sflx_out,
+                                                             double* __restrict__ eres) {
+  const long bs = G.bs;
+  int i, j, k;
+  if (!interior_ijk(G, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const long n = i + j * G.s1 + (long)k * G.s2;
+  const long st[3] = {1, G.s1, G.s2};
+  double re = 0.0, rf = 0.0, bgt = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const long s = st[d];
+    const double* Bd = U + (5 + d) * bs;
+    const double w1 = S.w1[d], w2 = S.w2[d];
+    re -= w1 * (sflx[n + s] * Bd[n + s] - sflx[n - s] * Bd[n - s]) +
+          w2 * (sflx[n + 2 * s] * Bd[n + 2 * s] - sflx[n - 2 * s] * Bd[n - 2 * s]);
+    rf -= S.hyp_diff * ((sflx[n + 2 * s] + sflx[n - 2 * s]) - 4. * (sflx[n + s] + sflx[n - s]) + 6. * sflx[n]);
+    bgt += Bd[n] * (w1 * (T[n + s] - T[n - s]) + w2 * (T[n + 2 * s] - T[n - 2 * s]));
+  }
+  eres[n] = re;
+  // primitives of the cell as ConsToPrim forms them (cons_to_prim.C:110-118)
+  const double d = U[n];
+  const double dn = 1.0 / d;
+  const double v0 = U[bs + n] * dn, v1 = U[2 * bs + n] * dn, v2 = U[3 * bs + n] * dn;
+  const double eint = U[4 * bs + n] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2) * d;
+  const double b0 = U[5 * bs + n], b1 = U[6 * bs + n], b2 = U[7 * bs + n];
+  const double vv = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+  const double bb = b0 * b0 + b1 * b1 + b2 * b2;
+  const double tempi = T[n];
+  const double c_lim = S.cmax_hyp - vv;
+  const double tt32 = tempi * sqrt(tempi);
+  double kap = S.k0 * tempi * tt32;
+  const double F_sat = S.sat_flx * d * tt32;
+  const double invBB = 1.0 / ((1e-100 < bb) ? bb : 1e-100);
+  double F = -kap * bgt;
+  const double f_lim = F_sat / (F_sat + fabs(F * sqrt(invBB)));
+  F *= invBB * f_lim;
+  kap *= f_lim;
+  const double nu_c = c_lim * c_lim * eint / (tempi * kap);
+  const double nu = (S.nu_hyp_max < nu_c) ? S.nu_hyp_max : nu_c;
+  rf = rf + nu * (F - sflx[n]);
+  sflx_out[n] = sflx0[n] + wdt * rf;
+}
+
 }  // namespace MRM_NS

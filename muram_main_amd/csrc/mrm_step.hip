@@ -91,7 +91,9 @@ struct mrm_ctx {
   int divb_next_iter = 3;
   bool divb_pending = false;
 
-  // Spitzer conduction (param_spitzer > 0): sflx0 and Rflx; sflx is field MRM_SFLX
+  // Spitzer conduction (param_spitzer > 0): sflx0, Rflx and, for the fused
+  // step, the second sflx buffer and the conductive energy term (4 x bs);
+  // sflx is field MRM_SFLX
   double* d_spt = nullptr;
 
   // diagnostics tvar1..8, Qres, Qvis (DG_N arrays of bs doubles, fields
@@ -138,8 +140,10 @@ enum {
 static const char* const timer_names[MRM_NTIMER] = {
   "cons_to_prim", "valf_reduce", "residual", "cmax_faces", "source_integrate", "save_cons", "tcheck",
   "exchange", "bc_sums", "bc_bottom", "bc_top", "tvd_sweep_x", "tvd_sweep_y", "tvd_sweep_z",
-  "stage_fused", "eos_p", "clean_divb", "t17", "t18", "t19", "t20", "t21", "t22", "t23"};
+  "stage_fused", "eos_p", "clean_divb", "spitzer", "eos_temp", "t19", "t20", "t21", "t22", "t23"};
 static const int T_DIVB = T_NAMED + 2;
+static const int T_SPT = T_NAMED + 3;
+static const int T_EOST = T_NAMED + 4;
 
 // brackets the launches of one timer id with events on the context's stream
 struct TScope {
@@ -298,8 +302,8 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (!rc && (e = hipMemsetAsync(c->dR, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipMemsetAsync(c->dF, 0, MRM_NFIELD * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && phys->params[3] > 0.0) {   // Spitzer: sflx0, Rflx
-    if ((e = hipMalloc(&c->d_spt, 2 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc sflx", e);
-    else if ((e = hipMemsetAsync(c->d_spt, 0, 2 * bs * sizeof(double), c->stream)) != hipSuccess)
+    if ((e = hipMalloc(&c->d_spt, 4 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc sflx", e);
+    else if ((e = hipMemsetAsync(c->d_spt, 0, 4 * bs * sizeof(double), c->stream)) != hipSuccess)
       rc = fail(c, MRM_ERR_HIP, "memset sflx", e);
   }
   if (!rc && (e = hipStreamSynchronize(c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "sync", e);
@@ -566,7 +570,9 @@ extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double s
   c->fast_valid = false;
   int rc = diag_track(c, run);
   if (rc) return rc;
-  HIPCHK(c, hipMemsetAsync(c->d_red, 0, RED_NSLOT * sizeof(unsigned long long), c->stream));
+  // slots RED_CMAX..RED_CNT_E only: a fused step's Clean_divB maxima may
+  // still wait in RED_DIVB0/1 for their host update
+  HIPCHK(c, hipMemsetAsync(c->d_red, 0, (RED_CNT_E + 1) * sizeof(unsigned long long), c->stream));
   long n = ninterior(c);
   unsigned nb = nblk(n);
   if (nb > 4096) nb = 4096;
@@ -733,25 +739,10 @@ static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
 // Spitzer terms of MHD_Residual on the primitive state: constants from the
 // dt and CFL of the call (mhdres_SR.C:37-49, 136-145; Run.dt is still the
 // previous step's at stage 1)
+static SptParams spitzer_params(const mrm_ctx* c, double rdt, double rcfl);
+
 static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run) {
-  SptParams S;
-  for (int d = 0; d < 3; d++) {
-    S.w1[d] = 8. / (12. * c->dx[d]);
-    S.w2[d] = -1. / (12. * c->dx[d]);
-  }
-  const double CFL_hyp = 1.8, avr_max = 0.25, eps_hyp = 0.02, X_H = 0.7, Rgas = 8.314e7;
-  S.k0 = c->phys.params[3];
-  S.sat_flx = (1.0 + X_H) * pow(Rgas, 1.5) * sqrt(1836.) / 8.0;
-  const double rdt = run ? run->dt : 0.0, rcfl = run ? run->CFL : 0.0;
-  if (rdt > 0.0) {
-    S.cmax_hyp = std::min(rcfl, CFL_hyp) * c->dxmin / rdt;
-    S.nu_hyp_max = avr_max / rdt;
-    S.hyp_diff = eps_hyp / rdt;
-  } else {
-    S.cmax_hyp = 0.0;
-    S.nu_hyp_max = avr_max * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
-    S.hyp_diff = eps_hyp * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
-  }
+  const SptParams S = spitzer_params(c, run ? run->dt : 0.0, run ? run->CFL : 0.0);
   const long bs = c->G.bs;
   DiagArgs DA;
   int rc = diag_args(c, run, DA);
@@ -761,6 +752,27 @@ static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run) {
                                                                    DA.dg ? DA.dg + DG_T2 * bs : nullptr);
   LAUNCHCHK(c, "k_spitzer_resid");
   return MRM_OK;
+}
+
+static SptParams spitzer_params(const mrm_ctx* c, double rdt, double rcfl) {
+  SptParams S;
+  for (int d = 0; d < 3; d++) {
+    S.w1[d] = 8. / (12. * c->dx[d]);
+    S.w2[d] = -1. / (12. * c->dx[d]);
+  }
+  const double CFL_hyp = 1.8, avr_max = 0.25, eps_hyp = 0.02, X_H = 0.7, Rgas = 8.314e7;
+  S.k0 = c->phys.params[3];
+  S.sat_flx = (1.0 + X_H) * pow(Rgas, 1.5) * sqrt(1836.) / 8.0;
+  if (rdt > 0.0) {
+    S.cmax_hyp = std::min(rcfl, CFL_hyp) * c->dxmin / rdt;
+    S.nu_hyp_max = avr_max / rdt;
+    S.hyp_diff = eps_hyp / rdt;
+  } else {
+    S.cmax_hyp = 0.0;
+    S.nu_hyp_max = avr_max * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
+    S.hyp_diff = eps_hyp * S.cmax_hyp / (std::min(rcfl, CFL_hyp) * c->dxmin);
+  }
+  return S;
 }
 
 static int read_cmax(mrm_ctx* c, double* cmax) {
@@ -1326,9 +1338,44 @@ static void valf_constants(mrm_ctx* c) {
   c->va_max_old = va_max;
 }
 
+// Spitzer conduction in the fused step, one stage: ConsToPrim's temperature
+// of the stage input, then the conduction terms from sflx_in (constants from
+// dt and CFL of the stage's residual) into d_spt + 3 bs for the stage kernel
+// and sflx_out = sflx0 + wdt Rflx
+static int spitzer_stage(mrm_ctx* c, double dt_res, double cfl, double wdt, const double* U, const double* sflx_in,
+                         double* sflx_out) {
+  const Geom& G = c->G;
+  const SptParams S = spitzer_params(c, dt_res, cfl);
+  double* T = c->dF + MRM_TEMP * G.bs;
+  unsigned nb = nblk(G.bs);
+  if (nb > 65536u) nb = 65536u;
+  {
+    TScope ts(c, T_EOST);
+    k_eos_temp<<<nb, MRM_BLOCK, 0, c->stream>>>(G, c->E, U, T);
+  }
+  LAUNCHCHK(c, "k_eos_temp");
+  TScope ts(c, T_SPT);
+  k_spitzer_stage<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, S, wdt, U, T, sflx_in, c->d_spt, sflx_out,
+                                                                   c->d_spt + 3 * G.bs);
+  LAUNCHCHK(c, "k_spitzer_stage");
+  return MRM_OK;
+}
+
+// ghosts of a heat-flux buffer: periodic copies, then the vertical BC
+// (exchange.C:111-120; boundary_pdmp_1_fftw3.C:306-310, 455-466)
+static int spitzer_ghosts(mrm_ctx* c, double* f) {
+  int rc = exchange_arrays(c, f, 1);
+  if (rc) return rc;
+  const long nyz = (long)c->G.n1 * c->G.n2;
+  k_spitzer_bc<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.bnd[3] > 0.0, f);
+  LAUNCHCHK(c, "k_spitzer_bc");
+  return MRM_OK;
+}
+
 static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const double* Uin, const double* Pin,
-                        const double* U0, double* Uout, double* Pout) {
+                        const double* U0, double* Uout, double* Pout, const double* eres = nullptr) {
   StageArgs A;
+  A.eres = eres;
   A.G = c->G;
   A.E = c->E;
   for (int d = 0; d < 3; d++) {
@@ -1396,16 +1443,34 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     if ((rc = read_cmax(c, &cm))) return rc;
   }
   double cfl = 0.0;
+  // stage 1's conduction constants use Run.dt before Sync_timestep updates it
+  // (MHD_Residual runs first, mhdres_SR.C:136-145)
+  const double dt_prev = run->dt;
   if ((rc = mrm_sync_timestep(c, run, dt_rad, eta_dt_limit(c, run, c->dxmin / cm), &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
   double* U0 = c->S[i0];
   struct { double *in, *pin, *out, *pout; } st[4] = {
       {c->S[i0], Pc, c->S[i1], Pa}, {c->S[i1], Pa, c->S[i2], Pc},
       {c->S[i2], Pc, c->S[i1], Pa}, {c->S[i1], Pa, c->S[i2], nullptr}};
+  // Spitzer conduction: sflx alternates between MRM_SFLX and a second buffer
+  // (an even number of stages ends in MRM_SFLX); sflx0 is saved first
+  const bool spt = c->d_spt != nullptr;
+  double* sflx[2] = {c->dF + MRM_SFLX * G.bs, c->d_spt + 2 * G.bs};
+  if (spt) {
+    TScope ts(c, T_SPT);
+    k_spitzer_save<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, sflx[0], c->d_spt);
+    LAUNCHCHK(c, "k_spitzer_save");
+  }
   for (int s = 0; s < 4; s++) {
-    if ((rc = stage_launch(c, run, s + 1, st[s].in, st[s].pin, U0, st[s].out, st[s].pout))) return rc;
+    if (spt && (rc = spitzer_stage(c, s == 0 ? dt_prev : run->dt, run->CFL, run->dt / double(4 - s), st[s].in,
+                                   sflx[s % 2], sflx[(s + 1) % 2])))
+      return rc;
+    if ((rc = stage_launch(c, run, s + 1, st[s].in, st[s].pin, U0, st[s].out, st[s].pout,
+                           spt ? c->d_spt + 3 * G.bs : nullptr)))
+      return rc;
     if ((rc = exchange_buf(c, st[s].out, st[s].pout))) return rc;
     if ((rc = bc_buf(c, st[s].out, st[s].pout, nullptr))) return rc;
+    if (spt && (rc = spitzer_ghosts(c, sflx[(s + 1) % 2]))) return rc;
   }
   // TVD on U4 with the pressure of the stage-4 input (tvdlimit_SR.C:332 reads
   // Grid.pres of the last ConsToPrim)
@@ -1611,7 +1676,7 @@ extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps
   if (want_diag && mx) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_step: diagnostics need an FP64 context");
   // Spitzer conduction, and the step that collects diagnostics, run through
   // the per-operation kernels
-  const bool fused = !c->force_sequence && !c->d_spt;
+  const bool fused = !c->force_sequence;
   run->need_diagnostics = 0;
   int rc = MRM_OK;
   for (int it = 0; it < nsteps; it++) {

@@ -115,6 +115,7 @@ struct StageArgs {
   SrcParams SP;
   int ntx, nty, nkc;       // tiles in dim 0, dim 1, chunks in dim 2
   int write_p;             // compute the EOS pressure of the new state
+  const double* eres = nullptr;   // extra energy residual per cell (conduction), or null
 };
 
 struct TvdLineArgs {

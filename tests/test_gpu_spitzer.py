@@ -1,7 +1,7 @@
 """Spitzer heat conduction (param_spitzer > 0) on the GPU against the
 compiled reference's outputs in tests/golden/ref_spitzer.npz: the state and
-the heat flux sflx after 1 and 5 steps from sflx = 0, within 1e-12 per field
-(mrm_step runs the per-operation kernels with conduction on)."""
+the heat flux sflx after 1 and 5 steps from sflx = 0, within 1e-12 per field,
+on the fused step and on the per-operation sequence."""
 import os
 import sys
 
@@ -33,11 +33,24 @@ def maxrel(a, b):
     return max(relerr(a[v], b[v]) for v in range(a.shape[0]))
 
 
+def make(box, phys, path):
+    saved = os.environ.get("MRM_STEP_PATH")
+    os.environ["MRM_STEP_PATH"] = path
+    try:
+        return MhdGrid(config.grid_desc(box.gsize, box.dx), phys)
+    finally:
+        if saved is None:
+            os.environ.pop("MRM_STEP_PATH", None)
+        else:
+            os.environ["MRM_STEP_PATH"] = saved
+
+
+@pytest.mark.parametrize("path", ["fused", "sequence"])
 @pytest.mark.parametrize("case", ["spt_cool", "spt_corona"])
-def test_spitzer_steps_match_reference(table, gold, case):
-    print(f"case: Spitzer conduction, {case}", flush=True)
+def test_spitzer_steps_match_reference(table, gold, case, path):
+    print(f"case: Spitzer conduction, {case}, {path} step", flush=True)
     box, phys = make_golden_spitzer.cases()[case]
-    m = MhdGrid(config.grid_desc(box.gsize, box.dx), phys)
+    m = make(box, phys, path)
     m.eos_init(table)
     r = config.run_desc(CFL=0.8)
     m.set_state(gold[f"{case}_in"])

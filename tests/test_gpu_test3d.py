@@ -56,12 +56,12 @@ def restored(d, g, table, precision=abi.PREC_FP64, va_adjust=1.0, path="fused", 
     return m, r
 
 
-@pytest.mark.parametrize("va_adjust,divb,spitzer,path", [(1.0, True, True, "fused"), (1.0, True, False, "fused"),
-                                                        (1.0, True, False, "sequence"), (1.0, False, False, "fused"),
-                                                        (0.0, False, False, "fused")])
+@pytest.mark.parametrize("va_adjust,divb,spitzer,path", [(1.0, True, True, "fused"), (1.0, True, True, "sequence"),
+                                                        (1.0, True, False, "fused"), (1.0, True, False, "sequence"),
+                                                        (1.0, False, False, "fused"), (0.0, False, False, "fused")])
 def test_restored_test3d_steps_match_oracle(tmp_path, crop, table, va_adjust, divb, spitzer, path):
     # va_adjust 1 with divB cleaning and Spitzer conduction is the case's own
-    # setting (conduction runs the per-operation kernels)
+    # setting
     print(f"case: Test_3D crop 192x10x10, restore + 4 FP64 {path} steps vs oracle, va_adjust {va_adjust},"
           f" divB {divb}, Spitzer {spitzer}", flush=True)
     d, g = restart_dir(tmp_path, crop)
