@@ -1355,8 +1355,8 @@ static int spitzer_stage(mrm_ctx* c, double dt_res, double cfl, double wdt, cons
   }
   LAUNCHCHK(c, "k_eos_temp");
   TScope ts(c, T_SPT);
-  k_spitzer_stage<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, S, wdt, U, T, sflx_in, c->d_spt, sflx_out,
-                                                                   c->d_spt + 3 * G.bs);
+  k_spitzer_stage<<<spitzer_blocks(G), SPT_TX * SPT_TY, 0, c->stream>>>(G, S, wdt, U, T, sflx_in, c->d_spt,
+                                                                        sflx_out, c->d_spt + 3 * G.bs);
   LAUNCHCHK(c, "k_spitzer_stage");
   return MRM_OK;
 }
