@@ -221,6 +221,125 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------------ drop-in residual, tiled
+// MHD_Residual of the per-operation path (mhdres_SR.C:27-854) with the stage
+// kernel's tiling: U holds ConsToPrim's primitives, P the pressure; the
+// plane sits in LDS with its two-cell ring in dims 0 and 1, dim-2
+// neighbours come from a five-plane register window whose next plane is
+// loaded during the current plane's work.  R += ((R_x) + R_y) + R_z per
+// interior cell in the reference's order, and the interior points' cmax goes
+// to red[RED_CMAX] (the ghost faces are k_cmax_faces').
+template <bool ETA>
+__global__ void __launch_bounds__(ST_SX* ST_SY, 1)
+k_resid_tile(StageArgs A, const double* __restrict__ U, const double* __restrict__ P, double* __restrict__ R,
+             unsigned long long* red) {
+  __shared__ double lds[ST_NF][ST_LY][ST_LX];
+  __shared__ double lred[ST_SX * ST_SY / 64];
+  const Geom& G = A.G;
+  const long bs = G.bs;
+  const long nblocks = (long)A.ntx * A.nty * A.nkc;
+  long lb = xcd_remap(blockIdx.x, nblocks);
+  const int tx_t = (int)(lb % A.ntx);
+  lb /= A.ntx;
+  const int ty_t = (int)(lb % A.nty);
+  const int kc = (int)(lb / A.nty);
+  const int tx = threadIdx.x % ST_SX, ty = threadIdx.x / ST_SX;
+  const int i0 = G.lb0 + tx_t * ST_SX, j0 = G.lb1 + ty_t * ST_SY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = (i <= G.le0) && (j <= G.le1);
+  const int kbeg = G.lb2 + kc * ST_KC;
+  const int kend = min(kbeg + ST_KC, G.le2 + 1);     // exclusive
+  int hx_c = -1, hx_i = 0;
+  if (tx < 2) { hx_c = tx; hx_i = i0 - 2 + tx; }
+  else if (tx >= ST_SX - 2) { hx_c = tx + 4; hx_i = i0 + tx + 2; }
+  int hy_r = -1, hy_j = 0;
+  if (ty < 4) {
+    hy_r = ty < 2 ? ty : ST_SY + ty;
+    hy_j = ty < 2 ? j0 - 2 + ty : j0 + ST_SY + (ty - 2);
+  }
+  const bool hx_ok = hx_c >= 0 && hx_i < G.n0 && j < G.n1;
+  const bool hy_ok = hy_r >= 0 && hy_j < G.n1 && i < G.n0;
+  const bool own_ok = i < G.n0 && j < G.n1;
+  const unsigned col = (unsigned)i + (unsigned)j * (unsigned)G.s1;
+  const unsigned s1u = (unsigned)G.s1, s2u = (unsigned)G.s2;
+  double cm[1] = {0.0};
+  double w[5][ST_NF];
+  double rz[ST_NF], rhx[ST_NF], rhy[ST_NF];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (own_ok) load_raw(U, P, bs, col + (unsigned)(kbeg - 2 + q) * s2u, w[q + 1]);
+    else {
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0;
+    }
+  }
+  if (own_ok) load_raw(U, P, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
+  if (hx_ok) load_raw(U, P, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)kbeg * s2u, rhx);
+  if (hy_ok) load_raw(U, P, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)kbeg * s2u, rhy);
+  for (int k = kbeg; k < kend; k++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
+    if (own_ok) {
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) w[4][f] = rz[f];
+    }
+#pragma unroll
+    for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][tx + 2] = w[2][f];
+    if (hx_ok) {
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) lds[f][ty + 2][hx_c] = rhx[f];
+    }
+    if (hy_ok) {
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) lds[f][hy_r][tx + 2] = rhy[f];
+    }
+    if (k + 1 < kend) {
+      if (own_ok) load_raw(U, P, bs, col + (unsigned)(k + 3) * s2u, rz);
+      if (hx_ok) load_raw(U, P, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)(k + 1) * s2u, rhx);
+      if (hy_ok) load_raw(U, P, bs, (unsigned)i + (unsigned)hy_j * s1u + (unsigned)(k + 1) * s2u, rhy);
+    }
+    __syncthreads();
+    if (valid) {
+      const unsigned node = col + (unsigned)k * s2u;
+      double acc[MRM_NV], res[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] = ld32(R + q * bs, node);
+      const double lfc = lfac_of(w[2], A.RP.inv_va2max);
+      auto getx = [&](int s, double o[9]) {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + 2][tx + s];
+      };
+      double J[3] = {0.0, 0.0, 0.0};
+      sweep_residual_lr<0, ETA>(getx, w[2], A.RP.w1[0], A.RP.w2[0], lfc, res, &A.RP, J);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      auto gety = [&](int s, double o[9]) {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + s][tx + 2];
+      };
+      sweep_residual_lr<1, ETA>(gety, w[2], A.RP.w1[1], A.RP.w2[1], lfc, res, &A.RP, J);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      auto getz = [&](int s, double o[9]) {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) o[f] = w[s][f];
+      };
+      sweep_residual_lr<2, ETA>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res, &A.RP, J);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+      if (ETA) acc[4] += A.RP.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) st32(R + q * bs, node, acc[q]);
+      cm[0] = smax(cm[0], smax(0.0, point_cmax(w[2], w[2][8], A.RP.inv_va2max)));
+    }
+    __syncthreads();
+  }
+  block_reduce<OpMax, 1>(cm, lred);
+  if (threadIdx.x == 0) atomic_max_nonneg(red + RED_CMAX, cm[0]);
+}
+
 // ------------------------------------------------------------------ ghost fill of the fused path
 // periodic copies of U and p (exchange.C:532-782 on one rank): dim 1 at
 // interior (i,k), dim 2 at interior i and all j

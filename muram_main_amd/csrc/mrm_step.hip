@@ -720,8 +720,20 @@ static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
   HIPCHK(c, hipMemsetAsync(c->d_red + RED_CMAX, 0, sizeof(unsigned long long), c->stream));
   const double* P = c->dF + MRM_PRES * c->G.bs;
   { TScope ts(c, T_RESID);
-  if (DA.dg) k_residual<true><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red, DA);
-  else k_residual<false><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red, DA);
+  if (DA.dg) {
+    k_residual<true><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red, DA);
+  } else {
+    // the tiled kernel (plane in LDS, dim-2 register window)
+    StageArgs A;
+    A.G = c->G;
+    A.RP = RP;
+    A.ntx = (c->lsize[0] + ST_SX - 1) / ST_SX;
+    A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
+    A.nkc = (c->lsize[2] + ST_KC - 1) / ST_KC;
+    const unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
+    if (RP.eta > 0.0) k_resid_tile<true><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+    else k_resid_tile<false><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+  }
   }
   LAUNCHCHK(c, "k_residual");
   long nf = 2L * c->G.g0 * c->lsize[1] * c->lsize[2] + 2L * c->G.g1 * c->lsize[0] * c->lsize[2] +
