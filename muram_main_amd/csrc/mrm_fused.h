@@ -528,9 +528,9 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
   const int j = (int)(line % G.n1), k = (int)(line / G.n1);
   const long base = j * G.s1 + k * G.s2;
   double* sv = sm;                     // [8][sz] primitives (log rho/eps)
-  double* sS = sv + MRM_NV * sz;       // [8][sz] slopes
-  double* sF = sS + MRM_NV * sz;       // [8][sz] interface fluxes
-  double* scm = sF + MRM_NV * sz;      // cm
+  double* sS = sv + MRM_NV * sz;       // [8][sz] slopes, then the interface fluxes
+  double* sF = sS;
+  double* scm = sS + MRM_NV * sz;      // cm
   double* shd = scm + sz;              // hyperdiffusion coefficient
   double* scz = shd + sz;              // CZ
   double* slr = scz + sz;              // ln rho
@@ -566,15 +566,21 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
   __syncthreads();
   double F[MRM_NV];
   const int istart = 1 + (P.is_gbeg0 ? gh : 0);
-  if (t >= 1 && t <= sz - 3) {
+  const bool iface = t >= 1 && t <= sz - 3;
+  // the fluxes overwrite the slopes: every slope is read before any flux is stored
+  double Sc[MRM_NV];
+  if (iface) {
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) Sc[q] = sS[q * sz + t + 1];
+  }
+  __syncthreads();
+  if (iface) {
     TvdPt a, c, d;
-    double Sc[MRM_NV];
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) {
       a.v[q] = sv[q * sz + t - 1];
       c.v[q] = sv[q * sz + t + 1];
       d.v[q] = sv[q * sz + t + 2];
-      Sc[q] = sS[q * sz + t + 1];
     }
     c.cm = scm[t + 1];
     c.hd = shd[t + 1];

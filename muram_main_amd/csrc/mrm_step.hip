@@ -1223,7 +1223,7 @@ static int tvd_params(mrm_ctx* c, double dt_tvd, TvdParams& P) {
   return MRM_OK;
 }
 
-static size_t tvd_x_lds(const mrm_ctx* c) { return (size_t)(3 * MRM_NV + 5) * c->G.n0 * sizeof(double); }
+static size_t tvd_x_lds(const mrm_ctx* c) { return (size_t)(2 * MRM_NV + 5) * c->G.n0 * sizeof(double); }
 
 // the three sweeps of TVDlimit on buffer U with pressure Pr; with epi the
 // dim-2 sweep finishes each cell (TCheck, new pressure, cmax)
