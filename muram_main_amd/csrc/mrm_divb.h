@@ -10,14 +10,6 @@
 
 namespace MRM_NS {
 
-struct DivbArgs {
-  Geom G;
-  double w1[3], w2[3];     // 8/(12 dx), -1/(12 dx) per dimension (grid.C:339-340)
-  double fac, damp;        // divB_fac, divB_damp (clean_divB.C:47-48, 110)
-  double* qres = nullptr;  // diagnostics: Qres += phi divB / dt (clean_divB.C:264-270)
-  double invdt = 0.0;
-};
-
 // Tiling of the interior for the two cell kernels: a 64 x 4 (i, j) tile per
 // 256-thread block, one thread per column, marching along k through a chunk
 // of DIVB_KC planes; the k-neighbours of the marched array sit in a five-plane
@@ -88,8 +80,8 @@ __global__ void __launch_bounds__(DIVB_TX * DIVB_TY) k_divb(DivbArgs A, const do
 
 // periodic ghosts of na arrays base + q * vstride along dim DIM (the ranges of
 // exchange_B_acc / exchange_single_acc on one rank, exchange.C:785-1120)
-template <int DIM>
-__global__ void __launch_bounds__(MRM_BLOCK) k_exchange_arr(Geom G, double* __restrict__ base, int na,
+template <int DIM, class T>
+__global__ void __launch_bounds__(MRM_BLOCK) k_exchange_arr(Geom G, T* __restrict__ base, int na,
                                                             long vstride) {
   const int nx = G.le0 - G.lb0 + 1;
   const int gh = DIM == 1 ? G.g1 : G.g2;
@@ -107,7 +99,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_exchange_arr(Geom G, double* __re
   long src_r = c0 + (long)(lb + igh) * sd, dst_r = c0 + (long)(le + 1 + igh) * sd;
   long src_l = c0 + (long)(le - gh + 1 + igh) * sd, dst_l = c0 + (long)(lb - gh + igh) * sd;
   for (int q = 0; q < na; q++) {
-    double* a = base + q * vstride;
+    T* a = base + q * vstride;
     a[dst_r] = a[src_r];
     a[dst_l] = a[src_l];
   }

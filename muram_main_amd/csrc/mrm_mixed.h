@@ -13,6 +13,7 @@
 // Add_Sources_Integrate_SR.C, tvdlimit_SR.C, cons_to_prim.C).
 #pragma once
 #include "mrm_fused.h"
+#include "mrm_divb.h"
 
 namespace MRM_NS {
 
@@ -1004,6 +1005,102 @@ __global__ void __launch_bounds__(TX_MAXN) k_tvd_x_lines_mx(Geom G, MxTvdParams 
     for (int q = 0; q < MRM_NV; q++) Fm[q] = sF[q * sz + t - 1];
     mx_tvd_update(P, me, Fm, F, sq[t - 1], sq[t], (float)P.qft[t], u, e);
     mx_store_cons(S, bs, base + t, u, e);
+  }
+}
+
+// ------------------------------------------------------------------ Clean_divB
+// Clean_divB (clean_divB.C:35-369) on the mixed state: B is stored in FP32,
+// the divergence, the potential phi (FP64 field) and the corrections are
+// formed in FP64 from the widened values (k_divb / k_divb_update's
+// arithmetic), B is rounded back to FP32 and e takes phi divB in FP64.  The
+// last correction pass can finish the cells as the mixed TVD sweep would
+// (TCheck, FP32 pressure, cmax, Adjust_Valf_Max statistics).
+template <bool PHI>
+__global__ void __launch_bounds__(DIVB_TX * DIVB_TY) k_divb_mx(DivbArgs A, MxState S, double* __restrict__ divB,
+                                                               double* __restrict__ phi, unsigned long long* err) {
+  __shared__ double lds[DIVB_TX * DIVB_TY / 64];
+  const Geom& G = A.G;
+  const float* Bx = S.f + 4 * G.bs;
+  const float* By = S.f + 5 * G.bs;
+  const float* Bz = S.f + 6 * G.bs;
+  const long s1 = G.s1, s2 = G.s2;
+  double m[1] = {0.0};
+  const DivbTile t = divb_tile(G);
+  if (t.ok) {
+    double zw[5];
+#pragma unroll
+    for (int q = 0; q < 4; q++) zw[q] = (double)Bz[t.c0 + (long)(t.k0 - 2 + q) * s2];
+    for (int k = t.k0; k < t.k1; k++) {
+      const long n = t.c0 + (long)k * s2;
+      zw[4] = (double)Bz[n + 2 * s2];
+      double d = 0.0;
+      d += (A.w1[0] * ((double)Bx[n + 1] - (double)Bx[n - 1]) + A.w2[0] * ((double)Bx[n + 2] - (double)Bx[n - 2]));
+      d += (A.w1[1] * ((double)By[n + s1] - (double)By[n - s1]) +
+            A.w2[1] * ((double)By[n + 2 * s1] - (double)By[n - 2 * s1]));
+      d += (A.w1[2] * (zw[3] - zw[1]) + A.w2[2] * (zw[4] - zw[0]));
+      divB[n] = d;
+      m[0] = fmax(m[0], fabs(d));
+      if (PHI) phi[n] += A.fac * d - A.damp * phi[n];
+#pragma unroll
+      for (int q = 0; q < 4; q++) zw[q] = zw[q + 1];
+    }
+  }
+  block_reduce<OpMax, 1>(m, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(err, m[0]);
+}
+
+template <bool EPI>
+__global__ void __launch_bounds__(DIVB_TX * DIVB_TY) k_divb_update_mx(DivbArgs A, MxState S,
+                                                                      const double* __restrict__ phi,
+                                                                      const double* __restrict__ divB,
+                                                                      MxEpilogue X) {
+  __shared__ double lds[5 * DIVB_TX * DIVB_TY / 64];
+  const Geom& G = A.G;
+  const long bs = G.bs, s1 = G.s1, s2 = G.s2;
+  float* Bx = S.f + 4 * bs;
+  float* By = S.f + 5 * bs;
+  float* Bz = S.f + 6 * bs;
+  double cm[1] = {0.0};
+  ValfAcc va;
+  const DivbTile t = divb_tile(G);
+  if (t.ok) {
+    double pw[5];
+#pragma unroll
+    for (int q = 0; q < 4; q++) pw[q] = phi[t.c0 + (long)(t.k0 - 2 + q) * s2];
+    for (int k = t.k0; k < t.k1; k++) {
+      const long n = t.c0 + (long)k * s2;
+      pw[4] = phi[n + 2 * s2];
+      Bx[n] = (float)((double)Bx[n] + (A.w1[0] * (phi[n + 1] - phi[n - 1]) + A.w2[0] * (phi[n + 2] - phi[n - 2])));
+      By[n] = (float)((double)By[n] +
+                      (A.w1[1] * (phi[n + s1] - phi[n - s1]) + A.w2[1] * (phi[n + 2 * s1] - phi[n - 2 * s1])));
+      Bz[n] = (float)((double)Bz[n] + (A.w1[2] * (pw[3] - pw[1]) + A.w2[2] * (pw[4] - pw[0])));
+      double e = S.e[n] + pw[2] * divB[n];
+      if (EPI) {
+        float u[MRM_NV];
+        double ee;
+        mx_load_cons(S, bs, (unsigned)n, u, ee);
+        mx_tcheck(u, e, X.rho_min, X.eps_min, X.eps_max, X.vmax);
+        st32(S.f, (unsigned)n, u[0]);
+        st32(S.f + bs, (unsigned)n, u[1]);
+        st32(S.f + 2 * bs, (unsigned)n, u[2]);
+        st32(S.f + 3 * bs, (unsigned)n, u[3]);
+        const float pn = mx_pres(X.E, u[0], u[1], u[2], u[3], e);
+        st32(X.Pnew, (unsigned)n, pn);
+        cm[0] = smax(cm[0], mx_cmax(u[0], u[1], u[2], u[3], u[5], u[6], u[7], pn, X.inv_va2max));
+        if (X.valf) mx_valf_cell(va, u[0], u[1], u[2], u[3], e, pn, X.vlim95, X.elim95);
+      }
+      S.e[n] = e;
+#pragma unroll
+      for (int q = 0; q < 4; q++) pw[q] = pw[q + 1];
+    }
+  }
+  if (EPI) {
+    block_reduce<OpMax, 1>(cm, lds);
+    if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
+    if (X.valf) {
+      __syncthreads();
+      valf_flush(va, lds, X.valf);
+    }
   }
 }
 

@@ -86,4 +86,31 @@ hipError_t launch_convert(hipStream_t st, bool to_mixed, long bs, double* U, MxS
   return hipGetLastError();
 }
 
+hipError_t launch_divb(hipStream_t st, const DivbArgs& A, MxState S, double* divB, double* phi,
+                       unsigned long long* err, bool phi_update) {
+  const unsigned nb = divb_blocks(A.G);
+  if (phi_update) k_divb_mx<true><<<nb, DIVB_TX * DIVB_TY, 0, st>>>(A, S, divB, phi, err);
+  else k_divb_mx<false><<<nb, DIVB_TX * DIVB_TY, 0, st>>>(A, S, divB, phi, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_divb_update(hipStream_t st, const DivbArgs& A, MxState S, const double* phi, const double* divB,
+                              const MxEpilogue* epi) {
+  const unsigned nb = divb_blocks(A.G);
+  if (epi) k_divb_update_mx<true><<<nb, DIVB_TX * DIVB_TY, 0, st>>>(A, S, phi, divB, *epi);
+  else k_divb_update_mx<false><<<nb, DIVB_TX * DIVB_TY, 0, st>>>(A, S, phi, divB, MxEpilogue{});
+  return hipGetLastError();
+}
+
+hipError_t launch_exchange_b(hipStream_t st, const Geom& G, MxState S) {
+  const int nx = G.le0 - G.lb0 + 1;
+  long n1 = (long)nx * (G.le2 - G.lb2 + 1) * G.g1;
+  k_exchange_arr<1><<<nblk_(n1), MRM_BLOCK, 0, st>>>(G, S.f + 4 * G.bs, 3, G.bs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  long n2 = (long)nx * G.n1 * G.g2;
+  k_exchange_arr<2><<<nblk_(n2), MRM_BLOCK, 0, st>>>(G, S.f + 4 * G.bs, 3, G.bs);
+  return hipGetLastError();
+}
+
 }  // namespace mixed

@@ -22,4 +22,12 @@ hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t 
 hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const MxEpilogue* epi, unsigned nb, const Geom& G,
                             const MxTvdParams& P, MxState S, const float* Pr);
 hipError_t launch_convert(hipStream_t st, bool to_mixed, long bs, double* U, MxState S);
+// Clean_divB on the mixed state: div B (with the next potential update when
+// phi_update), the B/e correction (finishing the cells when epi is given),
+// and the periodic ghosts of the three FP32 B components
+hipError_t launch_divb(hipStream_t st, const DivbArgs& A, MxState S, double* divB, double* phi,
+                       unsigned long long* err, bool phi_update);
+hipError_t launch_divb_update(hipStream_t st, const DivbArgs& A, MxState S, const double* phi, const double* divB,
+                              const MxEpilogue* epi);
+hipError_t launch_exchange_b(hipStream_t st, const Geom& G, MxState S);
 }  // namespace mixed

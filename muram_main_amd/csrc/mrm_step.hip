@@ -213,7 +213,7 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
   if ((phys->params[3] > 0.0 || phys->params[4] > 0.0) && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
-  if (phys->divB[0] > 0.0 && (precision != MRM_PREC_FP64 || phys->divB[1] < 1.0)) return MRM_ERR_UNSUPPORTED;
+  if (phys->divB[0] > 0.0 && phys->divB[1] < 1.0) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
     if (grid->gsize[d] < 1 || grid->ghosts[d] != 2) return MRM_ERR_ARG;
     if (!(grid->gxmax[d] > grid->gxmin[d])) return MRM_ERR_ARG;
@@ -1027,6 +1027,35 @@ static int clean_divb_dev(mrm_ctx* c, double* U, const TvdEpilogue* epi, const D
   return MRM_OK;
 }
 
+// Clean_divB on a mixed state (FP32 B): the passes of clean_divb_dev with
+// the mixed kernels; phi and divB stay FP64 fields
+static int clean_divb_mx(mrm_ctx* c, MxState M, const MxEpilogue* epi) {
+  int rc;
+  if ((rc = divb_settle(c, nullptr))) return rc;
+  const Geom& G = c->G;
+  const DivbArgs A = divb_args(c);
+  double* divB = c->dF + MRM_DIVB * G.bs;
+  double* phi = c->dF + MRM_PHI * G.bs;
+  unsigned long long* red = c->d_red;
+  const int npass = c->divb_next_iter;
+  TScope ts(c, T_DIVB);
+  HIPCHK(c, hipMemsetAsync(red + RED_DIVB0, 0, 2 * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, mixed::launch_exchange_b(c->stream, G, M));
+  HIPCHK(c, mixed::launch_divb(c->stream, A, M, divB, phi, red + RED_DIVB0, true));
+  for (int it = 0; it < npass; it++) {
+    const bool last = it == npass - 1;
+    if ((rc = exchange_arrays(c, phi, 1))) return rc;
+    const long ncol = (long)c->lsize[1] * c->lsize[2];
+    k_phi_vertical<<<nblk(ncol), MRM_BLOCK, 0, c->stream>>>(G, phi);
+    LAUNCHCHK(c, "k_phi_vertical");
+    HIPCHK(c, mixed::launch_divb_update(c->stream, A, M, phi, divB, last ? epi : nullptr));
+    HIPCHK(c, mixed::launch_exchange_b(c->stream, G, M));
+    HIPCHK(c, mixed::launch_divb(c->stream, A, M, divB, phi, last ? red + RED_DIVB1 : red + RED_SCRATCH, !last));
+  }
+  c->divb_pending = true;
+  return MRM_OK;
+}
+
 extern "C" int mrm_clean_divB(mrm_ctx* c, const mrm_run_desc* run, double err[2]) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_clean_divB");
@@ -1629,19 +1658,26 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     HIPCHK(c, mixed::launch_eos_p(c->stream, nb, G, c->E, c->M[c->cur], Pc, c->inv_va2max, cmax,
                                   va_adj ? c->d_red : nullptr, 0.95 * c->v_lim, 0.95 * c->e_lim));
   }
+  // one round trip brings cmax, the limiter statistics and the previous
+  // Clean_divB's maxima
   double cm;
+  const int nread = c->divb_pending ? RED_DIVB1 + 1 : va_adj ? RED_CNT_E + 1 : 1;
+  HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, nread * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->divb_pending) divb_settle_host(c, nullptr);
+  cm = bits_to_double(c->h_red[RED_CMAX]);
   if (va_adj) {
-    HIPCHK(c, hipMemcpyAsync(c->h_red, c->d_red, (RED_CNT_E + 1) * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     valf_apply(c, nullptr);
     HIPCHK(c, hipMemsetAsync(cmax, 0, sizeof(unsigned long long), c->stream));
     unsigned nb = nblk(G.bs);
     if (nb > 65536u) nb = 65536u;
-    TScope ts(c, T_VALF);
-    HIPCHK(c, mixed::launch_cmax(c->stream, nb, G, c->M[c->cur], Pc, c->inv_va2max, cmax));
+    {
+      TScope ts(c, T_VALF);
+      HIPCHK(c, mixed::launch_cmax(c->stream, nb, G, c->M[c->cur], Pc, c->inv_va2max, cmax));
+    }
+    if ((rc = read_cmax(c, &cm))) return rc;
   }
-  if ((rc = read_cmax(c, &cm))) return rc;
   double cfl = 0.0;
   if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
@@ -1666,13 +1702,17 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.valf = va_adj ? c->d_red : nullptr;
   X.vlim95 = 0.95 * c->v_lim;
   X.elim95 = 0.95 * c->e_lim;
+  // with Clean_divB its last correction pass finishes the cells
+  const bool divb = c->phys.divB[0] > 0.0;
+  const MxEpilogue* tvd_epi = divb ? nullptr : &X;
   if (cfl <= run->CFL_tvd) {
-    if ((rc = tvd_mx(c, M[i2], Pa, run->dt, &X))) return rc;
+    if ((rc = tvd_mx(c, M[i2], Pa, run->dt, tvd_epi))) return rc;
   } else {
     if ((rc = tvd_mx(c, M[i2], Pa, 0.5 * run->dt, nullptr))) return rc;
     if ((rc = exchange_mx(c, M[i2], nullptr))) return rc;
-    if ((rc = tvd_mx(c, M[i2], Pa, 0.5 * run->dt, &X))) return rc;
+    if ((rc = tvd_mx(c, M[i2], Pa, 0.5 * run->dt, tvd_epi))) return rc;
   }
+  if (divb && (rc = clean_divb_mx(c, M[i2], &X))) return rc;
   if ((rc = exchange_mx(c, M[i2], Pc))) return rc;
   if ((rc = bc_mx(c, M[i2], Pc, cmax))) return rc;
   c->cur = i2;

@@ -92,6 +92,15 @@ struct DiagArgs {
   double idt = 0.0, dt_fac = 0.0;
 };
 
+// Clean_divB (clean_divB.C:35-369)
+struct DivbArgs {
+  Geom G;
+  double w1[3], w2[3];     // 8/(12 dx), -1/(12 dx) per dimension (grid.C:339-340)
+  double fac, damp;        // divB_fac, divB_damp (clean_divB.C:47-48, 110)
+  double* qres = nullptr;  // diagnostics: Qres += phi divB / dt (clean_divB.C:264-270)
+  double invdt = 0.0;
+};
+
 struct BcParams {
   double p_bc, s_bc, B_crit, eps_top, cc;
   int open_top;

@@ -92,6 +92,29 @@ def test_steps_with_clean_divb_match_reference(table, gold, path, case, va_max, 
     assert m.limits()[0] == pytest.approx(gold[f"{case}_step5_sc"][6], rel=TOL)
 
 
+@pytest.mark.parametrize("case,va_max,vadj", [("steps", 3.0e7, 0.0), ("steps_vadj", 1.0e6, 1.0)])
+def test_mixed_steps_with_clean_divb(table, gold, case, va_max, vadj):
+    """the FP32-mixed form (FP32 B, FP64 phi, divergence and corrections)
+    against the reference's FP64 states and potential after 5 steps"""
+    print(f"case: FP32-mixed steps with Clean_divB, {case}", flush=True)
+    m = MhdGrid(config.grid_desc(GSIZE, DX), divb_phys(va_max, vadj), abi.PREC_FP32_MIXED)
+    m.eos_init(table)
+    r = config.run_desc(CFL=0.8)
+    m.set_state(gold[f"{case}_in"])
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
+    m.step(r, 1e30, 5)
+    U = m.get_state()
+    errs = [relerr(U[v][INNER], gold[f"{case}_step5_U"][v][INNER]) for v in range(8)]
+    ephi = relerr(m.get_field(abi.PHI)[INNER], gold[f"{case}_step5_phi"][INNER])
+    print("per-field rel err", ["%.2e" % e for e in errs], "phi", "%.2e" % ephi, "dt", r.dt)
+    assert max(errs) <= 1e-5 and ephi <= 1e-4
+    assert r.time == pytest.approx(gold[f"{case}_step5_sc"][1], rel=1e-5)
+
+
 def test_clean_divb_refused_where_unsupported(table):
-    with pytest.raises(abi.MrmError):
-        MhdGrid(config.grid_desc(GSIZE, DX), divb_phys(), abi.PREC_FP32_MIXED)
+    p = divb_phys()
+    p.divB[1] = 0   # divB_itmax < 1
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_MIXED):
+        with pytest.raises(abi.MrmError):
+            MhdGrid(config.grid_desc(GSIZE, DX), p, prec)

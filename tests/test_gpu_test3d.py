@@ -117,13 +117,13 @@ def test_backup_restore_round_trip(tmp_path, crop, table):
         assert relerr(U2[v][inner], U[v][inner]) <= 1e-6, v
 
 
-@pytest.mark.parametrize("va_adjust", [0.0, 1.0])
-def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust):
-    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}, no divB cleaning"
-          " or conduction", flush=True)
+@pytest.mark.parametrize("va_adjust,divb", [(0.0, False), (1.0, False), (1.0, True)])
+def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust, divb):
+    print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}, divB {divb},"
+          " no conduction", flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=False, spitzer=False)
-    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=False, spitzer=False)
+    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=divb, spitzer=False)
+    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=divb, spitzer=False)
     m64.step(r64, config.DT_RAD_OFF, 20)
     mx.step(rx, config.DT_RAD_OFF, 20)
     errs = field_errors(mx.get_state(), m64.get_state(), interior=(slice(2, -2),) * 3)
