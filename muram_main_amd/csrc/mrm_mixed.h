@@ -116,13 +116,16 @@ __device__ __forceinline__ float mx_lfac(const float c[ST_NF], float inv_va2max)
 // residual of sweep D at the centre c (mhdres_SR.C:361-412) in FP32.  The
 // Lorentz force of the sweep is returned in lor[3] instead of being added to
 // res[1..3] and its work to res[4]: the caller combines it with the Boris
-// correction without cancellation.
-template <int D, class GET>
+// correction without cancellation.  ETA: the sweep's share of J = curl B and
+// eta times the second derivative of B (mhdres_SR.C:422-462), as eta_sweep.
+template <int D, bool ETA = false, class GET>
 __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1, float w2, float lf,
-                                         float res[MRM_NV], float lor[3]) {
+                                         float res[MRM_NV], float lor[3], const MxStageArgs* A = nullptr,
+                                         float* J = nullptr) {
   constexpr int D2 = D == 0 ? 1 : 0;
   constexpr int D3 = D == 2 ? 1 : 2;
   float d1[13], d2[13];
+  float bs1[3], bs2[3];   // B(+1) + B(-1), B(+2) + B(-2) (ETA only)
   {
     float pa[ST_NF], pb[ST_NF], fa[13], fb[13];
     get(1, pa);
@@ -131,6 +134,10 @@ __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1
     mx_point_flux<D>(pb, fb, D2, D3);
 #pragma unroll
     for (int q = 0; q < 13; q++) d1[q] = fb[q] - fa[q];
+    if constexpr (ETA) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) bs1[q] = pb[5 + q] + pa[5 + q];
+    }
   }
   {
     float pa[ST_NF], pb[ST_NF], fa[13], fb[13];
@@ -140,6 +147,10 @@ __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1
     mx_point_flux<D>(pb, fb, D2, D3);
 #pragma unroll
     for (int q = 0; q < 13; q++) d2[q] = fb[q] - fa[q];
+    if constexpr (ETA) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) bs2[q] = pb[5 + q] + pa[5 + q];
+    }
   }
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) res[q] = 0.0f - (w1 * d1[q] + w2 * d2[q]);
@@ -155,6 +166,14 @@ __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1
   lor[0] = lf * lrt0 + (1.0f - lf) * cb[0];
   lor[1] = lf * lrt1 + (1.0f - lf) * cb[1];
   lor[2] = lf * lrt2 + (1.0f - lf) * cb[2];
+  if constexpr (ETA) {
+    constexpr float sgn = D == 1 ? 1.0f : -1.0f;
+    J[D2] += sgn * dB3;
+    J[D3] += -sgn * dB2;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      res[5 + q] += A->eta * (A->ww0[D] * c[5 + q] + A->ww1[D] * bs1[q] + A->ww2[D] * bs2[q]);
+  }
 }
 
 // pressure from the mixed conservative values (ConsToPrim's p and its
@@ -255,6 +274,8 @@ __device__ __forceinline__ void mx_tcheck(float u[MRM_NV], double& e, float rho_
 }
 
 // ------------------------------------------------------------------ stage kernel
+// ETA: resistive terms in the sweeps and eta |J|^2 in the energy residual
+template <bool ETA = false>
 __global__ void __launch_bounds__(ST_SX* ST_SY, 2)
 k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0, MxState Sout,
            float* __restrict__ Pout, const double* __restrict__ Qtot) {
@@ -339,7 +360,8 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + 2][tx + s];
       };
-      mx_sweep<0>(getx, w[2], A.w1[0], A.w2[0], lfc, res, lor);
+      float J[3] = {0.0f, 0.0f, 0.0f};
+      mx_sweep<0, ETA>(getx, w[2], A.w1[0], A.w2[0], lfc, res, lor, &A, J);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
 #pragma unroll
@@ -348,7 +370,7 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = lds[f][ty + s][tx + 2];
       };
-      mx_sweep<1>(gety, w[2], A.w1[1], A.w2[1], lfc, res, lor);
+      mx_sweep<1, ETA>(gety, w[2], A.w1[1], A.w2[1], lfc, res, lor, &A, J);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
 #pragma unroll
@@ -357,11 +379,12 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
 #pragma unroll
         for (int f = 0; f < ST_NF; f++) o[f] = w[s][f];
       };
-      mx_sweep<2>(getz, w[2], A.w1[2], A.w2[2], lfc, res, lor);
+      mx_sweep<2, ETA>(getz, w[2], A.w1[2], A.w2[2], lfc, res, lor, &A, J);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
 #pragma unroll
       for (int q = 0; q < 3; q++) accL[q] += lor[q];
+      if (ETA) acc[4] += A.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
 
       // sources, Boris correction (Add_Sources_Integrate_SR.C:331-392), FP32
       const unsigned node = col + (unsigned)k * s2u;

@@ -10,7 +10,8 @@ static inline unsigned nblk_(long n) { return (unsigned)((n + MRM_BLOCK - 1) / M
 
 hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
                         MxState out, float* Pout, const double* Q) {
-  k_stage_mx<<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  if (A.eta > 0.0f) k_stage_mx<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  else k_stage_mx<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
   return hipGetLastError();
 }
 

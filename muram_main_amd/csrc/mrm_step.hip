@@ -211,7 +211,7 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
   if (phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
-  if ((phys->params[3] > 0.0 || phys->params[4] > 0.0) && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
+  if (phys->params[3] > 0.0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
   if (phys->divB[0] > 0.0 && phys->divB[1] < 1.0) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
@@ -1630,6 +1630,16 @@ static int stage_mx(mrm_ctx* c, const mrm_run_desc* run, int stage, MxState in, 
   A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
   A.nkc = (c->lsize[2] + ST_KC - 1) / ST_KC;
   A.write_p = Pout != nullptr;
+  {
+    ResidParams RP;
+    eta_params(c, RP);
+    A.eta = (float)RP.eta;
+    for (int d = 0; d < 3; d++) {
+      A.ww0[d] = (float)RP.ww0[d];
+      A.ww1[d] = (float)RP.ww1[d];
+      A.ww2[d] = (float)RP.ww2[d];
+    }
+  }
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
   unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
   TScope ts(c, T_NAMED);
@@ -1679,7 +1689,7 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     if ((rc = read_cmax(c, &cm))) return rc;
   }
   double cfl = 0.0;
-  if ((rc = mrm_sync_timestep(c, run, dt_rad, c->dxmin / cm, &cfl))) return rc;
+  if ((rc = mrm_sync_timestep(c, run, dt_rad, eta_dt_limit(c, run, c->dxmin / cm), &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
   MxState* M = c->M;
   struct { int in; float* pin; int out; float* pout; } st[4] = {

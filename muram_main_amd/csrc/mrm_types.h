@@ -149,6 +149,9 @@ struct MxStageArgs {
   double eps_min, eps_max;
   int ntx, nty, nkc;
   int write_p;
+  // resistivity (param_eta > 0): eta and the second-derivative weights
+  float eta = 0.0f;
+  float ww0[3] = {0, 0, 0}, ww1[3] = {0, 0, 0}, ww2[3] = {0, 0, 0};
 };
 
 struct MxTvdParams {

@@ -88,6 +88,22 @@ def test_resistive_steps_match_reference(table, gold, case, path):
     assert r.time == pytest.approx(gold[f"{case}_step5_sc"][1], rel=TOL)
 
 
-def test_resistivity_refused_where_unsupported(table):
-    with pytest.raises(abi.MrmError):
-        MhdGrid(config.grid_desc(GSIZE, DX), eta_phys(1e11), abi.PREC_FP32_MIXED)
+@pytest.mark.parametrize("case", ["eta_cfl", "eta_diff"])
+def test_mixed_resistive_steps(table, gold, case):
+    """the FP32-mixed form with resistivity (FP32 stencils, eta terms
+    included) against the reference's FP64 states after 5 steps; dt, which
+    the diffusion limit fixes in eta_diff, to FP32 accuracy"""
+    print(f"case: FP32-mixed resistive steps, {case}", flush=True)
+    m = MhdGrid(config.grid_desc(GSIZE, DX), eta_phys(float(gold[f"{case}_eta"])), abi.PREC_FP32_MIXED)
+    m.eos_init(table)
+    r = config.run_desc(CFL=0.8)
+    m.set_state(gold[f"{case}_in"])
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
+    m.step(r, 1e30, 5)
+    U = m.get_state()
+    inner = (slice(None),) + (slice(2, -2),) * 3
+    errs = [relerr(U[inner][v], gold[f"{case}_step5_U"][inner][v]) for v in range(8)]
+    print("per-field rel err", ["%.2e" % e for e in errs], "dt", r.dt, gold[f"{case}_step5_sc"][0])
+    assert max(errs) <= 1e-5
+    assert r.dt == pytest.approx(gold[f"{case}_step5_sc"][0], rel=1e-5)
