@@ -335,7 +335,7 @@ def main():
     ap.add_argument("--eta", type=float, default=None,
                     help="param_eta (resistivity, cm^2/s); default: the workload's (0)")
     ap.add_argument("--spitzer", type=float, default=None,
-                    help="param_spitzer (Spitzer heat conduction; FP64 only, so the FP32-mixed leg is skipped)")
+                    help="param_spitzer (Spitzer heat conduction); default: the workload's (0)")
     ap.add_argument("--divb", action="store_true",
                     help="Clean_divB on (divB_switch 1, at most 5 passes, error target 0.2)")
     ap.add_argument("--stage-zwin", default=None, choices=["0", "1"])
@@ -372,9 +372,6 @@ def main():
         wl.phys.params[config.I_PARAM_ETA] = args.eta
     if args.spitzer is not None:
         wl.phys.params[config.I_PARAM_SPITZER] = args.spitzer
-        if args.precision == "both":
-            args.precision = "fp64"
-            log("Spitzer conduction runs in FP64 contexts only: FP32-mixed leg skipped")
     box = wl.box
     t = time.perf_counter()
     spec = eos_table.TableSpec(n_eps=args.table_n, n_lr=args.table_n, n_lp3=200, n_s3=150)

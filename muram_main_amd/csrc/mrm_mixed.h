@@ -14,6 +14,7 @@
 #pragma once
 #include "mrm_fused.h"
 #include "mrm_divb.h"
+#include "mrm_spitzer.h"
 
 namespace MRM_NS {
 
@@ -182,7 +183,10 @@ __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1
 // ln(x) - x_i = ln(x * exp(-x_i)) (a log of a ratio near one, accurate to
 // FP32 rounding), with weights that sum to one; the interpolated ln p is
 // split into the node value (FP64 product, hi + lo) and the FP32 increment.
-__device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, float m1, float m2, double e) {
+// WANT_T: the temperature is formed too and returned in *temp_out
+template <bool WANT_T = false>
+__device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, float m1, float m2, double e,
+                                         double* temp_out = nullptr) {
   const double eos_gamma = 1.65;
   const double c_pres = (eos_gamma - 1.);
   const double c_temp = (eos_gamma - 1.) * 0.62 / 8.314e7;
@@ -219,7 +223,7 @@ __device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, flo
   double temp;
   const bool lowr = (i_d == 0) && (tr < 0.0f);              // ln(rho) < xlr[0]
   const bool hot = eps > 0.8 * E.eps_max;
-  if (!lowr && !hot && tt > -80.0f && tt < 80.0f) {
+  if (!WANT_T && !lowr && !hot && tt > -80.0f && tt < 80.0f) {
     temp = 1.0;
   } else {
     temp = (double)expf(tt);
@@ -236,7 +240,11 @@ __device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, flo
   int bad = 0;
   if ((pres != pres) || (pres <= 0.) || (pres == __builtin_inf())) bad = 1;
   if ((temp != temp) || (temp <= 0.) || (temp == __builtin_inf())) bad = 1;
-  if (bad) pres = c_pres * ei;
+  if (bad) {
+    pres = c_pres * ei;
+    temp = c_temp * eps;
+  }
+  if (WANT_T) *temp_out = temp;
   return (float)pres;
 }
 
@@ -275,7 +283,9 @@ __device__ __forceinline__ void mx_tcheck(float u[MRM_NV], double& e, float rho_
 
 // ------------------------------------------------------------------ stage kernel
 // ETA: resistive terms in the sweeps and eta |J|^2 in the energy residual
-template <bool ETA = false>
+// ERES: A.eres holds a further energy residual per cell (conduction), added
+// after the sweeps
+template <bool ETA = false, bool ERES = false>
 __global__ void __launch_bounds__(ST_SX* ST_SY, 2)
 k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0, MxState Sout,
            float* __restrict__ Pout, const double* __restrict__ Qtot) {
@@ -385,6 +395,7 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
 #pragma unroll
       for (int q = 0; q < 3; q++) accL[q] += lor[q];
       if (ETA) acc[4] += A.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+      if (ERES) acc[4] += (float)ld32(A.eres, col + (unsigned)k * s2u);
 
       // sources, Boris correction (Add_Sources_Integrate_SR.C:331-392), FP32
       const unsigned node = col + (unsigned)k * s2u;
@@ -1124,6 +1135,29 @@ __global__ void __launch_bounds__(DIVB_TX * DIVB_TY) k_divb_update_mx(DivbArgs A
       __syncthreads();
       valf_flush(va, lds, X.valf);
     }
+  }
+}
+
+// ------------------------------------------------------------------ Spitzer conduction
+// The conduction kernels of the fused step on the mixed state: the
+// temperature from the FP32 lookup, the conduction terms in FP64 from the
+// widened values (sflx, its update and the energy term stay FP64).
+struct SptStateMx {
+  const float* f;
+  const double* e;
+  long bs;
+  __device__ __forceinline__ double rho(long n) const { return (double)f[n]; }
+  __device__ __forceinline__ double mom(int q, long n) const { return (double)f[(1 + q) * bs + n]; }
+  __device__ __forceinline__ double ene(long n) const { return e[n]; }
+  __device__ __forceinline__ double mag(int q, long n) const { return (double)f[(4 + q) * bs + n]; }
+};
+
+__global__ void __launch_bounds__(MRM_BLOCK) k_eos_temp_mx(Geom G, EosDev E, MxState S, double* __restrict__ T) {
+  const long bs = G.bs;
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    double t;
+    mx_pres<true>(E, S.f[n], S.f[bs + n], S.f[2 * bs + n], S.f[3 * bs + n], S.e[n], &t);
+    T[n] = t;
   }
 }
 

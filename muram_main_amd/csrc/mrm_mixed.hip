@@ -10,7 +10,10 @@ static inline unsigned nblk_(long n) { return (unsigned)((n + MRM_BLOCK - 1) / M
 
 hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
                         MxState out, float* Pout, const double* Q) {
-  if (A.eta > 0.0f) k_stage_mx<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  const bool eta = A.eta > 0.0f, eres = A.eres != nullptr;
+  if (eta && eres) k_stage_mx<true, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  else if (eta) k_stage_mx<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  else if (eres) k_stage_mx<false, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
   else k_stage_mx<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
   return hipGetLastError();
 }
@@ -100,6 +103,21 @@ hipError_t launch_divb_update(hipStream_t st, const DivbArgs& A, MxState S, cons
   const unsigned nb = divb_blocks(A.G);
   if (epi) k_divb_update_mx<true><<<nb, DIVB_TX * DIVB_TY, 0, st>>>(A, S, phi, divB, *epi);
   else k_divb_update_mx<false><<<nb, DIVB_TX * DIVB_TY, 0, st>>>(A, S, phi, divB, MxEpilogue{});
+  return hipGetLastError();
+}
+
+hipError_t launch_eos_temp(hipStream_t st, const Geom& G, const EosDev& E, MxState S, double* T) {
+  unsigned nb = nblk_(G.bs);
+  if (nb > 65536u) nb = 65536u;
+  k_eos_temp_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, S, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_spitzer_stage(hipStream_t st, const Geom& G, const SptParams& P, double wdt, MxState S,
+                                const double* T, const double* sflx_in, const double* sflx0, double* sflx_out,
+                                double* eres) {
+  k_spitzer_stage<<<spitzer_blocks(G), SPT_TX * SPT_TY, 0, st>>>(G, P, wdt, SptStateMx{S.f, S.e, G.bs}, T, sflx_in,
+                                                                 sflx0, sflx_out, eres);
   return hipGetLastError();
 }
 

@@ -30,4 +30,10 @@ hipError_t launch_divb(hipStream_t st, const DivbArgs& A, MxState S, double* div
 hipError_t launch_divb_update(hipStream_t st, const DivbArgs& A, MxState S, const double* phi, const double* divB,
                               const MxEpilogue* epi);
 hipError_t launch_exchange_b(hipStream_t st, const Geom& G, MxState S);
+// Spitzer conduction of one stage on the mixed state (the fused step's
+// k_eos_temp / k_spitzer_stage)
+hipError_t launch_eos_temp(hipStream_t st, const Geom& G, const EosDev& E, MxState S, double* T);
+hipError_t launch_spitzer_stage(hipStream_t st, const Geom& G, const SptParams& P, double wdt, MxState S,
+                                const double* T, const double* sflx_in, const double* sflx0, double* sflx_out,
+                                double* eres);
 }  // namespace mixed

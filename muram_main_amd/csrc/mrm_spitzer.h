@@ -13,13 +13,6 @@
 
 namespace MRM_NS {
 
-struct SptParams {
-  double w1[3], w2[3];
-  double k0;          // param_spitzer
-  double sat_flx;     // (1 + X_H) Rgas^1.5 sqrt(1836) / 8 (mhdres_SR.C:46)
-  double hyp_diff, cmax_hyp, nu_hyp_max;   // from the call's dt and CFL (mhdres_SR.C:136-145)
-};
-
 // one interior cell per thread; U primitive (after ConsToPrim), T its
 // temperature over the whole buffer.  With t2 (diagnostics), tvar2 += the
 // conductive energy flux divergence (mhdres_SR.C:368-376, 689-694)
@@ -139,14 +132,24 @@ inline unsigned spitzer_blocks(const Geom& G) {
                     ((nz + SPT_KC - 1) / SPT_KC));
 }
 
-__global__ void __launch_bounds__(SPT_TX * SPT_TY) k_spitzer_stage(Geom G, SptParams S, double wdt,
-                                                                   const double* __restrict__ U,
+// the conservative state as the kernel reads it: FP64 SoA U
+struct SptStateF64 {
+  const double* U;
+  long bs;
+  __device__ __forceinline__ double rho(long n) const { return U[n]; }
+  __device__ __forceinline__ double mom(int q, long n) const { return U[(1 + q) * bs + n]; }
+  __device__ __forceinline__ double ene(long n) const { return U[4 * bs + n]; }
+  __device__ __forceinline__ double mag(int q, long n) const { return U[(5 + q) * bs + n]; }
+};
+
+template <class ST>
+__global__ void __launch_bounds__(SPT_TX * SPT_TY) k_spitzer_stage(Geom G, SptParams S, double wdt, ST U,
                                                                    const double* __restrict__ T,
                                                                    const double* __restrict__ sflx,
                                                                    const double* __restrict__ sflx0,
                                                                    double* __restrict__ sflx_out,
                                                                    double* __restrict__ eres) {
-  const long bs = G.bs, s1 = G.s1, s2 = G.s2;
+  const long s1 = G.s1, s2 = G.s2;
   const int nx = G.le0 - G.lb0 + 1, ny = G.le1 - G.lb1 + 1, nz = G.le2 - G.lb2 + 1;
   const int ntx = (nx + SPT_TX - 1) / SPT_TX, nty = (ny + SPT_TY - 1) / SPT_TY;
   const int b = blockIdx.x;
@@ -156,15 +159,15 @@ __global__ void __launch_bounds__(SPT_TX * SPT_TY) k_spitzer_stage(Geom G, SptPa
   if (i > G.le0 || j > G.le1) return;
   const int k0 = G.lb2 + bz * SPT_KC, k1 = min(k0 + SPT_KC, G.lb2 + nz);
   const long c0 = i + (long)j * s1;
-  const double* B0 = U + 5 * bs;
-  const double* B1 = U + 6 * bs;
-  const double* B2 = U + 7 * bs;
+  auto B0 = [&](long m) { return U.mag(0, m); };
+  auto B1 = [&](long m) { return U.mag(1, m); };
+  auto B2 = [&](long m) { return U.mag(2, m); };
   double fw[5], bw[5], tw[5];   // sflx, B_z, T at planes k-2 .. k+2
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const long m = c0 + (long)(k0 - 2 + q) * s2;
     fw[q + 1] = sflx[m];
-    bw[q + 1] = B2[m];
+    bw[q + 1] = B2(m);
     tw[q + 1] = T[m];
   }
   for (int k = k0; k < k1; k++) {
@@ -176,23 +179,23 @@ __global__ void __launch_bounds__(SPT_TX * SPT_TY) k_spitzer_stage(Geom G, SptPa
       tw[q] = tw[q + 1];
     }
     fw[4] = sflx[n + 2 * s2];
-    bw[4] = B2[n + 2 * s2];
+    bw[4] = B2(n + 2 * s2);
     tw[4] = T[n + 2 * s2];
     const double fc = fw[2];
     double re = 0.0, rf = 0.0, bgt = 0.0;
     {   // dim 0
       const double w1 = S.w1[0], w2 = S.w2[0];
-      re -= w1 * (sflx[n + 1] * B0[n + 1] - sflx[n - 1] * B0[n - 1]) +
-            w2 * (sflx[n + 2] * B0[n + 2] - sflx[n - 2] * B0[n - 2]);
+      re -= w1 * (sflx[n + 1] * B0(n + 1) - sflx[n - 1] * B0(n - 1)) +
+            w2 * (sflx[n + 2] * B0(n + 2) - sflx[n - 2] * B0(n - 2));
       rf -= S.hyp_diff * ((sflx[n + 2] + sflx[n - 2]) - 4. * (sflx[n + 1] + sflx[n - 1]) + 6. * fc);
-      bgt += B0[n] * (w1 * (T[n + 1] - T[n - 1]) + w2 * (T[n + 2] - T[n - 2]));
+      bgt += B0(n) * (w1 * (T[n + 1] - T[n - 1]) + w2 * (T[n + 2] - T[n - 2]));
     }
     {   // dim 1
       const double w1 = S.w1[1], w2 = S.w2[1];
-      re -= w1 * (sflx[n + s1] * B1[n + s1] - sflx[n - s1] * B1[n - s1]) +
-            w2 * (sflx[n + 2 * s1] * B1[n + 2 * s1] - sflx[n - 2 * s1] * B1[n - 2 * s1]);
+      re -= w1 * (sflx[n + s1] * B1(n + s1) - sflx[n - s1] * B1(n - s1)) +
+            w2 * (sflx[n + 2 * s1] * B1(n + 2 * s1) - sflx[n - 2 * s1] * B1(n - 2 * s1));
       rf -= S.hyp_diff * ((sflx[n + 2 * s1] + sflx[n - 2 * s1]) - 4. * (sflx[n + s1] + sflx[n - s1]) + 6. * fc);
-      bgt += B1[n] * (w1 * (T[n + s1] - T[n - s1]) + w2 * (T[n + 2 * s1] - T[n - 2 * s1]));
+      bgt += B1(n) * (w1 * (T[n + s1] - T[n - s1]) + w2 * (T[n + 2 * s1] - T[n - 2 * s1]));
     }
     {   // dim 2, from the windows
       const double w1 = S.w1[2], w2 = S.w2[2];
@@ -202,11 +205,11 @@ __global__ void __launch_bounds__(SPT_TX * SPT_TY) k_spitzer_stage(Geom G, SptPa
     }
     eres[n] = re;
     // primitives of the cell as ConsToPrim forms them (cons_to_prim.C:110-118)
-    const double d = U[n];
+    const double d = U.rho(n);
     const double dn = 1.0 / d;
-    const double v0 = U[bs + n] * dn, v1 = U[2 * bs + n] * dn, v2 = U[3 * bs + n] * dn;
-    const double eint = U[4 * bs + n] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2) * d;
-    const double b0 = B0[n], b1 = B1[n], b2 = bw[2];
+    const double v0 = U.mom(0, n) * dn, v1 = U.mom(1, n) * dn, v2 = U.mom(2, n) * dn;
+    const double eint = U.ene(n) - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2) * d;
+    const double b0 = B0(n), b1 = B1(n), b2 = bw[2];
     const double vv = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
     const double bb = b0 * b0 + b1 * b1 + b2 * b2;
     const double tempi = tw[2];

@@ -211,7 +211,6 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
   if (phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
-  if (phys->params[3] > 0.0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
   if (phys->divB[0] > 0.0 && phys->divB[1] < 1.0) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
@@ -1396,8 +1395,8 @@ static int spitzer_stage(mrm_ctx* c, double dt_res, double cfl, double wdt, cons
   }
   LAUNCHCHK(c, "k_eos_temp");
   TScope ts(c, T_SPT);
-  k_spitzer_stage<<<spitzer_blocks(G), SPT_TX * SPT_TY, 0, c->stream>>>(G, S, wdt, U, T, sflx_in, c->d_spt,
-                                                                        sflx_out, c->d_spt + 3 * G.bs);
+  k_spitzer_stage<<<spitzer_blocks(G), SPT_TX * SPT_TY, 0, c->stream>>>(G, S, wdt, SptStateF64{U, G.bs}, T, sflx_in,
+                                                                        c->d_spt, sflx_out, c->d_spt + 3 * G.bs);
   LAUNCHCHK(c, "k_spitzer_stage");
   return MRM_OK;
 }
@@ -1609,8 +1608,9 @@ static int tvd_mx(mrm_ctx* c, MxState M, const float* Pr, double dt_tvd, const M
 }
 
 static int stage_mx(mrm_ctx* c, const mrm_run_desc* run, int stage, MxState in, const float* Pin, MxState s0,
-                    MxState out, float* Pout) {
+                    MxState out, float* Pout, const double* eres = nullptr) {
   MxStageArgs A;
+  A.eres = eres;
   A.G = c->G;
   A.E = c->E;
   for (int d = 0; d < 3; d++) {
@@ -1689,15 +1689,38 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
     if ((rc = read_cmax(c, &cm))) return rc;
   }
   double cfl = 0.0;
+  const double dt_prev = run->dt;   // stage 1's conduction constants (mhdres_SR.C:136-145)
   if ((rc = mrm_sync_timestep(c, run, dt_rad, eta_dt_limit(c, run, c->dxmin / cm), &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
   MxState* M = c->M;
   struct { int in; float* pin; int out; float* pout; } st[4] = {
       {i0, Pc, i1, Pa}, {i1, Pa, i2, Pc}, {i2, Pc, i1, Pa}, {i1, Pa, i2, nullptr}};
+  // Spitzer conduction as in step_fused: sflx alternates between two FP64
+  // buffers, the conduction kernels read the mixed state
+  const bool spt = c->d_spt != nullptr;
+  double* sflx[2] = {c->dF + MRM_SFLX * G.bs, c->d_spt + 2 * G.bs};
+  double* eres = spt ? c->d_spt + 3 * G.bs : nullptr;
+  if (spt) {
+    TScope ts(c, T_SPT);
+    k_spitzer_save<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(G, sflx[0], c->d_spt);
+    LAUNCHCHK(c, "k_spitzer_save");
+  }
   for (int s = 0; s < 4; s++) {
-    if ((rc = stage_mx(c, run, s + 1, M[st[s].in], st[s].pin, M[i0], M[st[s].out], st[s].pout))) return rc;
+    if (spt) {
+      const SptParams S = spitzer_params(c, s == 0 ? dt_prev : run->dt, run->CFL);
+      double* T = c->dF + MRM_TEMP * G.bs;
+      {
+        TScope ts(c, T_EOST);
+        HIPCHK(c, mixed::launch_eos_temp(c->stream, G, c->E, M[st[s].in], T));
+      }
+      TScope ts(c, T_SPT);
+      HIPCHK(c, mixed::launch_spitzer_stage(c->stream, G, S, run->dt / double(4 - s), M[st[s].in], T, sflx[s % 2],
+                                            c->d_spt, sflx[(s + 1) % 2], eres));
+    }
+    if ((rc = stage_mx(c, run, s + 1, M[st[s].in], st[s].pin, M[i0], M[st[s].out], st[s].pout, eres))) return rc;
     if ((rc = exchange_mx(c, M[st[s].out], st[s].pout))) return rc;
     if ((rc = bc_mx(c, M[st[s].out], st[s].pout, nullptr))) return rc;
+    if (spt && (rc = spitzer_ghosts(c, sflx[(s + 1) % 2]))) return rc;
   }
   HIPCHK(c, hipMemsetAsync(c->d_red, 0, (va_adj ? RED_CNT_E + 1 : 1) * sizeof(unsigned long long), c->stream));
   MxEpilogue X;

@@ -92,6 +92,14 @@ struct DiagArgs {
   double idt = 0.0, dt_fac = 0.0;
 };
 
+// Spitzer conduction (mhdres_SR.C:37-49, 136-145)
+struct SptParams {
+  double w1[3], w2[3];
+  double k0;          // param_spitzer
+  double sat_flx;     // (1 + X_H) Rgas^1.5 sqrt(1836) / 8 (mhdres_SR.C:46)
+  double hyp_diff, cmax_hyp, nu_hyp_max;   // from the call's dt and CFL (mhdres_SR.C:136-145)
+};
+
 // Clean_divB (clean_divB.C:35-369)
 struct DivbArgs {
   Geom G;
@@ -149,6 +157,7 @@ struct MxStageArgs {
   double eps_min, eps_max;
   int ntx, nty, nkc;
   int write_p;
+  const double* eres = nullptr;   // extra energy residual per cell (conduction), or null
   // resistivity (param_eta > 0): eta and the second-derivative weights
   float eta = 0.0f;
   float ww0[3] = {0, 0, 0}, ww1[3] = {0, 0, 0}, ww2[3] = {0, 0, 0};

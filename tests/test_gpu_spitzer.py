@@ -68,7 +68,23 @@ def test_spitzer_steps_match_reference(table, gold, case, path):
     assert r.dt == pytest.approx(gold[f"{case}_step5_sc"][0], rel=TOL)
 
 
-def test_spitzer_refused_where_unsupported(table):
-    box, phys = make_golden_spitzer.cases()["spt_cool"]
-    with pytest.raises(abi.MrmError):
-        MhdGrid(config.grid_desc(box.gsize, box.dx), phys, abi.PREC_FP32_MIXED)
+@pytest.mark.parametrize("case", ["spt_cool", "spt_corona"])
+def test_mixed_spitzer_steps(table, gold, case):
+    """the FP32-mixed form with conduction (FP32 temperature lookup, FP64
+    heat flux) against the reference's FP64 state and heat flux after 5 steps"""
+    print(f"case: FP32-mixed Spitzer conduction, {case}", flush=True)
+    box, phys = make_golden_spitzer.cases()[case]
+    m = MhdGrid(config.grid_desc(box.gsize, box.dx), phys, abi.PREC_FP32_MIXED)
+    m.eos_init(table)
+    r = config.run_desc(CFL=0.8)
+    m.set_state(gold[f"{case}_in"])
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
+    m.step(r, 1e30, 5)
+    inner = (slice(2, -2),) * 3
+    U = m.get_state()
+    errs = [relerr(U[v][inner], gold[f"{case}_step5_U"][v][inner]) for v in range(8)]
+    es = relerr(m.get_field(abi.SFLX)[inner], gold[f"{case}_step5_sflx"][inner])
+    print("per-field rel err", ["%.2e" % e for e in errs], "sflx", "%.2e" % es, "dt", r.dt)
+    assert max(errs) <= 1e-5 and es <= 1e-4
+    assert r.dt == pytest.approx(gold[f"{case}_step5_sc"][0], rel=1e-5)

@@ -117,19 +117,24 @@ def test_backup_restore_round_trip(tmp_path, crop, table):
         assert relerr(U2[v][inner], U[v][inner]) <= 1e-6, v
 
 
-@pytest.mark.parametrize("va_adjust,divb", [(0.0, False), (1.0, False), (1.0, True)])
-def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust, divb):
+@pytest.mark.parametrize("va_adjust,divb,spitzer", [(0.0, False, False), (1.0, False, False), (1.0, True, False),
+                                                   (1.0, True, True)])
+def test_mixed_precision_error_on_test3d(tmp_path, crop, table, va_adjust, divb, spitzer):
     print(f"case: Test_3D crop FP32-mixed vs FP64, 20 fused steps, va_adjust {va_adjust}, divB {divb},"
-          " no conduction", flush=True)
+          f" Spitzer {spitzer}", flush=True)
     d, g = restart_dir(tmp_path, crop)
-    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=divb, spitzer=False)
-    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=divb, spitzer=False)
+    m64, r64 = restored(d, g, table, va_adjust=va_adjust, divb=divb, spitzer=spitzer)
+    mx, rx = restored(d, g, table, abi.PREC_FP32_MIXED, va_adjust=va_adjust, divb=divb, spitzer=spitzer)
     m64.step(r64, config.DT_RAD_OFF, 20)
     mx.step(rx, config.DT_RAD_OFF, 20)
     errs = field_errors(mx.get_state(), m64.get_state(), interior=(slice(2, -2),) * 3)
     print("FP32-mixed vs FP64 after 20 steps (rel L2, rel Linf):")
     for k, e in errs.items():
         print(f"  {k:4s} {e['l2']:.2e} {e['linf']:.2e}")
+    if spitzer:
+        es = relerr(mx.get_field(abi.SFLX)[(slice(2, -2),) * 3], m64.get_field(abi.SFLX)[(slice(2, -2),) * 3])
+        print(f"  sflx rel Linf {es:.2e}")
+        assert es <= 1e-3
     print("dt", rx.dt, r64.dt, "limits", mx.limits(), m64.limits())
     assert max(e["l2"] for e in errs.values()) <= 1e-4
     assert rx.dt == pytest.approx(r64.dt, rel=1e-5)
