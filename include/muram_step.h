@@ -111,9 +111,11 @@ typedef struct mrm_ctx mrm_ctx;
 /* Create a context on the current HIP device (GridData::Init, grid.C:147-348).
  * precision: MRM_PREC_FP64 or MRM_PREC_FP32_MIXED.  An FP32-mixed context
  * stores rho, M, B and p in FP32 and e in FP64; it runs whole steps
- * (mrm_step with va_adjust = 0), mrm_set_state/mrm_get_state of U (converted
- * on the device), mrm_exchange_grid and mrm_set_boundary_conditions; the other
- * per-operation calls return MRM_ERR_UNSUPPORTED there. */
+ * (mrm_step, with va_adjust, Clean_divB, resistivity and Spitzer conduction
+ * as configured), mrm_set_state/mrm_get_state of U (converted on the
+ * device), mrm_set_field/mrm_get_field, mrm_exchange_grid and
+ * mrm_set_boundary_conditions; the other per-operation calls and
+ * diagnostics return MRM_ERR_UNSUPPORTED there. */
 int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, int precision,
                mrm_ctx** out);
 int mrm_destroy(mrm_ctx* ctx);
