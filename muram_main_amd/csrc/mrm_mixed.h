@@ -871,12 +871,12 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_tvd_sweep_mx(Geom G, MxTvdParams 
     base = j * G.s1 + k * G.s2; str = 1; sz = G.n0; gh = G.g0;
   } else {
     const int nx = G.le0 - G.lb0 + 1;
-    const int nb = D == 1 ? G.n2 : (G.le1 - G.lb1 + 1);
+    const int nb = D == 1 ? P.y_nk : (G.le1 - G.lb1 + 1);
     active = t < (long)nx * nb;
     int i = G.lb0 + (int)(t % nx);
     int b = (int)(t / nx);
     lineX = i;
-    if (D == 1) { base = i + b * G.s2; str = G.s1; sz = G.n1; gh = G.g1; }
+    if (D == 1) { base = i + (long)(P.y_k0 + b) * G.s2; str = G.s1; sz = G.n1; gh = G.g1; }
     else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
   }
   double cm[1] = {0.0};

@@ -29,6 +29,13 @@ hipError_t launch_exchange(hipStream_t st, const Geom& G, MxState S, float* P) {
   return hipGetLastError();
 }
 
+hipError_t launch_exchange_dim2(hipStream_t st, const Geom& G, MxState S) {
+  const int nx = G.le0 - G.lb0 + 1;
+  long n2 = (long)nx * G.n1 * G.g2;
+  k_exchange_mx<2><<<nblk_(n2), MRM_BLOCK, 0, st>>>(G, S, nullptr);
+  return hipGetLastError();
+}
+
 hipError_t launch_bc_sums(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, int mode,
                           double* part) {
   k_bc_sums_mx<<<nb, MRM_BLOCK, 0, st>>>(G, E, S, mode, part);

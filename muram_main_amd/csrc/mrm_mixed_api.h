@@ -8,6 +8,8 @@ namespace mixed {
 hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
                         MxState out, float* Pout, const double* Q);
 hipError_t launch_exchange(hipStream_t st, const Geom& G, MxState S, float* P);
+// the periodic copy of the dim-2 ghost planes only (after the dim-1 TVD sweep)
+hipError_t launch_exchange_dim2(hipStream_t st, const Geom& G, MxState S);
 hipError_t launch_bc_sums(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, MxState S, int mode,
                           double* part);
 hipError_t launch_bc(hipStream_t st, unsigned nb, const Geom& G, const EosDev& E, const BcParams& B,

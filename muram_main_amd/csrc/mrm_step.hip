@@ -1248,6 +1248,8 @@ static int tvd_params(mrm_ctx* c, double dt_tvd, TvdParams& P) {
   P.hft = c->d_prof + n0;
   P.qft = c->d_prof + 2 * n0;
   P.tau = c->tau_set ? c->dF + MRM_TAU * c->G.bs : nullptr;
+  P.y_k0 = 0;        // every plane, ghosts included (tvdlimit_SR.C:199-205)
+  P.y_nk = c->G.n2;
   return MRM_OK;
 }
 
@@ -1264,19 +1266,33 @@ static int tvd_sweep_params(mrm_ctx* c, double dt_tvd, int d, TvdParams& P) {
   return MRM_OK;
 }
 
+// With y_interior (the whole-step paths, which fill the ghosts before
+// TVDlimit) and no Tau field, the dim-1 sweep runs on the interior planes only
+// and the periodic copy fills the dim-2 ghost planes afterwards: those planes
+// hold copies of interior planes (exchange and BC made them so, and the
+// dim-0 sweep treats each line alone), so sweeping them would compute the
+// same values, and the interior lines alone fill one round of the GPU's
+// threads.  A caller-supplied Tau need not be periodic in its ghost planes,
+// and a drop-in TVDlimit call gets whatever ghosts the caller left, so both
+// sweep every plane as the reference does.
 static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi,
-                   bool contracted = false, int only_d = -1) {
+                   bool contracted = false, int only_d = -1, bool y_interior = false) {
   TvdParams P;
   int rc = tvd_params(c, dt_tvd, P);
   if (rc) return rc;
   const Geom& G = c->G;
+  const bool y_copy = y_interior && P.tau == nullptr;
+  if (y_copy) {
+    P.y_k0 = G.lb2;
+    P.y_nk = c->lsize[2];
+  }
   const double idx[3] = {1. / c->dx[0], 1. / c->dx[1], 1.0 / c->dx[2]};
   const bool tau = P.tau != nullptr;
   for (int d = 0; d < 3; d++) {
     if (only_d >= 0 && d != only_d) continue;
     P.cmax = 0.975 * c->dx[d] / dt_tvd;
     P.idxd1 = idx[d];
-    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
+    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * P.y_nk : (long)c->lsize[0] * c->lsize[1];
     unsigned nb = nblk(nl);
     TScope ts(c, T_TVD0 + d);
     hipError_t e;
@@ -1292,6 +1308,11 @@ static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const
                      : exact::launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr);
     }
     if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep", e);
+    if (d == 1 && y_copy) {
+      const long n2 = (long)c->lsize[0] * G.n1 * G.g2;
+      k_exchange_p<2><<<nblk(n2), MRM_BLOCK, 0, c->stream>>>(G, U, nullptr);
+      LAUNCHCHK(c, "k_exchange_p<2>");
+    }
   }
   return MRM_OK;
 }
@@ -1535,11 +1556,11 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   const bool divb = c->phys.divB[0] > 0.0;
   const TvdEpilogue* tvd_epi = divb ? nullptr : &X;
   if (cfl <= run->CFL_tvd) {
-    if ((rc = tvd_buf(c, Uf, Pa, run->dt, tvd_epi, c->fma))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, run->dt, tvd_epi, c->fma, -1, true))) return rc;
   } else {
-    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, nullptr, c->fma))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, nullptr, c->fma, -1, true))) return rc;
     if ((rc = exchange_buf(c, Uf, nullptr))) return rc;
-    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, tvd_epi, c->fma))) return rc;
+    if ((rc = tvd_buf(c, Uf, Pa, 0.5 * run->dt, tvd_epi, c->fma, -1, true))) return rc;
   }
   if (divb && (rc = clean_divb_dev(c, Uf, &X))) return rc;
   if ((rc = exchange_buf(c, Uf, Pc))) return rc;
@@ -1575,6 +1596,7 @@ static MxTvdParams mx_tvd_params(const TvdParams& P) {
   Q.eps_min = P.eps_min;
   Q.need_tvd_coeff = P.need_tvd_coeff; Q.rho_log = P.rho_log;
   Q.is_gbeg0 = P.is_gbeg0; Q.is_gend0 = P.is_gend0;
+  Q.y_k0 = P.y_k0; Q.y_nk = P.y_nk;
   Q.hfb = P.hfb; Q.hft = P.hft; Q.qft = P.qft; Q.tau = P.tau;
   return Q;
 }
@@ -1591,8 +1613,13 @@ static int tvd_mx(mrm_ctx* c, MxState M, const float* Pr, double dt_tvd, const M
   for (int d = 0; d < 3; d++) {
     P.cmax = 0.975 * c->dx[d] / dt_tvd;
     P.idxd1 = idx[d];
+    const bool y_copy = P.tau == nullptr;   // as in tvd_buf
+    if (y_copy) {
+      P.y_k0 = G.lb2;
+      P.y_nk = c->lsize[2];
+    }
     MxTvdParams Q = mx_tvd_params(P);
-    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * G.n2 : (long)c->lsize[0] * c->lsize[1];
+    long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * P.y_nk : (long)c->lsize[0] * c->lsize[1];
     TScope ts(c, T_TVD0 + d);
     hipError_t e;
     size_t lds = (size_t)(3 * MRM_NV + 5) * G.n0 * sizeof(float);
@@ -1603,6 +1630,7 @@ static int tvd_mx(mrm_ctx* c, MxState M, const float* Pr, double dt_tvd, const M
       e = mixed::launch_tvd_sweep(c->stream, d, tau, d == 2 ? epi : nullptr, nblk(nl), G, Q, M, Pr);
     }
     if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep (mixed)", e);
+    if (d == 1 && y_copy) HIPCHK(c, mixed::launch_exchange_dim2(c->stream, G, M));
   }
   return MRM_OK;
 }
@@ -1868,7 +1896,7 @@ static int adj_recompute(mrm_ctx* c, int n, const double* Un, AdjBufs& B, double
   for (int pass = 0; pass < passes; pass++) {
     for (int d = 0; d < 3; d++) {
       HIPCHK(c, hipMemcpyAsync(B.V[3 * pass + d], B.W, f8b, hipMemcpyDeviceToDevice, c->stream));
-      if ((rc = tvd_buf(c, B.W, B.P[3], dtv, nullptr, c->fma, d))) return rc;
+      if ((rc = tvd_buf(c, B.W, B.P[3], dtv, nullptr, c->fma, d, true))) return rc;
     }
     if (split && pass == 0 && (rc = exchange_buf(c, B.W, nullptr))) return rc;
   }

@@ -59,6 +59,7 @@ struct TvdParams {
   double tvd_h[4], tvd_cs[4], tc[4];
   double inv_va2max, vsqr_diff, cmax, idxd1, dt_tvd;
   int need_tvd_coeff, rho_log, is_gbeg0, is_gend0;
+  int y_k0, y_nk;      // planes k0 .. k0 + nk - 1 of the dim-1 sweep's lines
   const double* hfb;
   const double* hft;
   const double* qft;
@@ -172,6 +173,7 @@ struct MxTvdParams {
   float inv_va2max, vsqr_diff, cmax, idxd1, dt_tvd;
   double eps_min;
   int need_tvd_coeff, rho_log, is_gbeg0, is_gend0;
+  int y_k0, y_nk;
   const double* hfb;
   const double* hft;
   const double* qft;
