@@ -119,3 +119,53 @@ def test_fused_pressure_is_eos_of_new_state(table, fma, tol):
     p_ref = m.get_field(abi.PRES)
     assert relerr(p_fused, p_ref) <= tol
     m.set_state(U)
+
+
+def full_phys():
+    p = config.stratified_phys(va_max=1.0e8, va_adjust=1.0)
+    p.divB[:] = [1, 5, 0.2]
+    p.params[config.I_PARAM_ETA] = 1e10
+    p.params[config.I_PARAM_SPITZER] = 1e-6
+    return p
+
+
+@pytest.mark.parametrize("gsize", [(13, 7, 5), (33, 17, 37)])
+@pytest.mark.parametrize("path", ["fused", "sequence", "mixed"])
+def test_odd_sizes_every_option(table, gsize, path):
+    """grids below one tile and with ragged tiles in every kernel (stage
+    16x16, Clean_divB and conduction 64x4, 32-plane chunks), with va_adjust,
+    Clean_divB, resistivity and Spitzer conduction on together: the FP64
+    paths to 1e-12 of the oracle, the FP32-mixed form to 1e-5"""
+    print(f"case: odd sizes {gsize}, every physics option, {path}", flush=True)
+    saved = os.environ.get("MRM_STEP_PATH")
+    os.environ["MRM_STEP_PATH"] = "sequence" if path == "sequence" else "fused"
+    try:
+        tab = eos_table.load_tables(table)
+        box = initial.BoxSpec(gsize=gsize, dx=(3.2e6, 3.2e6, 4.8e6), seed=37, v_rms=2.0e5)
+        U = initial.make_state(tab, box)
+        g = config.grid_desc(box.gsize, box.dx)
+        m = MhdGrid(g, full_phys(), abi.PREC_FP32_MIXED if path == "mixed" else abi.PREC_FP64)
+    finally:
+        if saved is None:
+            os.environ.pop("MRM_STEP_PATH", None)
+        else:
+            os.environ["MRM_STEP_PATH"] = saved
+    m.eos_init(table)
+    o = Oracle(g, full_phys(), table)
+    r, ro = config.run_desc(CFL=0.8), config.run_desc(CFL=0.8)
+    o.set_state(U)
+    m.set_state(U)
+    o.exchange_grid()
+    o.set_boundary_conditions(ro, 0)
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
+    o.step(ro, 1e30, 3)
+    m.step(r, 1e30, 3)
+    inner = (slice(2, -2),) * 3
+    Ug = m.get_state()
+    errs = [relerr(Ug[v][inner], o.state(0)[v][inner]) for v in range(8)]
+    errs.append(relerr(m.get_field(abi.SFLX)[inner], o.field(abi.SFLX)[inner]))
+    print("per-field rel err (rho, M, e, B, sflx):", ["%.2e" % e for e in errs], "dt", r.dt, ro.dt)
+    tol = 1e-5 if path == "mixed" else TOL
+    assert max(errs[:8]) <= tol and errs[8] <= (1e-4 if path == "mixed" else TOL)
+    assert r.dt == pytest.approx(ro.dt, rel=tol)
