@@ -9,7 +9,7 @@
 // (mhdres_SR.C:176, :456-466), primitives are formed with ConsToPrim's
 // expressions (cons_to_prim.C:110-118).
 //
-// Tiling: a 16x16 (dim0 x dim1) column tile per block, marched along dim 2.
+// Tiling: a 32x8 (dim0 x dim1) column tile per block, marched along dim 2.
 // The plane being updated sits in LDS with a two-cell ring in dims 0 and 1;
 // dim 2 neighbours come from a five-plane register window per thread.
 #pragma once

@@ -286,10 +286,10 @@ __device__ __forceinline__ void mx_tcheck(float u[MRM_NV], double& e, float rho_
 // ERES: A.eres holds a further energy residual per cell (conduction), added
 // after the sweeps
 template <bool ETA = false, bool ERES = false>
-__global__ void __launch_bounds__(ST_SX* ST_SY, 2)
+__global__ void __launch_bounds__(MX_SX* MX_SY, 2)
 k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0, MxState Sout,
            float* __restrict__ Pout, const double* __restrict__ Qtot) {
-  __shared__ float lds[ST_NF][ST_LY][ST_LX];
+  __shared__ float lds[ST_NF][MX_LY][MX_LX];
   const Geom& G = A.G;
   const long bs = G.bs;
   const long nblocks = (long)A.ntx * A.nty * A.nkc;
@@ -298,19 +298,19 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
   lb /= A.ntx;
   const int ty_t = (int)(lb % A.nty);
   const int kc = (int)(lb / A.nty);
-  const int tx = threadIdx.x % ST_SX, ty = threadIdx.x / ST_SX;
-  const int i0 = G.lb0 + tx_t * ST_SX, j0 = G.lb1 + ty_t * ST_SY;
+  const int tx = threadIdx.x % MX_SX, ty = threadIdx.x / MX_SX;
+  const int i0 = G.lb0 + tx_t * MX_SX, j0 = G.lb1 + ty_t * MX_SY;
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = (i <= G.le0) && (j <= G.le1);
-  const int kbeg = G.lb2 + kc * ST_KC;
-  const int kend = min(kbeg + ST_KC, G.le2 + 1);
+  const int kbeg = G.lb2 + kc * MX_KC;
+  const int kend = min(kbeg + MX_KC, G.le2 + 1);
   int hx_c = -1, hx_i = 0;
   if (tx < 2) { hx_c = tx; hx_i = i0 - 2 + tx; }
-  else if (tx >= ST_SX - 2) { hx_c = tx + 4; hx_i = i0 + tx + 2; }
+  else if (tx >= MX_SX - 2) { hx_c = tx + 4; hx_i = i0 + tx + 2; }
   int hy_r = -1, hy_j = 0;
   if (ty < 4) {
-    hy_r = ty < 2 ? ty : ST_SY + ty;
-    hy_j = ty < 2 ? j0 - 2 + ty : j0 + ST_SY + (ty - 2);
+    hy_r = ty < 2 ? ty : MX_SY + ty;
+    hy_j = ty < 2 ? j0 - 2 + ty : j0 + MX_SY + (ty - 2);
   }
   const bool hx_ok = hx_c >= 0 && hx_i < G.n0 && j < G.n1;
   const bool hy_ok = hy_r >= 0 && hy_j < G.n1 && i < G.n0;

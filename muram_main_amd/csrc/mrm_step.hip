@@ -1654,9 +1654,9 @@ static int stage_mx(mrm_ctx* c, const mrm_run_desc* run, int stage, MxState in, 
   A.vmax = (float)SP.vmax;
   A.eps_min = SP.eps_min;
   A.eps_max = SP.eps_max;
-  A.ntx = (c->lsize[0] + ST_SX - 1) / ST_SX;
-  A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
-  A.nkc = (c->lsize[2] + ST_KC - 1) / ST_KC;
+  A.ntx = (c->lsize[0] + MX_SX - 1) / MX_SX;
+  A.nty = (c->lsize[1] + MX_SY - 1) / MX_SY;
+  A.nkc = (c->lsize[2] + MX_KC - 1) / MX_KC;
   A.write_p = Pout != nullptr;
   {
     ResidParams RP;

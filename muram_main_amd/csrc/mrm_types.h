@@ -15,9 +15,22 @@
 #define RED_SCRATCH 15 // reductions nobody reads
 #define RED_NSLOT 16
 #define MRM_BLOCK 256
-#define ST_SX 16
-#define ST_SY 16
-#define ST_KC 32
+#ifndef ST_SX
+#define ST_SX 32           // tile of the FP64 stage kernels: dim 0 x dim 1, 256 threads
+#define ST_SY 8
+#endif
+#ifndef MX_SX
+#define MX_SX ST_SX        // tile of the FP32-mixed stage kernel
+#define MX_SY ST_SY
+#endif
+#define MX_LX (MX_SX + 4)
+#define MX_LY (MX_SY + 4)
+#ifndef ST_KC
+#define ST_KC 64           // dim-2 planes per block of the FP64 stage kernels
+#endif
+#ifndef MX_KC
+#define MX_KC 32           // dim-2 planes per block of the FP32-mixed stage kernel
+#endif
 #define ST_LX (ST_SX + 4)
 #define ST_LY (ST_SY + 4)
 #define ST_NF 9            // rho, v0, v1, v2, e_int, B0, B1, B2, p

@@ -133,7 +133,8 @@ def full_phys():
 @pytest.mark.parametrize("path", ["fused", "sequence", "mixed"])
 def test_odd_sizes_every_option(table, gsize, path):
     """grids below one tile and with ragged tiles in every kernel (stage
-    16x16, Clean_divB and conduction 64x4, 32-plane chunks), with va_adjust,
+    32x8 with 64-plane FP64 and 32-plane mixed chunks, Clean_divB and
+    conduction 64x4), with va_adjust,
     Clean_divB, resistivity and Spitzer conduction on together: the FP64
     paths to 1e-12 of the oracle, the FP32-mixed form to 1e-5"""
     print(f"case: odd sizes {gsize}, every physics option, {path}", flush=True)
