@@ -516,7 +516,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_cmax_visited(Geom G, const double
 // MAXT: the block size bound (lines of up to MAXT points); a 512 bound
 // removes the register spill but measured slower on box512 lines
 template <int MAXT>
-__global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
+__global__ void __launch_bounds__(MAXT, TX_MINW) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
                                                         const double* __restrict__ Pr) {
   extern __shared__ double sm[];
   const Geom& G = A.G;
@@ -537,8 +537,8 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
   double* sq = slr + sz;               // resistive interface term
   const int t = threadIdx.x;
   TvdPt me;
-  double u[MRM_NV];
   if (t < sz) {
+    double u[MRM_NV];
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + base + t];
     double tau = P.tau ? P.tau[base + t] : 0.0;
@@ -593,9 +593,14 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
   }
   __syncthreads();
   if (t >= gh && t <= sz - gh - 1) {
-    double Fm[MRM_NV];
+    // the conserved values are read again (an L2 hit) rather than kept in
+    // registers through the three barriers
+    double Fm[MRM_NV], u[MRM_NV];
 #pragma unroll
-    for (int q = 0; q < MRM_NV; q++) Fm[q] = sF[q * sz + t - 1];
+    for (int q = 0; q < MRM_NV; q++) {
+      Fm[q] = sF[q * sz + t - 1];
+      u[q] = U[q * bs + base + t];
+    }
     tvd_update(P, me, Fm, F, sq[t - 1], sq[t], P.qft[t], u);
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) U[q * bs + base + t] = u[q];

@@ -183,45 +183,63 @@ __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1
 // ln(x) - x_i = ln(x * exp(-x_i)) (a log of a ratio near one, accurate to
 // FP32 rounding), with weights that sum to one; the interpolated ln p is
 // split into the node value (FP64 product, hi + lo) and the FP32 increment.
-// WANT_T: the temperature is formed too and returned in *temp_out
-template <bool WANT_T = false>
-__device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, float m1, float m2, double e,
-                                         double* temp_out = nullptr) {
-  const double eos_gamma = 1.65;
-  const double c_pres = (eos_gamma - 1.);
-  const double c_temp = (eos_gamma - 1.) * 0.62 / 8.314e7;
-  double dd = (double)d;
-  double mm = (double)m0 * (double)m0 + (double)m1 * (double)m1 + (double)m2 * (double)m2;
+// The lookup is split in two so that a caller can issue the memory reads of
+// one cell (mx_eos_issue: the node indices and every table value) and do
+// other work before the arithmetic that consumes them (mx_eos_finish).
+struct MxEosG {
+  double dd, ei, eps, rxe, rxr;
+  int i_d;
+  float p00, p10, p01, p11, t00, t10, t01, t11;   // ln p and ln T at the cell's four nodes
+};
+
+__device__ __forceinline__ void mx_eos_issue(const EosDev& E, float d, float m0, float m1, float m2, double e,
+                                             MxEosG& g) {
+  const double dd = (double)d;
+  const double mm = (double)m0 * (double)m0 + (double)m1 * (double)m1 + (double)m2 * (double)m2;
   const double rd = drcp(dd);
-  double ei = e - 0.5 * mm * rd;
-  double eps = ei * rd + E.eps_off;
-  float lr = logf(d);
-  float ep = logf((float)eps);
+  g.dd = dd;
+  g.ei = e - 0.5 * mm * rd;
+  g.eps = g.ei * rd + E.eps_off;
+  const float lr = logf(d);
+  const float ep = logf((float)g.eps);
   int i_d = cvt_int((double)(lr - (float)E.xlr[0]) * E.inv_del_lr);
   i_d = imax_(0, i_d);
   i_d = imin_(E.N_lr - 2, i_d);
   int i_e = cvt_int((double)(ep - (float)E.xeps[0]) * E.inv_del_eps);
   i_e = imax_(0, i_e);
   i_e = imin_(E.N_eps - 2, i_e);
-  const float te = logf((float)(eps * E.rxeps[i_e]));   // ln(eps) - xeps[i_e]
-  const float tr = logf((float)(dd * E.rxlr[i_d]));     // ln(rho) - xlr[i_d]
+  g.i_d = i_d;
+  g.rxe = E.rxeps[i_e];
+  g.rxr = E.rxlr[i_d];
+  const long L = E.N_lr, NT = (long)E.N_eps * E.N_lr;
+  const long o00 = i_e * L + i_d, o01 = o00 + 1, o10 = o00 + L, o11 = o10 + 1;
+  g.p00 = E.tab[o00]; g.p10 = E.tab[o10]; g.p01 = E.tab[o01]; g.p11 = E.tab[o11];
+  g.t00 = E.tab[NT + o00]; g.t10 = E.tab[NT + o10]; g.t01 = E.tab[NT + o01]; g.t11 = E.tab[NT + o11];
+}
+
+// WANT_T: the temperature is formed too and returned in *temp_out
+template <bool WANT_T = false>
+__device__ __forceinline__ float mx_eos_finish(const EosDev& E, const MxEosG& g, double* temp_out = nullptr) {
+  const double eos_gamma = 1.65;
+  const double c_pres = (eos_gamma - 1.);
+  const double c_temp = (eos_gamma - 1.) * 0.62 / 8.314e7;
+  const double ei = g.ei, eps = g.eps;
+  const float te = logf((float)(eps * g.rxe));     // ln(eps) - xeps[i_e]
+  const float tr = logf((float)(g.dd * g.rxr));    // ln(rho) - xlr[i_d]
   const float fe = te * (float)E.inv_del_eps;
   const float fr = tr * (float)E.inv_del_lr;
   const double area = 1.0 / (E.inv_del_eps * E.inv_del_lr);
-  const long L = E.N_lr, NT = (long)E.N_eps * E.N_lr;
-  const long o11 = (i_e + 1) * L + i_d + 1, o10 = (i_e + 1) * L + i_d, o01 = i_e * L + i_d + 1, o00 = i_e * L + i_d;
   // T(fe, fr) = T00 + fe (T10 - T00) + fr (T01 - T00) + fe fr (T11 - T10 - T01 + T00)
-  auto incr = [&](const float* T) {
-    float t00 = T[o00], t10 = T[o10], t01 = T[o01], t11 = T[o11];
+  auto incr = [&](float t00, float t10, float t01, float t11) {
     float de = t10 - t00, dr = t01 - t00, dx = (t11 - t10) - (t01 - t00);
     return (fe * de + fr * dr + fe * fr * dx) * (float)area;
   };
-  const double pb = (double)E.tab[o00] * area;
+  const double pb = (double)g.p00 * area;
   const float pb_hi = (float)pb, pb_lo = (float)(pb - (double)pb_hi);
-  double pres = (double)expf(pb_hi) * (double)expf(pb_lo + incr(E.tab));
-  const float tt = (float)((double)E.tab[NT + o00] * area) + incr(E.tab + NT);
+  double pres = (double)expf(pb_hi) * (double)expf(pb_lo + incr(g.p00, g.p10, g.p01, g.p11));
+  const float tt = (float)((double)g.t00 * area) + incr(g.t00, g.t10, g.t01, g.t11);
   double temp;
-  const bool lowr = (i_d == 0) && (tr < 0.0f);              // ln(rho) < xlr[0]
+  const bool lowr = (g.i_d == 0) && (tr < 0.0f);            // ln(rho) < xlr[0]
   const bool hot = eps > 0.8 * E.eps_max;
   if (!WANT_T && !lowr && !hot && tt > -80.0f && tt < 80.0f) {
     temp = 1.0;
@@ -246,6 +264,14 @@ __device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, flo
   }
   if (WANT_T) *temp_out = temp;
   return (float)pres;
+}
+
+template <bool WANT_T = false>
+__device__ __forceinline__ float mx_pres(const EosDev& E, float d, float m0, float m1, float m2, double e,
+                                         double* temp_out = nullptr) {
+  MxEosG g;
+  mx_eos_issue(E, d, m0, m1, m2, e, g);
+  return mx_eos_finish<WANT_T>(E, g, temp_out);
 }
 
 // characteristic speed in FP64 from the mixed point (mhdres_SR.C:310-328)
@@ -285,7 +311,11 @@ __device__ __forceinline__ void mx_tcheck(float u[MRM_NV], double& e, float rho_
 // ETA: resistive terms in the sweeps and eta |J|^2 in the energy residual
 // ERES: A.eres holds a further energy residual per cell (conduction), added
 // after the sweeps
-template <bool ETA = false, bool ERES = false>
+// WP: the pressure of the updated cells is written to Pout.  The pressure of
+// plane k is formed during plane k+1's work: its table reads are issued
+// before that plane's sweeps and consumed after them, so their latency hides
+// behind the sweeps; the chunk's last plane is finished after the loop.
+template <bool ETA = false, bool ERES = false, bool WP = true>
 __global__ void __launch_bounds__(MX_SX* MX_SY, 2)
 k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0, MxState Sout,
            float* __restrict__ Pout, const double* __restrict__ Qtot) {
@@ -330,6 +360,11 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
   }
   // the next plane's values (own column k+3, x and y halo of k+1) are loaded
   // during the current plane's work; kbeg + 2 <= lend2 + 2 = n2 - 1
+  // the cell whose pressure is pending (plane k - 1 of this column); the
+  // initial values are a harmless placeholder lookup that is never stored
+  float pd_d = 1.0f, pd_m0 = 0.0f, pd_m1 = 0.0f, pd_m2 = 0.0f;
+  double pd_e = 1.0;
+  unsigned pd_node = 0;
   MxRaw rz, rhx, rhy;
   if (own_ok) mx_load_raw(Sin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
   if (hx_ok) mx_load_raw(Sin, Pin, bs, (unsigned)hx_i + (unsigned)j * s1u + (unsigned)kbeg * s2u, rhx);
@@ -362,6 +397,8 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
     }
     __syncthreads();
     if (valid) {
+      MxEosG pg;
+      if (WP) mx_eos_issue(A.E, pd_d, pd_m0, pd_m1, pd_m2, pd_e, pg);
       float acc[MRM_NV], res[MRM_NV], accL[3] = {0.0f, 0.0f, 0.0f}, lor[3];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] = 0.0f;
@@ -450,10 +487,17 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
         st32(Sout.f + mx_slot(q) * bs, node, o[q]);
       }
       st32(Sout.e, node, oe);
-      if (A.write_p) st32(Pout, node, mx_pres(A.E, o[0], o[1], o[2], o[3], oe));
+      if (WP) {
+        const float pk = mx_eos_finish(A.E, pg);
+        if (k > kbeg) st32(Pout, pd_node, pk);
+        pd_d = o[0]; pd_m0 = o[1]; pd_m1 = o[2]; pd_m2 = o[3];
+        pd_e = oe;
+        pd_node = node;
+      }
     }
     __syncthreads();
   }
+  if (WP && valid && kend > kbeg) st32(Pout, pd_node, mx_pres(A.E, pd_d, pd_m0, pd_m1, pd_m2, pd_e));
 }
 
 // ------------------------------------------------------------------ ghost fill

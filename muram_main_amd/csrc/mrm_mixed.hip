@@ -10,11 +10,19 @@ static inline unsigned nblk_(long n) { return (unsigned)((n + MRM_BLOCK - 1) / M
 
 hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
                         MxState out, float* Pout, const double* Q) {
-  const bool eta = A.eta > 0.0f, eres = A.eres != nullptr;
-  if (eta && eres) k_stage_mx<true, true><<<nb, MX_SX * MX_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
-  else if (eta) k_stage_mx<true><<<nb, MX_SX * MX_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
-  else if (eres) k_stage_mx<false, true><<<nb, MX_SX * MX_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
-  else k_stage_mx<false><<<nb, MX_SX * MX_SY, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  const bool eta = A.eta > 0.0f, eres = A.eres != nullptr, wp = Pout != nullptr;
+  const unsigned nt = MX_SX * MX_SY;
+  if (wp) {
+    if (eta && eres) k_stage_mx<true, true, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    else if (eta) k_stage_mx<true, false, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    else if (eres) k_stage_mx<false, true, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    else k_stage_mx<false, false, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  } else {
+    if (eta && eres) k_stage_mx<true, true, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    else if (eta) k_stage_mx<true, false, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    else if (eres) k_stage_mx<false, true, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    else k_stage_mx<false, false, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+  }
   return hipGetLastError();
 }
 
