@@ -100,14 +100,23 @@ MRM_HD double eps3_interp(const EosDev& E, double pp, double ss) {
 // p and T are produced (T decides the validity fallback of p); NOUT = 5
 // produces all five.
 // WANT_T with NOUT = 2: the temperature is always formed (out[1])
-template <int NOUT, bool WANT_T = false>
-MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
-                                                  double& m3, double& e, double out[5]) {
-  const double eos_gamma = 1.65, eos_mu = 0.62, eos_mu_n = 1.3;
-  const double mh = 1.67262158e-24, kb = 1.380622e-16;
-  const double Rgas = 8.314e7;
-  const double c_temp = (eos_gamma - 1.) * eos_mu / Rgas;
-  const double c_pres = (eos_gamma - 1.);
+//
+// The cell's lookup comes in two halves, so that a kernel can issue the
+// memory reads of one cell (eos_issue: velocities, e_int, the node indices,
+// the node coordinates and the table values) and do other work before the
+// arithmetic that consumes them (eos_finish); cons_to_prim_cell is the two
+// in a row.
+template <int NOUT>
+struct EosLook {
+  double d, e, eps, lgr, ep;
+  double xe0, xe1, xr0, xr1;      // xeps[i_e], xeps[i_e + 1], xlr[i_d], xlr[i_d + 1]
+  float t11[NOUT], t10[NOUT], t01[NOUT], t00[NOUT];
+};
+
+// m1..m3 become the velocities and e the internal energy, as in ConsToPrim
+template <int NOUT>
+MRM_HD void eos_issue(const EosDev& E, double d, double& m1, double& m2, double& m3, double& e,
+                      EosLook<NOUT>& g) {
   double dn = 1.0 / d;
   m1 = m1 * dn;
   m2 = m2 * dn;
@@ -124,24 +133,50 @@ MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
   int i_e = cvt_int((ep - E.xeps[0]) * E.inv_del_eps);
   i_e = imax_(0, i_e);
   i_e = imin_(E.N_eps - 2, i_e);
-  double coeff0 = (ep - E.xeps[i_e]) * (lr - E.xlr[i_d]);
-  double coeff1 = (ep - E.xeps[i_e]) * (E.xlr[i_d + 1] - lr);
-  double coeff2 = (E.xeps[i_e + 1] - ep) * (lr - E.xlr[i_d]);
-  double coeff3 = (E.xeps[i_e + 1] - ep) * (E.xlr[i_d + 1] - lr);
+  g.d = d;
+  g.e = e;
+  g.eps = eps;
+  g.lgr = lgr;
+  g.ep = ep;
+  g.xe0 = E.xeps[i_e];
+  g.xe1 = E.xeps[i_e + 1];
+  g.xr0 = E.xlr[i_d];
+  g.xr1 = E.xlr[i_d + 1];
   const long L = E.N_lr;
   const long NT = (long)E.N_eps * E.N_lr;
   const long o11 = (i_e + 1) * L + i_d + 1, o10 = (i_e + 1) * L + i_d;
   const long o01 = i_e * L + i_d + 1, o00 = i_e * L + i_d;
   const int tabs[5] = {0, 1, 3, 4, 5};
+#pragma unroll
+  for (int q = 0; q < NOUT; q++) {
+    const float* T = E.tab + tabs[q] * NT;
+    g.t11[q] = T[o11];
+    g.t10[q] = T[o10];
+    g.t01[q] = T[o01];
+    g.t00[q] = T[o00];
+  }
+}
+
+template <int NOUT, bool WANT_T = false>
+MRM_HD void eos_finish(const EosDev& E, const EosLook<NOUT>& g, double out[5]) {
+  const double eos_gamma = 1.65, eos_mu = 0.62, eos_mu_n = 1.3;
+  const double mh = 1.67262158e-24, kb = 1.380622e-16;
+  const double Rgas = 8.314e7;
+  const double c_temp = (eos_gamma - 1.) * eos_mu / Rgas;
+  const double c_pres = (eos_gamma - 1.);
+  const double d = g.d, e = g.e, eps = g.eps, lgr = g.lgr, ep = g.ep, lr = g.lgr;
+  double coeff0 = (ep - g.xe0) * (lr - g.xr0);
+  double coeff1 = (ep - g.xe0) * (g.xr1 - lr);
+  double coeff2 = (g.xe1 - ep) * (lr - g.xr0);
+  double coeff3 = (g.xe1 - ep) * (g.xr1 - lr);
   double r[5];
   double tt = 0.0;
 #pragma unroll
   for (int q = 0; q < NOUT; q++) {
-    const float* T = E.tab + tabs[q] * NT;
-    double v = coeff0 * (double)T[o11];
-    v += coeff1 * (double)T[o10];
-    v += coeff2 * (double)T[o01];
-    v += coeff3 * (double)T[o00];
+    double v = coeff0 * (double)g.t11[q];
+    v += coeff1 * (double)g.t10[q];
+    v += coeff2 * (double)g.t01[q];
+    v += coeff3 * (double)g.t00[q];
     if (NOUT == 2 && q == 1) {
       tt = v;
       r[q] = 0.0;
@@ -190,6 +225,14 @@ MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2,
     out[3] = r[3];
     out[4] = r[4];
   }
+}
+
+template <int NOUT, bool WANT_T = false>
+MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2, double& m3, double& e,
+                              double out[5]) {
+  EosLook<NOUT> g;
+  eos_issue<NOUT>(E, d, m1, m2, m3, e, g);
+  eos_finish<NOUT, WANT_T>(E, g, out);
 }
 
 // ------------------------------------------------------------------ residual

@@ -75,6 +75,10 @@ __device__ __forceinline__ long xcd_remap(long b, long nb) {
 // ETA: resistive terms (param_eta > 0) in the three sweeps and eta |J|^2 in
 // the energy residual (mhdres_SR.C:422-462).  ERES: A.eres holds a further
 // energy residual per cell (Spitzer conduction's), added after the sweeps.
+// With A.write_p the pressure of plane k is formed during plane k+1's work:
+// its EOS reads are issued before that plane's sweeps and consumed after
+// them, so their latency hides behind the sweeps; the chunk's last plane is
+// finished after the loop.
 template <bool ZWIN, bool ETA = false, bool ERES = false>
 __global__ void __launch_bounds__(ST_SX* ST_SY, ZWIN ? 1 : 2)
 k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ Pin, const double* __restrict__ U0,
@@ -110,6 +114,10 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
   const unsigned col = (unsigned)i + (unsigned)j * (unsigned)G.s1;
   const unsigned s1u = (unsigned)G.s1, s2u = (unsigned)G.s2;
 
+  // the cell whose pressure is pending (plane k - 1 of this column); the
+  // initial values are a harmless placeholder lookup that is never stored
+  double pd_u[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+  unsigned pd_node = 0;
   double w[5][ST_NF];   // planes k-2 .. k+2 of this thread's column (w[2] is the centre)
   // with the register window, the next plane's raw values (own column k+3,
   // the x and y halo of k+1) are loaded during the current plane's work
@@ -168,6 +176,8 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
     }
     __syncthreads();
     if (valid) {
+      EosLook<2> pg;
+      if (A.write_p) eos_issue<2>(A.E, pd_u[0], pd_u[1], pd_u[2], pd_u[3], pd_u[4], pg);
       double acc[MRM_NV], res[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) acc[q] = 0.0;
@@ -212,12 +222,20 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) st32(Uout + q * bs, node, out[q]);
       if (A.write_p) {
-        double m1 = out[1], m2 = out[2], m3 = out[3], e = out[4], pr[5];
-        cons_to_prim_cell<2>(A.E, out[0], m1, m2, m3, e, pr);
-        st32(Pout, node, pr[0]);
+        double pr[5];
+        eos_finish<2>(A.E, pg, pr);
+        if (k > kbeg) st32(Pout, pd_node, pr[0]);
+#pragma unroll
+        for (int q = 0; q < 5; q++) pd_u[q] = out[q];
+        pd_node = node;
       }
     }
     __syncthreads();
+  }
+  if (A.write_p && valid && kend > kbeg) {
+    double pr[5];
+    cons_to_prim_cell<2>(A.E, pd_u[0], pd_u[1], pd_u[2], pd_u[3], pd_u[4], pr);
+    st32(Pout, pd_node, pr[0]);
   }
 }
 
