@@ -584,16 +584,31 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
       u[0] = S.f[n]; u[1] = S.f[bs + n]; u[2] = S.f[2 * bs + n]; u[3] = S.f[3 * bs + n];
       u[4] = S.e[n]; u[5] = S.f[4 * bs + n]; u[6] = S.f[5 * bs + n]; u[7] = S.f[6 * bs + n];
     };
-    auto st = [&](long n, const double u[MRM_NV]) {
-      float f0 = (float)u[0], f1 = (float)u[1], f2 = (float)u[2], f3 = (float)u[3];
-      S.f[n] = f0; S.f[bs + n] = f1; S.f[2 * bs + n] = f2; S.f[3 * bs + n] = f3;
-      S.e[n] = u[4];
-      S.f[4 * bs + n] = (float)u[5]; S.f[5 * bs + n] = (float)u[6]; S.f[6 * bs + n] = (float)u[7];
+    // both ghost cells' pressures are formed before any store, so that the
+    // two EOS lookups overlap (the stores could alias the tables for the
+    // compiler)
+    auto st2 = [&](long na, const double ua[MRM_NV], long nb, const double ub[MRM_NV]) {
+      float fa[MRM_NV], fb[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) { fa[q] = (float)ua[q]; fb[q] = (float)ub[q]; }
+      float pa = 0.0f, pb = 0.0f;
       if (P) {
-        float p = mx_pres(E, f0, f1, f2, f3, u[4]);
-        P[n] = p;
-        if (cmax_out && inner)
-          cm[0] = smax(cm[0], mx_cmax(f0, f1, f2, f3, (float)u[5], (float)u[6], (float)u[7], p, inv_va2max));
+        pa = mx_pres(E, fa[0], fa[1], fa[2], fa[3], ua[4]);
+        pb = mx_pres(E, fb[0], fb[1], fb[2], fb[3], ub[4]);
+      }
+      S.f[na] = fa[0]; S.f[bs + na] = fa[1]; S.f[2 * bs + na] = fa[2]; S.f[3 * bs + na] = fa[3];
+      S.e[na] = ua[4];
+      S.f[4 * bs + na] = fa[5]; S.f[5 * bs + na] = fa[6]; S.f[6 * bs + na] = fa[7];
+      S.f[nb] = fb[0]; S.f[bs + nb] = fb[1]; S.f[2 * bs + nb] = fb[2]; S.f[3 * bs + nb] = fb[3];
+      S.e[nb] = ub[4];
+      S.f[4 * bs + nb] = fb[5]; S.f[5 * bs + nb] = fb[6]; S.f[6 * bs + nb] = fb[7];
+      if (P) {
+        P[na] = pa;
+        P[nb] = pb;
+        if (cmax_out && inner) {
+          cm[0] = smax(cm[0], mx_cmax(fa[0], fa[1], fa[2], fa[3], fa[5], fa[6], fa[7], pa, inv_va2max));
+          cm[0] = smax(cm[0], mx_cmax(fb[0], fb[1], fb[2], fb[3], fb[5], fb[6], fb[7], pb, inv_va2max));
+        }
       }
     };
     if (!top) {
@@ -602,16 +617,14 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_mx(Geom G, EosDev E, BcParams 
       ld(n1, u1);
       ld(n2, u2);
       bc_bottom_cells(E, B, psum[0] * B.cc, psum[1] * B.cc, u1, u2);
-      st(G.lb0 - 1 + base, u1);
-      st(G.lb0 - 2 + base, u2);
+      st2(G.lb0 - 1 + base, u1, G.lb0 - 2 + base, u2);
     } else {
       long n1 = G.le0 + base, n2 = n1 - 1;
       double u1[MRM_NV], u2[MRM_NV];
       ld(n1, u1);
       ld(n2, u2);
       bc_top_cells(B, u1, u2);
-      st(G.le0 + 1 + base, u1);
-      st(G.le0 + 2 + base, u2);
+      st2(G.le0 + 1 + base, u1, G.le0 + 2 + base, u2);
     }
   }
   if (cmax_out) {
