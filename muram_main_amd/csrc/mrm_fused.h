@@ -246,8 +246,9 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
 // neighbours come from a five-plane register window whose next plane is
 // loaded during the current plane's work.  R += ((R_x) + R_y) + R_z per
 // interior cell in the reference's order, and the interior points' cmax goes
-// to red[RED_CMAX] (the ghost faces are k_cmax_faces').
-template <bool ETA>
+// to red[RED_CMAX] (the ghost faces are k_cmax_faces').  RZERO: the interior
+// of R is known to be zero; the sums start from 0.0 without reading it.
+template <bool ETA, bool RZERO = false>
 __global__ void __launch_bounds__(ST_SX* ST_SY, 1)
 k_resid_tile(StageArgs A, const double* __restrict__ U, const double* __restrict__ P, double* __restrict__ R,
              unsigned long long* red) {
@@ -323,7 +324,7 @@ k_resid_tile(StageArgs A, const double* __restrict__ U, const double* __restrict
       const unsigned node = col + (unsigned)k * s2u;
       double acc[MRM_NV], res[MRM_NV];
 #pragma unroll
-      for (int q = 0; q < MRM_NV; q++) acc[q] = ld32(R + q * bs, node);
+      for (int q = 0; q < MRM_NV; q++) acc[q] = RZERO ? 0.0 : ld32(R + q * bs, node);
       const double lfc = lfac_of(w[2], A.RP.inv_va2max);
       auto getx = [&](int s, double o[9]) {
 #pragma unroll

@@ -291,7 +291,10 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_cmax_faces(Geom G, double inv_va2
 
 // ---------------------------------------------------------------- sources / integrate
 // DIAG: tvar4..6 set and tvar7 added to (Add_Sources_Integrate_SR.C:122-296)
-template <bool DIAG = false>
+// ZERO_R: Res := 0 is stored; without it the caller records that the
+// interior of Res is zero and the next residual starts from zero instead of
+// reading it
+template <bool DIAG = false, bool ZERO_R = true>
 __global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParams SP, double* __restrict__ U,
                                                                 const double* __restrict__ U0, double* __restrict__ R,
                                                                 const double* __restrict__ Qtot, DiagArgs DA) {
@@ -313,7 +316,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParam
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
     U[q * bs + node] = out[q];
-    R[q * bs + node] = 0.0;
+    if (ZERO_R) R[q * bs + node] = 0.0;
   }
   if (DIAG) {
     DA.dg[DG_T4 * bs + node] = dg[0];
@@ -321,6 +324,16 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParam
     DA.dg[DG_T6 * bs + node] = dg[2];
     DA.dg[DG_T7 * bs + node] += (out[4] - dg[3]) * DA.idt;
   }
+}
+
+// Res := 0 over the interior (the store k_source_integrate<..., false> left out)
+__global__ void __launch_bounds__(MRM_BLOCK) k_res_zero(Geom G, double* __restrict__ R) {
+  int i, j, k;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (!interior_ijk(G, t, i, j, k)) return;
+  long node = i + j * G.s1 + k * G.s2;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) R[q * G.bs + node] = 0.0;
 }
 
 __global__ void __launch_bounds__(MRM_BLOCK) k_save_cons(Geom G, const double* __restrict__ U, double* __restrict__ U0) {

@@ -53,6 +53,10 @@ struct mrm_ctx {
   double* dR = nullptr;
   double* dP2 = nullptr;                         // second pressure buffer of the fused step
   bool fast_valid = false;                       // pres and cmax describe S[cur]
+  // the interior of dR is zero by Source_Integrate_Tcheck's Res := 0, which
+  // was not stored: the next residual starts from zero, every other reader
+  // stores the zeros first (res_settle)
+  bool res_zero = false;
   bool force_sequence = false;
   bool stage_zwin = false;                       // stage kernel variant
   bool fma = true;                               // fused step uses the contracted build
@@ -423,6 +427,8 @@ static double* state_ptr(mrm_ctx* c, int which) {
   return which == MRM_U ? c->dU : which == MRM_U0 ? c->dU0 : which == MRM_RES ? c->dR : nullptr;
 }
 
+static int res_settle(mrm_ctx* c);
+
 // In FP32-mixed contexts U is staged through dR (FP64) and converted on the
 // device: rho, M, B rounded to FP32, e kept in FP64.
 extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layout) {
@@ -432,6 +438,7 @@ extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layo
   double* d = mx ? c->dR : state_ptr(c, which);
   if (!d) return fail(c, MRM_ERR_ARG, "mrm_set_state: bad array id");
   c->fast_valid = false;
+  if (which == MRM_RES) c->res_zero = false;
   const size_t bs = c->G.bs;
   if (layout == MRM_LAYOUT_SOA) {
     HIPCHK(c, hipMemcpyAsync(d, host, 8 * bs * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -455,6 +462,10 @@ extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
   if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_get_state: FP32-mixed contexts hold U only");
   double* d = mx ? c->dR : state_ptr(c, which);
   if (!d) return fail(c, MRM_ERR_ARG, "mrm_get_state: bad array id");
+  if (which == MRM_RES) {
+    int rc = res_settle(c);
+    if (rc) return rc;
+  }
   const size_t bs = c->G.bs;
   if (mx) HIPCHK(c, mixed::launch_convert(c->stream, false, c->G.bs, c->dR, c->M[c->cur]));
   if (layout == MRM_LAYOUT_SOA) {
@@ -705,6 +716,15 @@ extern "C" int mrm_diag_zero(mrm_ctx* c) {
 
 static int launch_spitzer_resid(mrm_ctx* c, const mrm_run_desc* run);
 
+// store the Res := 0 that k_source_integrate<..., false> left out
+static int res_settle(mrm_ctx* c) {
+  if (!c->res_zero) return MRM_OK;
+  k_res_zero<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dR);
+  LAUNCHCHK(c, "k_res_zero");
+  c->res_zero = false;
+  return MRM_OK;
+}
+
 static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
   ResidParams RP;
   for (int d = 0; d < 3; d++) {
@@ -720,6 +740,7 @@ static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
   const double* P = c->dF + MRM_PRES * c->G.bs;
   { TScope ts(c, T_RESID);
   if (DA.dg) {
+    if ((rc = res_settle(c))) return rc;
     k_residual<true><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, RP, c->dU, P, c->dR, c->d_red, DA);
   } else {
     // the tiled kernel (plane in LDS, dim-2 register window)
@@ -730,8 +751,15 @@ static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
     A.nty = (c->lsize[1] + ST_SY - 1) / ST_SY;
     A.nkc = (c->lsize[2] + ST_KC - 1) / ST_KC;
     const unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
-    if (RP.eta > 0.0) k_resid_tile<true><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
-    else k_resid_tile<false><<<nb, ST_SX * ST_SY, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+    const unsigned nt = ST_SX * ST_SY;
+    if (c->res_zero) {
+      if (RP.eta > 0.0) k_resid_tile<true, true><<<nb, nt, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+      else k_resid_tile<false, true><<<nb, nt, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+    } else {
+      if (RP.eta > 0.0) k_resid_tile<true><<<nb, nt, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+      else k_resid_tile<false><<<nb, nt, 0, c->stream>>>(A, c->dU, P, c->dR, c->d_red);
+    }
+    c->res_zero = false;
   }
   }
   LAUNCHCHK(c, "k_residual");
@@ -855,11 +883,16 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
   DiagArgs DA;
   int rc = diag_args(c, run, DA);
   if (rc) return rc;
+  // the kernel reads Res: zeros left unstored by a previous call go in first
+  if ((rc = res_settle(c))) return rc;
   TScope ts(c, T_SRCINT);
-  if (DA.dg)
+  if (DA.dg) {
     k_source_integrate<true><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q, DA);
-  else
-    k_source_integrate<false><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR, Q, DA);
+  } else {
+    k_source_integrate<false, false><<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, SP, c->dU, c->dU0, c->dR,
+                                                                                     Q, DA);
+    c->res_zero = true;
+  }
   LAUNCHCHK(c, "k_source_integrate");
   if (c->d_spt) {
     const long bs = c->G.bs;
