@@ -88,7 +88,9 @@ def cpu_baseline(table, phys, box, steps=2):
 
 
 def timed_run(m, r, steps, warmup):
-    """warm-up steps, then `steps` timed steps with the per-kernel timers on."""
+    """warm-up steps, then `steps` timed steps with the per-kernel timers of
+    the volume kernels on (the surface kernels' timers are left off: each
+    event record idles the GPU for a few microseconds)."""
     import ctypes as C
     import torch
     from muram_main_amd import abi, config
@@ -96,7 +98,7 @@ def timed_run(m, r, steps, warmup):
         m.step(r, config.DT_RAD_OFF, 1)
     m.synchronize()
     lib = m.lib
-    lib.mrm_timers(m.h, 1)
+    lib.mrm_timers(m.h, 2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):

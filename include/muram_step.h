@@ -247,6 +247,9 @@ int mrm_adjoint_backward(mrm_ctx* ctx, const double* seed, double* grad, int fol
  * timers printed at verbose > 1 (solver.C:41-43, :428-471).  When enabled,
  * every launch is bracketed by HIP events on the context's stream.
  * mrm_timers(ctx, 1) clears and starts, mrm_timers(ctx, 0) stops;
+ * mrm_timers(ctx, 2) times only the volume kernels (not the exchange, the
+ * boundary kernels and the cmax faces): each event record idles the GPU for
+ * some microseconds, which is noticeable around the short surface kernels;
  * mrm_get_timers synchronizes and returns the accumulated milliseconds and
  * launch counts per timer id (MRM_NTIMER entries; names from mrm_timer_name). */
 #define MRM_NTIMER 24

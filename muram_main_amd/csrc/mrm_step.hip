@@ -111,7 +111,7 @@ struct mrm_ctx {
   unsigned long long* h_red = nullptr;   // pinned mirror of d_red
 
   // per-kernel timers (solver.C:428-471 counterpart)
-  bool timing = false;
+  int timing = 0;                                // 0 off, 1 every timer, 2 volume kernels only
   struct Rec { int id; hipEvent_t a, b; };
   std::vector<hipEvent_t> evpool;
   std::vector<Rec> pending;
@@ -155,7 +155,8 @@ struct TScope {
   int id;
   hipEvent_t a = nullptr;
   TScope(mrm_ctx* c_, int id_) : c(c_), id(id_) {
-    if (c->timing) { a = c->ev_get(); hipEventRecord(a, c->stream); }
+    const bool surface = id == T_EXCH || id == T_BCSUM || id == T_BCBOT || id == T_BCTOP || id == T_CMAXF;
+    if (c->timing == 1 || (c->timing == 2 && !surface)) { a = c->ev_get(); hipEventRecord(a, c->stream); }
   }
   ~TScope() {
     if (a) {
@@ -2075,7 +2076,7 @@ extern "C" int mrm_timers(mrm_ctx* c, int enable) {
   if (enable) {
     for (int i = 0; i < MRM_NTIMER; i++) { c->tms[i] = 0.0; c->tcnt[i] = 0; }
   }
-  c->timing = enable != 0;
+  c->timing = enable == 2 ? 2 : enable != 0 ? 1 : 0;
   return MRM_OK;
 }
 
