@@ -200,8 +200,11 @@ __device__ __forceinline__ void mx_eos_issue(const EosDev& E, float d, float m0,
   g.dd = dd;
   g.ei = e - 0.5 * mm * rd;
   g.eps = g.ei * rd + E.eps_off;
-  const float lr = logf(d);
-  const float ep = logf((float)g.eps);
+  // ln rho and ln eps only choose the table cell (the offsets from its node
+  // are formed accurately below), so the hardware log2 serves; a cell chosen
+  // one off at a node boundary interpolates to the same value
+  const float lr = __builtin_amdgcn_logf(d) * 0.693147180559945f;
+  const float ep = __builtin_amdgcn_logf((float)g.eps) * 0.693147180559945f;
   int i_d = cvt_int((double)(lr - (float)E.xlr[0]) * E.inv_del_lr);
   i_d = imax_(0, i_d);
   i_d = imin_(E.N_lr - 2, i_d);
