@@ -532,10 +532,12 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_cmax_visited(Geom G, const double
 // thread t owns point t of the line; the per-point, per-slope, per-interface
 // and per-update arithmetic is tvd_point / tvd_slope / tvd_iface / tvd_update
 
-// MAXT: the block size bound (lines of up to MAXT points); a 512 bound
-// removes the register spill but measured slower on box512 lines
+// MAXT: the block size bound (lines of up to MAXT points).  At 1024 the
+// kernel fits 128 VGPRs (4 waves per SIMD) without spilling; a 320- or
+// 512-thread bound lets it use more registers at fewer waves, which
+// measured slower on box512 lines
 template <int MAXT>
-__global__ void __launch_bounds__(MAXT, TX_MINW) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
+__global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
                                                         const double* __restrict__ Pr) {
   extern __shared__ double sm[];
   const Geom& G = A.G;

@@ -34,12 +34,7 @@
 #define ST_LX (ST_SX + 4)
 #define ST_LY (ST_SY + 4)
 #define ST_NF 9            // rho, v0, v1, v2, e_int, B0, B1, B2, p
-#ifndef TX_MAXN
 #define TX_MAXN 1024
-#endif
-#ifndef TX_MINW
-#define TX_MINW 1
-#endif
 
 struct Geom {
   int n0, n1, n2;          // buffer extents (gsize + 2*ghosts)
