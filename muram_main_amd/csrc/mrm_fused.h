@@ -390,6 +390,29 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_exchange_p(Geom G, double* __rest
   }
 }
 
+// whether the dim-2 ghost planes of U and P (every i: the dim-0 sweep reads
+// the dim-0 ghosts too; every j) hold exact copies of their periodic
+// images: any difference (a NaN included) sets *diff to 1
+__global__ void __launch_bounds__(MRM_BLOCK) k_ghost2_periodic(Geom G, const double* __restrict__ U,
+                                                               const double* __restrict__ P,
+                                                               unsigned long long* __restrict__ diff) {
+  const int nx = G.n0;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nx * G.n1 * G.g2) return;
+  int i = (int)(t % nx);
+  long r = t / nx;
+  int b = (int)(r % G.n1);
+  int igh = (int)(r / G.n1);
+  long base = i + b * G.s1;
+  long src_r = base + (long)(G.lb2 + igh) * G.s2, dst_r = base + (long)(G.le2 + 1 + igh) * G.s2;
+  long src_l = base + (long)(G.le2 - G.g2 + 1 + igh) * G.s2, dst_l = base + (long)(G.lb2 - G.g2 + igh) * G.s2;
+  bool same = P[dst_r] == P[src_r] && P[dst_l] == P[src_l];
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++)
+    same = same && U[q * G.bs + dst_r] == U[q * G.bs + src_r] && U[q * G.bs + dst_l] == U[q * G.bs + src_l];
+  if (!same) *diff = 1ull;
+}
+
 // pressure of a freshly written ghost cell (ConsToPrim's p, cons_to_prim.C:108-198)
 __device__ __forceinline__ double ghost_pres(const EosDev& E, const double u[MRM_NV]) {
   double m1 = u[1], m2 = u[2], m3 = u[3], e = u[4], pr[5];

@@ -1307,8 +1307,10 @@ static int tvd_sweep_params(mrm_ctx* c, double dt_tvd, int d, TvdParams& P) {
 // dim-0 sweep treats each line alone), so sweeping them would compute the
 // same values, and the interior lines alone fill one round of the GPU's
 // threads.  A caller-supplied Tau need not be periodic in its ghost planes,
-// and a drop-in TVDlimit call gets whatever ghosts the caller left, so both
-// sweep every plane as the reference does.
+// so with Tau every plane is swept as the reference does; a drop-in
+// TVDlimit call gets whatever ghosts the caller left, and takes the interior
+// sweep only after k_ghost2_periodic has found the ghost planes to be exact
+// copies (ghosts_periodic).
 static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi,
                    bool contracted = false, int only_d = -1, bool y_interior = false) {
   TvdParams P;
@@ -1351,6 +1353,21 @@ static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const
   return MRM_OK;
 }
 
+// whether U's and Pr's dim-2 ghost planes are exact copies of their periodic
+// images (one device pass over the ghost planes and a host read of the flag)
+static int ghosts_periodic(mrm_ctx* c, const double* U, const double* Pr, bool* yes) {
+  unsigned long long* flag = c->d_red + RED_SCRATCH;
+  HIPCHK(c, hipMemsetAsync(flag, 0, sizeof(unsigned long long), c->stream));
+  const long n = (long)c->G.n0 * c->G.n1 * c->G.g2;
+  k_ghost2_periodic<<<nblk(n), MRM_BLOCK, 0, c->stream>>>(c->G, U, Pr, flag);
+  LAUNCHCHK(c, "k_ghost2_periodic");
+  unsigned long long h = 1;
+  HIPCHK(c, hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *yes = h == 0;
+  return MRM_OK;
+}
+
 // the three sweeps with the diagnostics of every updated point: the marching
 // kernels in every dimension, exact build
 static int tvd_buf_diag(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const DiagArgs& DA) {
@@ -1378,7 +1395,10 @@ extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) 
   int rc = diag_args(c, run, DA, dt_tvd);
   if (rc) return rc;
   if (DA.dg) return tvd_buf_diag(c, c->dU, c->dF + MRM_PRES * c->G.bs, dt_tvd, DA);
-  return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, dt_tvd, nullptr);
+  const double* Pr = c->dF + MRM_PRES * c->G.bs;
+  bool periodic = false;
+  if (!c->tau_set && (rc = ghosts_periodic(c, c->dU, Pr, &periodic))) return rc;
+  return tvd_buf(c, c->dU, Pr, dt_tvd, nullptr, false, -1, periodic);
 }
 
 // ------------------------------------------------------------------ full step

@@ -3,6 +3,7 @@ reference's golden vectors and the oracle.  Tolerance: 1e-12 relative per
 field (max |gpu - ref| / max |ref|), the bar the FP64 plain method is held to;
 device exp/log differ from glibc's in the last ulp, nothing else may differ.
 """
+import ctypes as C
 import os
 
 import numpy as np
@@ -134,6 +135,45 @@ def test_recorded_res_zero(gold, table):
     d.Source_Integrate_Tcheck(r, 2)
     assert np.array_equal(c.get_state(), d.get_state())
     assert np.array_equal(c.get_state(abi.RES), d.get_state(abi.RES))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_tvdlimit_ghost_planes(table, periodic):
+    """The drop-in TVDlimit sweeps the interior dim-2 planes and copies the
+    ghost planes when they are exact periodic copies (checked on the device),
+    and sweeps every plane otherwise: both against the oracle, which sweeps
+    every plane, on a 12x10x14 box whose dim-2 ghost planes are, or are not,
+    the exchange's copies."""
+    print(f"case: drop-in TVDlimit, dim-2 ghost planes periodic={periodic}", flush=True)
+    tab = eos_table.load_tables(table)
+    box = initial.BoxSpec(gsize=(12, 10, 14), dx=(3.2e6,) * 3, seed=21, v_rms=2.0e5)
+    U = initial.make_state(tab, box)
+    g, p = config.grid_desc(box.gsize, box.dx), config.stratified_phys()
+    r = config.run_desc(CFL=0.8)
+    m = make(g, p, table)
+    o = Oracle(g, p, table)
+    for x in (m, o):
+        x.set_state(U)
+    m.exchange_grid()
+    o.exchange_grid()
+    V = m.get_state()
+    if not periodic:
+        # perturb a dim-2 ghost plane at each end (arrays are [v, k, j, i])
+        V[0, 0] *= 1.0 + 1e-3
+        V[4, -1] *= 1.0 - 1e-3
+    m.set_state(V)
+    o.set_state(V)
+    # the reference's order: ConsToPrim and Adjust_Valf_Max (pressure and
+    # limiter state), then TVDlimit on the conservative state
+    m.ConsToPrim()
+    o.cons_to_prim()
+    m.Adjust_Valf_Max(r)
+    o.adjust_valf_max(r, None)
+    m.set_state(V)
+    o.set_state(V)
+    m.TVDlimit(r, 1.0)
+    o.tvdlimit(C.byref(r), 1.0)
+    assert maxrel(m.get_state(), o.state(0)) <= TOL
 
 
 def test_cons_to_prim_fallbacks(gold, table):
