@@ -559,15 +559,20 @@ __device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
 // (va_adjust > 0); that variant is held to two waves per SIMD like the others.
 // DIAG: tvar8, tvar6, Qres, tvar7 and Qvis of every updated point
 // (tvdlimit_SR.C:577-666)
+// The conserved values of the last three points of each line wait in an LDS
+// ring ([slot][field][thread], 48 KB per block) for their update two points
+// later, instead of a second read from memory.
 template <int D, bool HAS_TAU, bool EPI, bool VALF = false, bool DIAG = false>
 __global__ void __launch_bounds__(MRM_BLOCK) __attribute__((amdgpu_waves_per_eu(VALF ? 2 : 1)))
 k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restrict__ Pr, TvdEpilogue X,
             DiagArgs DA) {
   __shared__ double lds[(VALF ? 5 : 1) * MRM_BLOCK / 64];
   __shared__ double vacc[VALF ? 5 * MRM_BLOCK : 1];
+  __shared__ double ring[3][MRM_NV][MRM_BLOCK];
   if (VALF) valf_lds_init(vacc + threadIdx.x, MRM_BLOCK);
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int tl = threadIdx.x;
   long base = 0, str = 1;
   int sz = 0, gh = 0, lineX = 0;
   bool active;
@@ -594,11 +599,17 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
     TvdPt W0, W1, W2, W3;      // points t-3, t-2, t-1, t
     double S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
     double qm = 0.0, qv = 0.0, qvm = 0.0;
+    int slot = 0;    // ring slot of point s; point s-2 is in slot (s+1) % 3
     for (int s = 0; s < sz; s++) {
       long node = base + s * str;
       double u[MRM_NV];
 #pragma unroll
-      for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + node];
+      for (int q = 0; q < MRM_NV; q++) {
+        u[q] = U[q * bs + node];
+        ring[slot][q][tl] = u[q];
+      }
+      const int slot2 = slot == 2 ? 0 : slot + 1;
+      slot = slot2;
       double tau = HAS_TAU ? P.tau[node] : 0.0;
       double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
       tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3);
@@ -618,7 +629,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
           long nd = base + ii * str;
           double uu[MRM_NV];
 #pragma unroll
-          for (int k = 0; k < MRM_NV; k++) uu[k] = U[k * bs + nd];
+          for (int k = 0; k < MRM_NV; k++) uu[k] = ring[slot2][k][tl];
           double dg[4];
           tvd_update<DIAG>(P, W1, Fm, F, qm, q, iq, uu, dg);
           if (DIAG) {
