@@ -35,20 +35,26 @@ MODEL_BYTES = {"fp64": 1400.0, "fp32": 788.0}   # algorithmic HBM bytes per cell
 # algorithmic bytes per unit of each kernel; unit = interior cell unless noted
 KERNEL_BYTES = {
     "cons_to_prim": (112.0, "buffer"),    # read rho,M,e 40; write M,e 32 + p,T,ne,rhoi,amb 40
-    "residual": (200.0, "interior"),      # read U 64 + p 8 + Res 64; write Res 64
-    "source_integrate": (320.0, "interior"),  # read U, Res, U0 192; write U, Res 128
+    # MHD_Residual after a Source_Integrate_Tcheck that recorded Res := 0:
+    # read U 64 + p 8, write Res 64 (the zeros are not read)
+    "residual": (136.0, "interior"),
+    # Source_Integrate_Tcheck: read U, U0, Res 192; write U 64 (Res := 0 recorded)
+    "source_integrate": (256.0, "interior"),
     "save_cons": (128.0, "interior"),
     "tcheck": (80.0, "interior"),
     "tvd_sweep_x": (136.0, "interior"),   # read U 64 + p 8; write U 64
     "tvd_sweep_y": (136.0, "interior"),
     "tvd_sweep_z": (136.0, "interior"),
-    # fused path: stage kernel reads U_in 64 + p 8 + U0 64, writes U_out 64 + p 8
-    "stage_fused": (208.0, "interior"),
     "eos_p": (72.0, "buffer"),
 }
+# fused path: the stage kernel reads U_in 64 + p 8 + U0 64 and writes
+# U_out 64 + p 8 = 208 B; stage 1 reads U0 as its input (144 B), stage 4
+# writes no pressure (200 B).  The timers average over the four launches of
+# a step, so the bytes per launch are the mean over the stages.
+STAGE_BYTES = {"fp64": (144.0, 208.0, 208.0, 200.0), "fp32": (80.0, 116.0, 116.0, 112.0)}
+KERNEL_BYTES_FUSED = dict(KERNEL_BYTES, tvd_sweep_z=(144.0, "interior"))   # + new p (epilogue)
 # FP32-mixed form: a cell is rho, M, B in FP32 (28 B) + e in FP64 (8 B) = 36 B, p 4 B
 KERNEL_BYTES_MX = {
-    "stage_fused": (116.0, "interior"),   # read U_in 36 + p 4 + U0 36; write U_out 36 + p 4
     "tvd_sweep_x": (76.0, "interior"),    # read U 36 + p 4; write U 36
     "tvd_sweep_y": (76.0, "interior"),
     "tvd_sweep_z": (80.0, "interior"),    # + new p 4
@@ -60,31 +66,117 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(table, phys, box, steps=2):
-    """Oracle FP64 step (C restatement of the reference, OpenMP) on the same
-    workload: one warm-up step, then `steps` timed steps."""
+def host_cpu():
+    """model name, sockets, cores per socket, threads per core and the CPUs
+    this process may run on"""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["usable_cpus"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def oracle_run(table, phys, box, steps, threads, count=False):
+    """the oracle (C restatement of the reference, FP64, OpenMP on `threads`
+    threads) from the seeded state of box: one warm-up step, then `steps`
+    timed steps.  Returns the oracle, its run descriptor, the dt of every
+    step and the seconds per timed step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from muram_main_amd import config, eos_table, initial
     from orc import Oracle
     tab = eos_table.load_tables(table)
     U = initial.make_state(tab, box)
     g = config.grid_desc(box.gsize, box.dx)
+    Oracle.lib().orc_set_threads(threads)
     o = Oracle(g, phys, table)
     r = config.run_desc()
     o.set_state(U)
     del U
     o.exchange_grid()
     o.set_boundary_conditions(r, 0)
+    if count:
+        o.branch_count(1)
+    dts = []
     o.step(r, config.DT_RAD_OFF, 1)          # warm-up
+    dts.append(r.dt)
     t0 = time.perf_counter()
-    o.step(r, config.DT_RAD_OFF, steps)
-    dt = time.perf_counter() - t0
+    for _ in range(steps):
+        o.step(r, config.DT_RAD_OFF, 1)
+        dts.append(r.dt)
+    sec = (time.perf_counter() - t0) / steps
+    return o, r, dts, sec
+
+
+def gpu_parity(table, wl, o, ro, dts, path):
+    """the GPU step (path: fused or sequence) from the same seeded state for
+    as many steps as the oracle ran, with the limiter branch accounting on:
+    per-field errors of the final state against the oracle's, the largest
+    relative dt difference over the steps and the branch mismatch"""
+    from muram_main_amd import abi, config, initial, eos_table
+    os.environ["MRM_STEP_PATH"] = path
+    try:
+        U = initial.make_state(eos_table.load_tables(table), wl.box)
+        m, r = make_grid(wl, table, U, abi.PREC_FP64)
+    finally:
+        os.environ["MRM_STEP_PATH"] = "fused"
+    del U
+    m.branch_count(True)
+    ddt = 0.0
+    for dt_o in dts:
+        m.step(r, config.DT_RAD_OFF, 1)
+        ddt = max(ddt, abs(r.dt - dt_o) / dt_o)
+    A = m.get_state()
+    bg = m.branch_counts()
+    m.close()
+    B = o.state(0)
+    errs = {}
+    for v, name in enumerate(FIELDS):
+        d = np.abs(A[v] - B[v])
+        errs[name] = float(f"{float(np.max(d) / max(np.max(np.abs(B[v])), 1e-300)):.3e}")
+    del A
+    bo = o.branch_counts()
+    return {"path": path, "steps": len(dts), "field_rel_err_max": errs, "max_rel_err": max(errs.values()),
+            "dt_rel_diff_max": float(f"{ddt:.3e}"), "branch_mismatch": abi.branch_mismatch(bg, bo),
+            "branch_counts": abi.branch_dict(bo)["counts"]}
+
+
+def cpu_baseline(table, wl, steps=3, parity_paths=("fused", "sequence")):
+    """Oracle FP64 step (C restatement of the reference, OpenMP) on the same
+    workload: all usable host cores on the full box (one warm-up, `steps`
+    timed steps, branch accounting on), then one core on a 256x128x128
+    sample of it; the GPU paths from the same state are compared with the
+    oracle's state after the same steps (parity columns)."""
+    from muram_main_amd import initial
+    box = wl.box
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", cpu["usable_cpus"]))
+    o, r, dts, sec = oracle_run(table, wl.phys, box, steps, threads, count=True)
     ncell = box.gsize[0] * box.gsize[1] * box.gsize[2]
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=ncell * steps / dt / 1e6, unit="Mcell-updates/s", cores=threads, kind="port",
+    log(f"cpu baseline: {threads} threads, {sec:.2f} s/step")
+    parity = [gpu_parity(table, wl, o, r, dts, p) for p in parity_paths]
+    del o
+    sub = initial.BoxSpec(gsize=(box.gsize[0], 128, 128), dx=box.dx, v_rms=box.v_rms, seed=box.seed)
+    o1, _, _, sec1 = oracle_run(table, wl.phys, sub, steps, 1)
+    del o1
+    n1 = sub.gsize[0] * sub.gsize[1] * sub.gsize[2]
+    log(f"cpu baseline: 1 thread, {sec1:.2f} s/step on {sub.gsize}")
+    return dict(value=ncell / sec / 1e6, unit="Mcell-updates/s", cores=threads, kind="port",
                 sample=f"oracle FP64 step on the same {box.gsize[0]}x{box.gsize[1]}x{box.gsize[2]} seeded state, "
-                       f"{steps} steps after 1 warm-up, {threads} OpenMP threads",
-                s_per_step=dt / steps)
+                       f"{steps} timed steps after 1 warm-up, {threads} OpenMP threads",
+                s_per_step=sec, host=cpu,
+                one_core={"value": n1 / sec1 / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+                          "s_per_step": sec1,
+                          "sample": f"{sub.gsize[0]}x{sub.gsize[1]}x{sub.gsize[2]} box of the same distributions, "
+                                    f"{steps} timed steps after 1 warm-up"},
+                parity=parity)
 
 
 def timed_run(m, r, steps, warmup):
@@ -117,9 +209,15 @@ def timed_run(m, r, steps, warmup):
     return wall, kernels
 
 
-def roofline(kernels, table, ncell, nbuf, traffic_key):
+def roofline(kernels, table, ncell, nbuf, traffic_key, stage_bytes=None):
+    """the dominant kernel's algorithmic bytes per launch over its mean launch
+    time (HIP events on the library's stream), next to its HBM traffic per
+    launch from the PMC counters in profiles/traffic.json"""
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    per, unit = table.get(dom, (0.0, "interior"))
+    if dom == "stage_fused" and stage_bytes:
+        per, unit = sum(stage_bytes) / len(stage_bytes), "interior"
+    else:
+        per, unit = table.get(dom, (0.0, "interior"))
     units = nbuf if unit == "buffer" else ncell
     bytes_launch = per * units
     achieved = bytes_launch / (kernels[dom]["mean_ms"] * 1e-3) / 1e9
@@ -129,9 +227,14 @@ def roofline(kernels, table, ncell, nbuf, traffic_key):
         tr = json.load(open(tpath)).get(traffic_key, {}).get(dom)
         if tr:
             traffic = tr.get("bytes_per_launch")
-    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                bytes_per_launch=bytes_launch, bytes_per_unit=per, units=units)
+    out = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+               bytes_per_launch=bytes_launch, bytes_per_unit=per, units=units)
+    if dom == "stage_fused" and stage_bytes:
+        out["bytes_per_unit_by_stage"] = list(stage_bytes)
+    if traffic:
+        out["traffic_over_algorithmic"] = round(traffic / bytes_launch, 3)
+    return out
 
 
 def kernel_summary(kernels, wall):
@@ -223,6 +326,9 @@ def precision_study(wl, table, tab, nsteps, control=True, positivity=False):
     if control:
         out["control_fp64_from_rounded_state"] = _errs(states["control"], B, sl_all)
         out["control_max_l2"] = max(v["l2"] for v in out["control_fp64_from_rounded_state"].values())
+        out["control_lower_half"] = _errs(states["control"], B, sl_lo)
+        out["control_max_l2_lower_half"] = max(v["l2"] for v in out["control_lower_half"].values())
+        out["control_upper_half"] = _errs(states["control"], B, sl_hi)
     if positivity:
         out["eint_fp64"] = _eint_stats(B)
         out["eint_fp32_mixed"] = _eint_stats(A)
@@ -330,6 +436,12 @@ def main():
     ap.add_argument("--corona-steps", type=int, default=10,
                     help="steps of the coronal low-beta FP64 vs FP32-mixed run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timed oracle steps of the CPU baseline (after one warm-up)")
+    ap.add_argument("--adjoint-dirs", type=int, default=16,
+                    help="random directions of the adjoint's finite-difference check")
+    ap.add_argument("--no-drop-in", dest="drop_in", action="store_false",
+                    help="skip the timed drop-in sequence sub-line")
     ap.add_argument("--table-n", type=int, default=512)
     ap.add_argument("--path", default="fused", choices=["fused", "sequence"])
     ap.add_argument("--va-adjust", type=float, default=None,
@@ -394,13 +506,35 @@ def main():
         nbuf = m.bufsize
         m.close()
         value = ncell * args.steps / wall / 1e6
-        kb = KERNEL_BYTES if form == "fp64" else KERNEL_BYTES_MX
+        if form == "fp32":
+            kb = KERNEL_BYTES_MX
+        else:
+            kb = KERNEL_BYTES if args.path == "sequence" else KERNEL_BYTES_FUSED
         res_forms[form] = dict(
             value=round(value, 2), ms_per_step=round(wall / args.steps * 1e3, 3),
             hbm_frac_model=round(value * 1e6 * MODEL_BYTES[form] / (HBM_PEAK_GBS * 1e9), 4),
-            roofline=roofline(kernels, kb, ncell, nbuf, form), kernels=kernel_summary(kernels, wall),
-            dt_s=r.dt)
+            roofline=roofline(kernels, kb, ncell, nbuf, form, STAGE_BYTES[form]),
+            kernels=kernel_summary(kernels, wall), dt_s=r.dt)
         log(f"{form}: {res_forms[form]['ms_per_step']} ms/step, {value:.1f} Mcell/s")
+    drop_in = None
+    if args.path == "fused" and "fp64" in forms and args.drop_in:
+        # the per-operation sequence the unchanged solver.C drives through the shim
+        os.environ["MRM_STEP_PATH"] = "sequence"
+        try:
+            m, r = make_grid(wl, table, U, abi.PREC_FP64)
+        finally:
+            os.environ["MRM_STEP_PATH"] = "fused"
+        nseq = max(1, args.steps // 2)
+        wall, kernels = timed_run(m, r, nseq, max(1, args.warmup // 2))
+        nbuf = m.bufsize
+        m.close()
+        value = ncell * nseq / wall / 1e6
+        drop_in = dict(value=round(value, 2), ms_per_step=round(wall / nseq * 1e3, 3), steps=nseq,
+                       hbm_frac_model=round(value * 1e6 * MODEL_BYTES["fp64"] / (HBM_PEAK_GBS * 1e9), 4),
+                       roofline=roofline(kernels, KERNEL_BYTES, ncell, nbuf, "fp64_sequence"),
+                       kernels=kernel_summary(kernels, wall),
+                       path="sequence: one C-ABI call per reference operation (solver.C:109-365)")
+        log(f"drop-in sequence: {drop_in['ms_per_step']} ms/step, {value:.1f} Mcell/s")
     del U
     main_form = forms[0]
     R = res_forms[main_form]
@@ -422,6 +556,8 @@ def main():
         "roofline": R["roofline"],
         "kernels": R["kernels"],
     }
+    if drop_in:
+        res["drop_in"] = drop_in
     if "fp32" in res_forms and main_form == "fp64":
         F = res_forms["fp32"]
         res["fp32_mixed"] = {k: F[k] for k in ("value", "ms_per_step", "hbm_frac_model", "roofline", "kernels")}
@@ -431,14 +567,14 @@ def main():
                                                  args.error_steps)
     if args.corona_steps > 0 and "fp32" in res_forms:
         res["corona_study"] = precision_study(config.workloads()["corona"], table, tab, args.corona_steps,
-                                              control=False, positivity=True)
+                                              control=True, positivity=True)
     if args.adjoint_steps > 0:
         t = time.perf_counter()
-        res["adjoint_study"] = adjoint_study(table, tab, steps=args.adjoint_steps)
+        res["adjoint_study"] = adjoint_study(table, tab, steps=args.adjoint_steps, ndir=args.adjoint_dirs)
         log(f"adjoint study in {time.perf_counter() - t:.1f}s")
     if not args.no_cpu:
         t = time.perf_counter()
-        res["cpu_baseline"] = cpu_baseline(table, wl.phys, box)
+        res["cpu_baseline"] = cpu_baseline(table, wl, steps=args.cpu_steps)
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s")
     else:
         res["cpu_baseline"] = None

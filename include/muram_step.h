@@ -257,6 +257,55 @@ int mrm_timers(mrm_ctx* ctx, int enable);
 int mrm_get_timers(mrm_ctx* ctx, double ms[MRM_NTIMER], long launches[MRM_NTIMER]);
 const char* mrm_timer_name(int id);
 
+/* Limiter branch accounting (FP64 contexts).  No counterpart in the
+ * reference: it counts the branch choices TVDlimit makes (tvdlimit_SR.C)
+ * so that a caller can check that two implementations chose alike.
+ * Counted are the evaluations on lines whose transverse indices are
+ * interior (every point, slope, interface and update of such a line, in
+ * each of the three sweeps), summed over every TVDlimit call while the
+ * accounting is on.  Slots 0..MRM_BR_NCOUNT-1 are counts; slots
+ * MRM_BR_NCOUNT.. are position-weighted checksums, one per branch point:
+ * the wrapping 64-bit sum over the evaluations of w(sweep, node, field) x
+ * code, w an odd hash of the position and code the choice's number (1 for a
+ * switch taken), so that two runs with equal counts but a choice moved from
+ * one cell to another still differ.  Categories:
+ *   CZ        Tau > 1e-5 at a point                        (:281)
+ *   CME       CME_mode = 1 at a point                      (:336)
+ *   SLOPE_*   the reconstruction slope per point and field (:392-396):
+ *             positive from sl, sr or slm, negative from sl, sr or slm, zero
+ *   QRHO      density slope zeroed by the jump detector    (:419-421)
+ *   VFAST     velocity slopes zeroed for fast flow         (:427-433)
+ *   MM_*      the minmod choice per interface and field    (:449): positive
+ *             from sl_lim or sl, negative from sl_lim or sl, zero
+ *   CLIP      max(0, 1 + dn (rf - 1)) clipped to 0         (:450)
+ *   CMAX_*    min(cmax, max(cm_i, cm_i+1)): cmax, cm_i or cm_i+1 (:457)
+ *   RHO_FLOOR max(rho_min, rho) raised rho after the update (:653)
+ *   E_FLOOR   the energy floor raised e after the update   (:656)
+ * mrm_branch_count(ctx, 1) zeroes the slots and turns the accounting on
+ * (the TVD sweeps then run counting instantiations of the same kernels);
+ * mrm_branch_count(ctx, 0) turns it off; mrm_get_branch_counts synchronizes
+ * and copies the MRM_BR_NSLOT slots.  The diagnostics path (an output step's
+ * collecting sweeps) is not counted. */
+enum {
+  MRM_BR_CZ = 0, MRM_BR_CME,
+  MRM_BR_SLOPE_POS_L, MRM_BR_SLOPE_POS_R, MRM_BR_SLOPE_POS_M,
+  MRM_BR_SLOPE_NEG_L, MRM_BR_SLOPE_NEG_R, MRM_BR_SLOPE_NEG_M, MRM_BR_SLOPE_ZERO,
+  MRM_BR_QRHO, MRM_BR_VFAST,
+  MRM_BR_MM_POS_LIM, MRM_BR_MM_POS_SL, MRM_BR_MM_NEG_LIM, MRM_BR_MM_NEG_SL, MRM_BR_MM_ZERO,
+  MRM_BR_CLIP,
+  MRM_BR_CMAX_LIMIT, MRM_BR_CMAX_LEFT, MRM_BR_CMAX_RIGHT,
+  MRM_BR_RHO_FLOOR, MRM_BR_E_FLOOR,
+  MRM_BR_NCOUNT
+};
+/* checksum slots: MRM_BR_NCOUNT + kind */
+enum {
+  MRM_BK_CZ = 0, MRM_BK_CME, MRM_BK_SLOPE, MRM_BK_QRHO, MRM_BK_VFAST, MRM_BK_MM, MRM_BK_CLIP,
+  MRM_BK_CMAX, MRM_BK_RHO_FLOOR, MRM_BK_E_FLOOR, MRM_BK_N
+};
+#define MRM_BR_NSLOT (MRM_BR_NCOUNT + MRM_BK_N)
+int mrm_branch_count(mrm_ctx* ctx, int on);
+int mrm_get_branch_counts(mrm_ctx* ctx, unsigned long long out[MRM_BR_NSLOT]);
+
 #ifdef __cplusplus
 }
 #endif

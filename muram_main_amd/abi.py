@@ -35,8 +35,34 @@ EXPORTS = [
     "mrm_io_write_backup", "mrm_io_read_backup", "mrm_io_read_var", "mrm_io_write_var",
     "mrm_adjoint_forward", "mrm_adjoint_backward",
     "mrm_timers", "mrm_get_timers", "mrm_timer_name",
+    "mrm_branch_count", "mrm_get_branch_counts",
 ]
 NTIMER = 24
+
+# limiter branch accounting: names of the count slots (MRM_BR_*) and of the
+# checksum slots (MRM_BK_*), in slot order
+BRANCH_COUNTS = ("cz", "cme", "slope_pos_l", "slope_pos_r", "slope_pos_m", "slope_neg_l", "slope_neg_r",
+                 "slope_neg_m", "slope_zero", "qrho", "vfast", "mm_pos_lim", "mm_pos_sl", "mm_neg_lim",
+                 "mm_neg_sl", "mm_zero", "clip", "cmax_limit", "cmax_left", "cmax_right", "rho_floor",
+                 "e_floor")
+BRANCH_SUMS = ("cz", "cme", "slope", "qrho", "vfast", "minmod", "clip", "cmax", "rho_floor", "e_floor")
+BR_NSLOT = len(BRANCH_COUNTS) + len(BRANCH_SUMS)
+
+
+def branch_dict(slots):
+    """{'counts': {name: n}, 'checksums': {name: hex}} of MRM_BR_NSLOT slots"""
+    n = len(BRANCH_COUNTS)
+    return {"counts": {k: int(slots[i]) for i, k in enumerate(BRANCH_COUNTS)},
+            "checksums": {k: f"{int(slots[n + i]):016x}" for i, k in enumerate(BRANCH_SUMS)}}
+
+
+def branch_mismatch(a, b):
+    """number of count slots that differ, summed absolute count difference and
+    number of differing checksums between two slot arrays"""
+    n = len(BRANCH_COUNTS)
+    diff = [abs(int(a[i]) - int(b[i])) for i in range(n)]
+    return {"slots_differing": sum(1 for d in diff if d), "count_diff": sum(diff),
+            "checksums_differing": sum(1 for i in range(n, BR_NSLOT) if int(a[i]) != int(b[i]))}
 
 
 class GridDesc(C.Structure):
@@ -119,6 +145,8 @@ def load(path: str = LIB_PATH):
         "mrm_timers": ([P, C.c_int], C.c_int),
         "mrm_get_timers": ([P, dp, C.POINTER(C.c_long)], C.c_int),
         "mrm_timer_name": ([C.c_int], C.c_char_p),
+        "mrm_branch_count": ([P, C.c_int], C.c_int),
+        "mrm_get_branch_counts": ([P, C.POINTER(C.c_ulonglong)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

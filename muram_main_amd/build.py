@@ -38,11 +38,15 @@ def _run(cmd, verbose):
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
-    for src, contract in UNITS:
-        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC] + FLAGS + [contract, "-I", CSRC, "-c", "-o", obj, os.path.join(CSRC, src)], verbose)
-        objs.append(obj)
+    """compile the translation units side by side (at most MAX_JOBS, default
+    the CPU count) and link the library"""
+    from concurrent.futures import ThreadPoolExecutor
+    objs = [os.path.join(CSRC, os.path.splitext(src)[0] + ".o") for src, _ in UNITS]
+    cmds = [[HIPCC] + FLAGS + [contract, "-I", CSRC, "-c", "-o", obj, os.path.join(CSRC, src)]
+            for (src, contract), obj in zip(UNITS, objs)]
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda cmd: _run(cmd, verbose), cmds))
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, OUT)

@@ -525,9 +525,46 @@ struct TvdPt {
   double vsqr, bsqr, cm, hyp_e, hyp_v, hyp_B, BC2, CZ, hd;
 };
 
+// Limiter branch accounting (counting instantiations only; the sweep
+// functions take a null TvdCnt* otherwise, and inlining removes the code).
+// n: counts per choice (MRM_BR_* of muram_step.h); h: per branch point the
+// wrapping sum of br_weight(position) x the choice's number; key: position of
+// the point or interface being evaluated, (sweep * bufsize + node) * 8, to
+// which the field index is added.
+MRM_HD unsigned long long br_weight(unsigned long long x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return (x ^ (x >> 31)) | 1ull;
+}
+struct TvdCnt {
+  unsigned n[MRM_BR_NCOUNT];
+  unsigned long long h[MRM_BK_N];
+  unsigned long long key;
+  bool on;            // the line being swept is counted
+  MRM_HD void clear() {
+    for (int i = 0; i < MRM_BR_NCOUNT; i++) n[i] = 0;
+    for (int i = 0; i < MRM_BK_N; i++) h[i] = 0;
+    key = 0;
+    on = false;
+  }
+  // a switch: counted and weighted when taken
+  MRM_HD void hit(int c, int kind, int q, bool taken) {
+    taken = taken && on;
+    n[c] += taken ? 1u : 0u;
+    h[kind] += taken ? br_weight(key + q) : 0ull;
+  }
+  // one of ncat choices (cat = 0 .. ncat-1, slots c0 ..)
+  MRM_HD void cat(int c0, int ncat, int kind, int q, int cat) {
+    for (int i = 0; i < ncat; i++) n[c0 + i] += (on && cat == i) ? 1u : 0u;
+    h[kind] += on ? br_weight(key + q) * (unsigned long long)(cat + 1) : 0ull;
+  }
+};
+
 // per-point quantities (tvdlimit_SR.C:269-385)
 MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, double tau,
-                                          double hfb1, double hft1, bool xsweep, TvdPt& o) {
+                                          double hfb1, double hft1, bool xsweep, TvdPt& o,
+                                          TvdCnt* cn = nullptr) {
   double v0 = smax(P.rho_min, u[0]);
   double v1 = u[1], v2 = u[2], v3 = u[3], v4 = u[4];
   o.v[5] = u[5]; o.v[6] = u[6]; o.v[7] = u[7];
@@ -550,6 +587,10 @@ MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, d
   double cs2 = 5.0 / 3.0 * pres * dn;
   double cfast = sqrt(va2 + cs2);
   double CME_mode = (1.0 - o.CZ) * ((double)(vv >= P.CME_thresh * cfast));
+  if (cn) {
+    cn->hit(MRM_BR_CZ, MRM_BK_CZ, 0, o.CZ != 0.0);
+    cn->hit(MRM_BR_CME, MRM_BK_CME, 0, CME_mode != 0.0);
+  }
   dn = v0 * v0;
   double rf = smin(lf, dn / (dn + P.rho_lev));
   double hh = 1.0 - hfb1 - hft1;
@@ -583,7 +624,7 @@ MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, d
 
 // limited slope at point i (tvdlimit_SR.C:388-434)
 MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
-                                          double hfb1, double hft1, double S[MRM_NV]) {
+                                          double hfb1, double hft1, double S[MRM_NV], TvdCnt* cn = nullptr) {
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
     double sl = b.v[q] - a.v[q];
@@ -592,6 +633,20 @@ MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const 
     double lower = smin(smin(sl, sr), slm);
     double upper = smax(smax(sl, sr), slm);
     S[q] = smax(0.0, lower) + smin(0.0, upper);
+    if (cn) {
+      // which of sl, sr, slm the min (positive) or max (negative) took
+      int k;
+      if (lower > 0.0) {
+        const double m1 = (sr < sl) ? sr : sl;
+        k = (slm < m1) ? 2 : (sr < sl) ? 1 : 0;
+      } else if (upper < 0.0) {
+        const double m1 = (sl < sr) ? sr : sl;
+        k = (m1 < slm) ? 5 : (sl < sr) ? 4 : 3;
+      } else {
+        k = 6;
+      }
+      cn->cat(MRM_BR_SLOPE_POS_L, 7, MRM_BK_SLOPE, q, k);
+    }
   }
   if ((P.visc_slope_bot > 0.0) || (P.visc_slope_top > 0.0)) {
     double cf = (1.0 - P.visc_slope_bot * hfb1) * (1.0 - P.visc_slope_top * hft1);
@@ -603,8 +658,10 @@ MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const 
   }
   double qrho = fabs(b.lnr - c.lnr);
   if (qrho > P.q_rho_max) S[0] = 0.0;
+  if (cn) cn->hit(MRM_BR_QRHO, MRM_BK_QRHO, 0, qrho > P.q_rho_max);
   if (P.vmax_lim > 0.0) {
     if (b.vsqr > P.vsqr_diff) { S[1] = 0.0; S[2] = 0.0; S[3] = 0.0; }
+    if (cn) cn->hit(MRM_BR_VFAST, MRM_BK_VFAST, 0, b.vsqr > P.vsqr_diff);
   }
 }
 
@@ -616,7 +673,8 @@ template <int D, bool DIAG = false>
 MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, const TvdPt& c,
                                             const TvdPt& d, const double Sb[MRM_NV], const double Sc[MRM_NV],
                                             double hfb1, double hft1, double qft1, bool hyper_on,
-                                            bool zero_bflux, double F[MRM_NV], double* qv = nullptr) {
+                                            bool zero_bflux, double F[MRM_NV], double* qv = nullptr,
+                                            TvdCnt* cn = nullptr) {
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
     double sl = c.v[q] - b.v[q];
@@ -624,9 +682,22 @@ MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, cons
     double dn = (q >= 1 && q <= 3) ? b.hyp_v : (q >= 5 ? b.hyp_B : b.hyp_e);
     double rf = fabs(sl_lim) / smax(1e-100, fabs(sl));
     double cf = smax(0.0, smin(sl_lim, sl)) + smin(0.0, smax(sl_lim, sl));
-    F[q] = smax(0.0, 1.0 + dn * (rf - 1.0)) * cf;
+    double cl = 1.0 + dn * (rf - 1.0);
+    F[q] = smax(0.0, cl) * cf;
+    if (cn) {
+      // the minmod's choice: both positive (the smaller of sl_lim, sl), both
+      // negative (the larger), or zero
+      const double mn = (sl < sl_lim) ? sl : sl_lim, mx = (sl_lim < sl) ? sl : sl_lim;
+      const int k = (mn > 0.0) ? ((sl < sl_lim) ? 1 : 0) : (mx < 0.0) ? ((sl_lim < sl) ? 3 : 2) : 4;
+      cn->cat(MRM_BR_MM_POS_LIM, 5, MRM_BK_MM, q, k);
+      cn->hit(MRM_BR_CLIP, MRM_BK_CLIP, q, cl < 0.0);
+    }
   }
   double tvd_faci = 0.5 * P.idxd1 * smin(P.cmax, smax(b.cm, c.cm));
+  if (cn) {
+    const double m = (b.cm < c.cm) ? c.cm : b.cm;
+    cn->cat(MRM_BR_CMAX_LIMIT, 3, MRM_BK_CMAX, 0, (P.cmax < m) ? 0 : (b.cm < c.cm) ? 2 : 1);
+  }
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) F[q] *= tvd_faci;
   F[5 + D] *= P.B_par_diff;
@@ -674,7 +745,7 @@ MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, cons
 template <bool DIAG = false>
 MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_NV],
                                            const double F[MRM_NV], double qm, double q, double qft1,
-                                           double u[MRM_NV], double* dg = nullptr) {
+                                           double u[MRM_NV], double* dg = nullptr, TvdCnt* cn = nullptr) {
   double r[MRM_NV];
 #pragma unroll
   for (int k = 0; k < MRM_NV; k++) r[k] = P.dt_tvd * (F[k] - Fm[k]);
@@ -691,13 +762,16 @@ MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_N
   r[4] += P.dt_tvd * qres;
 #pragma unroll
   for (int k = 0; k < MRM_NV; k++) u[k] += r[k];
+  if (cn) cn->hit(MRM_BR_RHO_FLOOR, MRM_BK_RHO_FLOOR, 0, u[0] < P.rho_min);
   u[0] = smax(P.rho_min, u[0]);
   if (DIAG) {
     dg[1] = boris3;
     dg[2] = qres;
     dg[3] = u[4];
   }
-  u[4] = smax(u[4], P.eps_min * u[0] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]);
+  const double efl = P.eps_min * u[0] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0];
+  if (cn) cn->hit(MRM_BR_E_FLOOR, MRM_BK_E_FLOOR, 0, u[4] < efl);
+  u[4] = smax(u[4], efl);
 }
 
 }  // namespace MRM_NS

@@ -406,10 +406,13 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_ghost2_periodic(Geom G, const dou
   long base = i + b * G.s1;
   long src_r = base + (long)(G.lb2 + igh) * G.s2, dst_r = base + (long)(G.le2 + 1 + igh) * G.s2;
   long src_l = base + (long)(G.le2 - G.g2 + 1 + igh) * G.s2, dst_l = base + (long)(G.lb2 - G.g2 + igh) * G.s2;
-  bool same = P[dst_r] == P[src_r] && P[dst_l] == P[src_l];
+  // bit patterns, so that -0.0 against +0.0 (equal as numbers, but the
+  // reference sweeps each plane with its own bits) counts as a difference
+  auto eq = [](double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); };
+  bool same = eq(P[dst_r], P[src_r]) && eq(P[dst_l], P[src_l]);
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++)
-    same = same && U[q * G.bs + dst_r] == U[q * G.bs + src_r] && U[q * G.bs + dst_l] == U[q * G.bs + src_l];
+    same = same && eq(U[q * G.bs + dst_r], U[q * G.bs + src_r]) && eq(U[q * G.bs + dst_l], U[q * G.bs + src_l]);
   if (!same) *diff = 1ull;
 }
 
@@ -555,11 +558,22 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_cmax_visited(Geom G, const double
 // thread t owns point t of the line; the per-point, per-slope, per-interface
 // and per-update arithmetic is tvd_point / tvd_slope / tvd_iface / tvd_update
 
+// lines whose arrays exceed 64 KB of LDS ask for the larger dynamic LDS
+// first (gfx950 gives one block up to 160 KB)
+template <class K>
+inline void allow_lds(K kernel, size_t lds) {
+  if (lds > 64 * 1024) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    (void)hipGetLastError();
+  }
+}
+
 // MAXT: the block size bound (lines of up to MAXT points).  At 1024 the
 // kernel fits 128 VGPRs (4 waves per SIMD) without spilling; a 320- or
 // 512-thread bound lets it use more registers at fewer waves, which
 // measured slower on box512 lines
-template <int MAXT>
+template <int MAXT, bool CNT = false>
 __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __restrict__ U,
                                                         const double* __restrict__ Pr) {
   extern __shared__ double sm[];
@@ -580,13 +594,20 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
   double* slr = scz + sz;              // ln rho
   double* sq = slr + sz;               // resistive interface term
   const int t = threadIdx.x;
+  TvdCnt cnt;
+  if (CNT) {
+    cnt.clear();
+    cnt.on = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
+    cnt.key = br_key(0, bs, base + t);
+  }
+  TvdCnt* const cn = CNT ? &cnt : nullptr;
   TvdPt me;
   if (t < sz) {
     double u[MRM_NV];
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + base + t];
     double tau = P.tau ? P.tau[base + t] : 0.0;
-    tvd_point(P, u, Pr[base + t], tau, P.hfb[t], P.hft[t], true, me);
+    tvd_point(P, u, Pr[base + t], tau, P.hfb[t], P.hft[t], true, me, cn);
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) sv[q * sz + t] = me.v[q];
     scm[t] = me.cm;
@@ -603,7 +624,7 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) { a.v[q] = sv[q * sz + t - 1]; c.v[q] = sv[q * sz + t + 1]; }
     c.lnr = slr[t + 1];
-    tvd_slope(P, a, me, c, P.hfb[t], P.hft[t], S);
+    tvd_slope(P, a, me, c, P.hfb[t], P.hft[t], S, cn);
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) sS[q * sz + t] = S[q];
   }
@@ -630,7 +651,7 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
     c.hd = shd[t + 1];
     c.CZ = scz[t + 1];
     bool zb = (P.is_gend0 && t == sz - gh - 1) || (P.is_gbeg0 && t == gh - 1);
-    double q = tvd_iface<0>(P, a, me, c, d, S, Sc, P.hfb[t], P.hft[t], P.qft[t], t >= istart, zb, F);
+    double q = tvd_iface<0>(P, a, me, c, d, S, Sc, P.hfb[t], P.hft[t], P.qft[t], t >= istart, zb, F, nullptr, cn);
 #pragma unroll
     for (int qq = 0; qq < MRM_NV; qq++) sF[qq * sz + t] = F[qq];
     sq[t] = q;
@@ -645,10 +666,11 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
       Fm[q] = sF[q * sz + t - 1];
       u[q] = U[q * bs + base + t];
     }
-    tvd_update(P, me, Fm, F, sq[t - 1], sq[t], P.qft[t], u);
+    tvd_update(P, me, Fm, F, sq[t - 1], sq[t], P.qft[t], u, nullptr, cn);
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) U[q * bs + base + t] = u[q];
   }
+  if (CNT) tvd_cnt_flush(cnt, P.bcount);
 }
 
 }  // namespace MRM_NS

@@ -562,7 +562,30 @@ __device__ double ghost_pres_(const EosDev& E, const double u[MRM_NV]) {
 // The conserved values of the last three points of each line wait in an LDS
 // ring ([slot][field][thread], 48 KB per block) for their update two points
 // later, instead of a second read from memory.
-template <int D, bool HAS_TAU, bool EPI, bool VALF = false, bool DIAG = false>
+// limiter branch slots of one thread added to the context's totals: one
+// atomic per slot and wave (every lane of the wave must call it)
+__device__ inline void tvd_cnt_flush(const TvdCnt& c, unsigned long long* out) {
+  const bool lead = (threadIdx.x & 63) == 0;
+#pragma unroll
+  for (int i = 0; i < MRM_BR_NCOUNT; i++) {
+    unsigned long long v = c.n[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lead && v) atomicAdd(out + i, v);
+  }
+#pragma unroll
+  for (int i = 0; i < MRM_BK_N; i++) {
+    unsigned long long v = c.h[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lead && v) atomicAdd(out + MRM_BR_NCOUNT + i, v);
+  }
+}
+
+// position key of node in sweep d (see TvdCnt)
+__device__ __forceinline__ unsigned long long br_key(int d, long bs, long node) {
+  return ((unsigned long long)d * (unsigned long long)bs + (unsigned long long)node) << 3;
+}
+
+template <int D, bool HAS_TAU, bool EPI, bool VALF = false, bool DIAG = false, bool CNT = false>
 __global__ void __launch_bounds__(MRM_BLOCK) __attribute__((amdgpu_waves_per_eu(VALF ? 2 : 1)))
 k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restrict__ Pr, TvdEpilogue X,
             DiagArgs DA) {
@@ -570,6 +593,9 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
   __shared__ double vacc[VALF ? 5 * MRM_BLOCK : 1];
   __shared__ double ring[3][MRM_NV][MRM_BLOCK];
   if (VALF) valf_lds_init(vacc + threadIdx.x, MRM_BLOCK);
+  TvdCnt cnt;
+  if (CNT) cnt.clear();
+  TvdCnt* const cn = CNT ? &cnt : nullptr;
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const int tl = threadIdx.x;
@@ -580,6 +606,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
     active = t < (long)G.n1 * G.n2;
     int j = (int)(t % G.n1), k = (int)(t / G.n1);
     base = j * G.s1 + k * G.s2; str = 1; sz = G.n0; gh = G.g0;
+    if (CNT) cnt.on = active && j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
   } else {
     const int nx = G.le0 - G.lb0 + 1;
     const int nb = D == 1 ? P.y_nk : (G.le1 - G.lb1 + 1);
@@ -589,6 +616,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
     lineX = i;
     if (D == 1) { base = i + (long)(P.y_k0 + b) * G.s2; str = G.s1; sz = G.n1; gh = G.g1; }
     else { base = i + (G.lb1 + b) * G.s1; str = G.s2; sz = G.n2; gh = G.g2; }
+    if (CNT) cnt.on = active && (D == 2 || (P.y_k0 + b >= G.lb2 && P.y_k0 + b <= G.le2));
   }
   double cm[1] = {0.0};
   if (active) {
@@ -612,26 +640,29 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
       slot = slot2;
       double tau = HAS_TAU ? P.tau[node] : 0.0;
       double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
-      tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3);
+      if (CNT) cnt.key = br_key(D, bs, node);
+      tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3, cn);
       if (s >= 2) {
         const int ip = s - 1;   // slope at point s-1
 #pragma unroll
         for (int q = 0; q < MRM_NV; q++) S0[q] = S1[q];
-        tvd_slope(P, W1, W2, W3, D == 0 ? P.hfb[ip] : lhfb, D == 0 ? P.hft[ip] : lhft, S1);
+        if (CNT) cnt.key = br_key(D, bs, base + ip * str);
+        tvd_slope(P, W1, W2, W3, D == 0 ? P.hfb[ip] : lhfb, D == 0 ? P.hft[ip] : lhft, S1, cn);
       }
       if (s >= 3) {
         const int ii = s - 2;   // interface between ii and ii+1; update of point ii
         const double ih = D == 0 ? P.hfb[ii] : lhfb, it = D == 0 ? P.hft[ii] : lhft;
         const double iq = D == 0 ? P.qft[ii] : lqft;
         bool zb = (D == 0) && ((P.is_gend0 && ii == sz - gh - 1) || (P.is_gbeg0 && ii == gh - 1));
-        double q = tvd_iface<D, DIAG>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F, &qv);
+        if (CNT) cnt.key = br_key(D, bs, base + ii * str);
+        double q = tvd_iface<D, DIAG>(P, W0, W1, W2, W3, S0, S1, ih, it, iq, ii >= istart, zb, F, &qv, cn);
         if (ii >= gh && ii <= sz - gh - 1) {
           long nd = base + ii * str;
           double uu[MRM_NV];
 #pragma unroll
           for (int k = 0; k < MRM_NV; k++) uu[k] = ring[slot2][k][tl];
           double dg[4];
-          tvd_update<DIAG>(P, W1, Fm, F, qm, q, iq, uu, dg);
+          tvd_update<DIAG>(P, W1, Fm, F, qm, q, iq, uu, dg, cn);
           if (DIAG) {
             DA.dg[DG_T8 * bs + nd] += dg[0] * DA.idt;
             DA.dg[DG_T6 * bs + nd] += dg[1] * DA.idt;
@@ -663,6 +694,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
       W0 = W1; W1 = W2; W2 = W3;
     }
   }
+  if (CNT) tvd_cnt_flush(cnt, P.bcount);
   if (EPI) {
     block_reduce<OpMax, 1>(cm, lds);
     if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);

@@ -26,13 +26,44 @@ inline hipError_t launch_stage(hipStream_t st, unsigned nb, bool zwin, const Sta
 
 inline hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const TvdLineArgs& A,
                                      double* U, const double* Pr) {
-  k_tvd_x_lines<TX_MAXN><<<nl, bsz, lds, st>>>(A, U, Pr);
+  if (A.P.bcount) {
+    allow_lds(k_tvd_x_lines<TX_MAXN, true>, lds);
+    k_tvd_x_lines<TX_MAXN, true><<<nl, bsz, lds, st>>>(A, U, Pr);
+  } else {
+    allow_lds(k_tvd_x_lines<TX_MAXN>, lds);
+    k_tvd_x_lines<TX_MAXN><<<nl, bsz, lds, st>>>(A, U, Pr);
+  }
+  return hipGetLastError();
+}
+
+// the counting instantiations of the marching sweeps (limiter branch accounting)
+inline hipError_t launch_tvd_sweep_cnt(hipStream_t st, int d, bool tau, const TvdEpilogue* epi, unsigned nb,
+                                       const Geom& G, const TvdParams& P, double* U, const double* Pr) {
+  TvdEpilogue X{};
+  const DiagArgs N{};
+  if (d == 0) {
+    if (tau) k_tvd_sweep<0, true, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, N);
+    else k_tvd_sweep<0, false, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, N);
+  } else if (d == 1) {
+    if (tau) k_tvd_sweep<1, true, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, N);
+    else k_tvd_sweep<1, false, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, N);
+  } else if (epi && epi->valf) {
+    if (tau) k_tvd_sweep<2, true, true, true, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, N);
+    else k_tvd_sweep<2, false, true, true, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, N);
+  } else if (epi) {
+    if (tau) k_tvd_sweep<2, true, true, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, N);
+    else k_tvd_sweep<2, false, true, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, N);
+  } else {
+    if (tau) k_tvd_sweep<2, true, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, N);
+    else k_tvd_sweep<2, false, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, N);
+  }
   return hipGetLastError();
 }
 
 // marching sweep along dim d (0, 1, 2); epi applies the finishing epilogue (d = 2)
 inline hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const TvdEpilogue* epi, unsigned nb,
                                    const Geom& G, const TvdParams& P, double* U, const double* Pr) {
+  if (P.bcount) return launch_tvd_sweep_cnt(st, d, tau, epi, nb, G, P, U, Pr);
   TvdEpilogue X{};
   if (d == 0) {
     if (tau) k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});

@@ -71,6 +71,7 @@ hipError_t launch_cmax(hipStream_t st, unsigned nb, const Geom& G, MxState S, co
 
 hipError_t launch_tvd_x_lines(hipStream_t st, unsigned nl, unsigned bsz, size_t lds, const Geom& G,
                               const MxTvdParams& P, MxState S, const float* Pr) {
+  allow_lds(k_tvd_x_lines_mx, lds);
   k_tvd_x_lines_mx<<<nl, bsz, lds, st>>>(G, P, S, Pr);
   return hipGetLastError();
 }

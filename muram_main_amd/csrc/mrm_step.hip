@@ -58,6 +58,8 @@ struct mrm_ctx {
   // stores the zeros first (res_settle)
   bool res_zero = false;
   bool force_sequence = false;
+  bool tvd_x_march = false;                      // dim-0 TVD always by the marching kernel
+  size_t lds_max = 64 * 1024;                    // LDS bytes one block may use (device attribute)
   bool stage_zwin = false;                       // stage kernel variant
   bool fma = true;                               // fused step uses the contracted build
   double last_dt = 0.0;                          // dt and TVD split of the last fused step
@@ -104,6 +106,10 @@ struct mrm_ctx {
   // MRM_TVAR1..MRM_QVIS), allocated by their first use
   double* d_diag = nullptr;
   int diag_last = 0;   // need_diagnostics of the last call that took a run descriptor
+
+  // limiter branch accounting (mrm_branch_count): MRM_BR_NSLOT slots
+  unsigned long long* d_bcount = nullptr;
+  bool bcount_on = false;
 
   unsigned long long* d_red = nullptr;   // RED_NSLOT slots
   double* d_part = nullptr;
@@ -303,6 +309,15 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   c->stage_zwin = !(zw && zw[0] == '0');
   const char* fm = getenv("MRM_FMA");
   c->fma = !(fm && fm[0] == '0');
+  const char* tx = getenv("MRM_TVD_X");
+  c->tvd_x_march = tx && strcmp(tx, "march") == 0;
+  // LDS one block may use: the larger of the default and the opt-in limit
+  // (the kernels above 64 KB ask for it with hipFuncSetAttribute)
+  for (hipDeviceAttribute_t a : {hipDeviceAttributeMaxSharedMemoryPerBlock, hipDeviceAttributeSharedMemPerBlockOptin}) {
+    int v = 0;
+    if (!rc && hipDeviceGetAttribute(&v, a, c->device) == hipSuccess && (size_t)v > c->lds_max) c->lds_max = (size_t)v;
+  }
+  (void)hipGetLastError();
   if (!rc && (e = hipMemsetAsync(c->dR, 0, 8 * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && (e = hipMemsetAsync(c->dF, 0, MRM_NFIELD * bs * sizeof(double), c->stream)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "memset", e);
   if (!rc && phys->params[3] > 0.0) {   // Spitzer: sflx0, Rflx
@@ -328,7 +343,7 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   for (int q = 0; q < 3; q++) { hipFree(c->S[q]); hipFree(c->mxmem[q]); }
   for (int q = 0; q < 2; q++) hipFree(c->Pm[q]);
   hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2); hipFree(c->d_spt); hipFree(c->d_diag);
-  hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part);
+  hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part); hipFree(c->d_bcount);
   for (double* p : c->adj_ckpt) hipFree(p);
   hipFree(c->adj_mem);
   if (c->h_red) hipHostFree(c->h_red);
@@ -1282,12 +1297,21 @@ static int tvd_params(mrm_ctx* c, double dt_tvd, TvdParams& P) {
   P.hft = c->d_prof + n0;
   P.qft = c->d_prof + 2 * n0;
   P.tau = c->tau_set ? c->dF + MRM_TAU * c->G.bs : nullptr;
+  P.bcount = c->bcount_on ? c->d_bcount : nullptr;
   P.y_k0 = 0;        // every plane, ghosts included (tvdlimit_SR.C:199-205)
   P.y_nk = c->G.n2;
   return MRM_OK;
 }
 
 static size_t tvd_x_lds(const mrm_ctx* c) { return (size_t)(2 * MRM_NV + 5) * c->G.n0 * sizeof(double); }
+
+// whether the dim-0 sweep runs one block per line (k_tvd_x_lines: the line's
+// points, slopes and fluxes in LDS, lds bytes of it): lines of at most TX_MAXN
+// points whose arrays fit one block's LDS (160 KB on gfx950); otherwise the
+// marching kernel, one thread per line
+static bool tvd_x_lines_fit(const mrm_ctx* c, size_t lds) {
+  return !c->tvd_x_march && c->G.n0 <= TX_MAXN && lds <= c->lds_max;
+}
 
 // the three sweeps of TVDlimit on buffer U with pressure Pr; with epi the
 // dim-2 sweep finishes each cell (TCheck, new pressure, cmax)
@@ -1330,21 +1354,25 @@ static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const
     P.idxd1 = idx[d];
     long nl = d == 0 ? (long)G.n1 * G.n2 : d == 1 ? (long)c->lsize[0] * P.y_nk : (long)c->lsize[0] * c->lsize[1];
     unsigned nb = nblk(nl);
-    TScope ts(c, T_TVD0 + d);
-    hipError_t e;
-    size_t lds = tvd_x_lds(c);
-    if (d == 0 && G.n0 <= TX_MAXN && lds <= 64 * 1024) {
-      TvdLineArgs A{G, P};
-      unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
-      e = contracted ? fast::p_launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, A, U, Pr)
-                     : exact::launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, A, U, Pr);
-    } else {
-      const TvdEpilogue* x = d == 2 ? epi : nullptr;
-      e = contracted ? fast::p_launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr)
-                     : exact::launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr);
+    {
+      TScope ts(c, T_TVD0 + d);
+      hipError_t e;
+      size_t lds = tvd_x_lds(c);
+      if (d == 0 && tvd_x_lines_fit(c, lds)) {
+        TvdLineArgs A{G, P};
+        unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
+        e = contracted ? fast::p_launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, A, U, Pr)
+                       : exact::launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, A, U, Pr);
+      } else {
+        const TvdEpilogue* x = d == 2 ? epi : nullptr;
+        e = contracted ? fast::p_launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr)
+                       : exact::launch_tvd_sweep(c->stream, d, tau, x, nb, G, P, U, Pr);
+      }
+      if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep", e);
     }
-    if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep", e);
     if (d == 1 && y_copy) {
+      // the copy of the dim-2 ghost planes counts as an exchange launch
+      TScope ts(c, T_EXCH);
       const long n2 = (long)c->lsize[0] * G.n1 * G.g2;
       k_exchange_p<2><<<nblk(n2), MRM_BLOCK, 0, c->stream>>>(G, U, nullptr);
       LAUNCHCHK(c, "k_exchange_p<2>");
@@ -1384,6 +1412,33 @@ static int tvd_buf_diag(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, 
     hipError_t e = exact::launch_tvd_sweep_diag(c->stream, d, P.tau != nullptr, nblk(nl), G, P, U, Pr, DA);
     if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "tvd sweep", e);
   }
+  return MRM_OK;
+}
+
+// limiter branch accounting: the sweeps of tvd_buf run their counting
+// instantiations while it is on (the FP32-mixed sweeps and the diagnostics
+// sweeps are not counted)
+extern "C" int mrm_branch_count(mrm_ctx* c, int on) {
+  if (!c) return MRM_ERR_ARG;
+  if (on) {
+    NEED_FP64(c, "mrm_branch_count");
+    if (!c->d_bcount)
+      HIPCHK(c, hipMalloc((void**)&c->d_bcount, MRM_BR_NSLOT * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemsetAsync(c->d_bcount, 0, MRM_BR_NSLOT * sizeof(unsigned long long), c->stream));
+  }
+  c->bcount_on = on != 0;
+  return MRM_OK;
+}
+
+extern "C" int mrm_get_branch_counts(mrm_ctx* c, unsigned long long out[MRM_BR_NSLOT]) {
+  if (!c || !out) return MRM_ERR_ARG;
+  if (!c->d_bcount) {
+    memset(out, 0, MRM_BR_NSLOT * sizeof(unsigned long long));
+    return MRM_OK;
+  }
+  HIPCHK(c, hipMemcpyAsync(out, c->d_bcount, MRM_BR_NSLOT * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRM_OK;
 }
 
@@ -1677,7 +1732,7 @@ static int tvd_mx(mrm_ctx* c, MxState M, const float* Pr, double dt_tvd, const M
     TScope ts(c, T_TVD0 + d);
     hipError_t e;
     size_t lds = (size_t)(3 * MRM_NV + 5) * G.n0 * sizeof(float);
-    if (d == 0 && G.n0 <= TX_MAXN && lds <= 64 * 1024) {
+    if (d == 0 && tvd_x_lines_fit(c, lds)) {
       unsigned bsz = (unsigned)((G.n0 + 63) / 64 * 64);
       e = mixed::launch_tvd_x_lines(c->stream, (unsigned)nl, bsz, lds, G, Q, M, Pr);
     } else {

@@ -1,6 +1,7 @@
 // mrm_types.h -- parameter blocks shared by the host code and the kernels of
 // the MHD step (plain structs passed by value to kernels).
 #pragma once
+#include "../../include/muram_step.h"
 
 #define MRM_NV 8
 #define RED_CMAX 0
@@ -77,6 +78,7 @@ struct TvdParams {
   const double* hft;
   const double* qft;
   const double* tau;   // NULL: Tau == 0 everywhere
+  unsigned long long* bcount;   // limiter branch slots (counting instantiations), MRM_BR_NSLOT
 };
 
 struct ResidParams {

@@ -146,6 +146,17 @@ class MhdGrid:
         """tvar1..tvar8, Qres, Qvis (grid.H:69-100) by name, each [nz, ny, nx]"""
         return {n: self.get_field(abi.TVAR1 + t) for t, n in enumerate(abi.DIAG_NAMES)}
 
+    def branch_count(self, on: bool = True):
+        """limiter branch accounting on (zeroed) or off"""
+        self._chk(self.lib.mrm_branch_count(self.h, 1 if on else 0), "mrm_branch_count")
+
+    def branch_counts(self) -> np.ndarray:
+        """the MRM_BR_NSLOT count and checksum slots (uint64)"""
+        out = np.zeros(abi.BR_NSLOT, dtype=np.uint64)
+        self._chk(self.lib.mrm_get_branch_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_ulonglong))),
+                  "mrm_get_branch_counts")
+        return out
+
     def SetBoundaryConditions(self, run, stage: int):
         self._chk(self.lib.mrm_set_boundary_conditions(self.h, C.byref(run), stage),
                   "SetBoundaryConditions")

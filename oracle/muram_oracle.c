@@ -75,7 +75,38 @@ struct orc_ctx {
   /* diagnostics tvar1..8, Qres, Qvis (grid.C:319-329, grid.H:69-100), allocated at the
      first orc_diag_zero or first call with need_diagnostics */
   double* diag[10];
+  /* limiter branch accounting (orc_branch_count): MRM_BR_NSLOT slots */
+  int bcount_on;
+  unsigned long long bcount[MRM_BR_NSLOT];
 };
+
+/* ---- limiter branch accounting: the same categories, position key and
+   weights as the library's counters (muram_step.h, MRM_BR_*); a test of the
+   checker, not of the reference, which keeps no such counts */
+static inline unsigned long long br_weight(unsigned long long x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return (x ^ (x >> 31)) | 1ull;
+}
+typedef struct { unsigned long long n[MRM_BR_NCOUNT], h[MRM_BK_N]; } br_acc;
+static inline void br_hit(br_acc* a, int c, int kind, unsigned long long key, int taken) {
+  if (taken) { a->n[c]++; a->h[kind] += br_weight(key); }
+}
+static inline void br_cat(br_acc* a, int c0, int kind, unsigned long long key, int cat) {
+  a->n[c0 + cat]++;
+  a->h[kind] += br_weight(key) * (unsigned long long)(cat + 1);
+}
+static inline unsigned long long br_key(int d, long bs, long node) {
+  return ((unsigned long long)d * (unsigned long long)bs + (unsigned long long)node) << 3;
+}
+void orc_branch_count(orc_ctx* c, int on) {
+  if (on) memset(c->bcount, 0, sizeof c->bcount);
+  c->bcount_on = on;
+}
+void orc_get_branch_counts(const orc_ctx* c, unsigned long long out[MRM_BR_NSLOT]) {
+  memcpy(out, c->bcount, sizeof c->bcount);
+}
 
 void orc_set_threads(int n) {
 #ifdef _OPENMP
@@ -902,6 +933,8 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
     const int kbeg = bounds[d3][0], kend = bounds[d3][1], jbeg = bounds[d2][0], jend = bounds[d2][1];
 #pragma omp parallel
     {
+      br_acc bc;
+      memset(&bc, 0, sizeof bc);
       const int NA = 3 * NV + 16;
       double* mem = (double*)malloc(sizeof(double) * (size_t)sz * NA);
       double* var = mem;
@@ -917,6 +950,9 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
       for (int k = kbeg; k <= kend; k++)
         for (int j = jbeg; j <= jend; j++) {
           long offset = j * strided2 + k * strided3;
+          /* branch accounting on lines with interior transverse indices */
+          const int cnt = c->bcount_on && j >= c->lbeg[d2] && j <= c->lend[d2] && k >= c->lbeg[d3] &&
+                          k <= c->lend[d3];
           for (int i = 0; i < sz; i++) {
             long node = offset + i * str;
             var[i] = smax(rho_min, U[node]);
@@ -947,6 +983,10 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             double cs2 = 5.0 / 3.0 * Pg[node] * dn;
             double cfast = sqrt(va2 + cs2);
             double CME_mode = (1.0 - CZ[i]) * ((double)(vv >= CME_thresh * cfast));
+            if (cnt) {
+              br_hit(&bc, MRM_BR_CZ, MRM_BK_CZ, br_key(d1, bs, node), CZ[i] != 0.0);
+              br_hit(&bc, MRM_BR_CME, MRM_BK_CME, br_key(d1, bs, node), CME_mode != 0.0);
+            }
             dn = var[i] * var[i];
             double rf = smin(lf, dn / (dn + rho_lev));
             double hh = 1.0 - hfb1[i] - hft1[i];
@@ -989,6 +1029,20 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
               double lower = smin(smin(sl, sr), slm);
               double upper = smax(smax(sl, sr), slm);
               S[i] = smax(0.0, lower) + smin(0.0, upper);
+              if (cnt) {
+                /* which of sl, sr, slm the min (positive) or max (negative) took */
+                int kc;
+                if (lower > 0.0) {
+                  double m1 = (sr < sl) ? sr : sl;
+                  kc = (slm < m1) ? 2 : (sr < sl) ? 1 : 0;
+                } else if (upper < 0.0) {
+                  double m1 = (sl < sr) ? sr : sl;
+                  kc = (m1 < slm) ? 5 : (sl < sr) ? 4 : 3;
+                } else {
+                  kc = 6;
+                }
+                br_cat(&bc, MRM_BR_SLOPE_POS_L, MRM_BK_SLOPE, br_key(d1, bs, offset + i * str) + v, kc);
+              }
             }
           }
           if ((visc_slope_bot > 0.0) || (visc_slope_top > 0.0)) {
@@ -1003,11 +1057,15 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
               slp[5 * sz + i] *= cf; slp[6 * sz + i] *= cf; slp[7 * sz + i] *= cf;
             }
           }
-          for (int i = 1; i < sz - 1; i++)
+          for (int i = 1; i < sz - 1; i++) {
             if (qrho[i] > q_rho_max) slp[i] = 0.0;
+            if (cnt) br_hit(&bc, MRM_BR_QRHO, MRM_BK_QRHO, br_key(d1, bs, offset + i * str), qrho[i] > q_rho_max);
+          }
           if (vmax_lim > 0.0) {
-            for (int i = 1; i < sz - 1; i++)
+            for (int i = 1; i < sz - 1; i++) {
               if (vsqr[i] > vsqr_diff) { slp[sz + i] = 0.0; slp[2 * sz + i] = 0.0; slp[3 * sz + i] = 0.0; }
+              if (cnt) br_hit(&bc, MRM_BR_VFAST, MRM_BK_VFAST, br_key(d1, bs, offset + i * str), vsqr[i] > vsqr_diff);
+            }
           }
           for (int v = 0; v < NV; v++) {
             const double* X = var + v * sz;
@@ -1020,11 +1078,25 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
               double dn = H[i];
               double rf = fabs(sl_lim) / smax(1e-100, fabs(sl));
               double cf = smax(0.0, smin(sl_lim, sl)) + smin(0.0, smax(sl_lim, sl));
-              F[i] = smax(0.0, 1.0 + dn * (rf - 1.0)) * cf;
+              double cl = 1.0 + dn * (rf - 1.0);
+              F[i] = smax(0.0, cl) * cf;
+              if (cnt) {
+                /* the minmod's choice and the clip */
+                double mn = (sl < sl_lim) ? sl : sl_lim, mx = (sl_lim < sl) ? sl : sl_lim;
+                int kc = (mn > 0.0) ? ((sl < sl_lim) ? 1 : 0) : (mx < 0.0) ? ((sl_lim < sl) ? 3 : 2) : 4;
+                unsigned long long key = br_key(d1, bs, offset + i * str) + v;
+                br_cat(&bc, MRM_BR_MM_POS_LIM, MRM_BK_MM, key, kc);
+                br_hit(&bc, MRM_BR_CLIP, MRM_BK_CLIP, key, cl < 0.0);
+              }
             }
           }
           for (int i = 1; i < sz - 2; i++) {
             double tvd_faci = 0.5 * idxd1 * smin(cmax, smax(cm[i], cm[i + 1]));
+            if (cnt) {
+              double m = (cm[i] < cm[i + 1]) ? cm[i + 1] : cm[i];
+              br_cat(&bc, MRM_BR_CMAX_LIMIT, MRM_BK_CMAX, br_key(d1, bs, offset + i * str),
+                     (cmax < m) ? 0 : (cm[i] < cm[i + 1]) ? 2 : 1);
+            }
             for (int v = 0; v < NV; v++) flx[v * sz + i] *= tvd_faci;
             flx[(5 + d1) * sz + i] *= B_par_diff;
           }
@@ -1105,10 +1177,13 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             double qres = 0.5 * (qrho[i] + qrho[i - 1]) * qft1[i];
             r[4] += dt_tvd * qres;
             for (int v = 0; v < NV; v++) U[v * bs + node] += r[v];
+            if (cnt) br_hit(&bc, MRM_BR_RHO_FLOOR, MRM_BK_RHO_FLOOR, br_key(d1, bs, node), U[node] < rho_min);
             U[node] = smax(rho_min, U[node]);
             double mx = U[bs + node], my = U[2 * bs + node], mz = U[3 * bs + node];
             const double e_pre = U[4 * bs + node];
-            U[4 * bs + node] = smax(U[4 * bs + node], eps_min * U[node] + 0.5 * (mx * mx + my * my + mz * mz) / U[node]);
+            const double efl = eps_min * U[node] + 0.5 * (mx * mx + my * my + mz * mz) / U[node];
+            if (cnt) br_hit(&bc, MRM_BR_E_FLOOR, MRM_BK_E_FLOOR, br_key(d1, bs, node), e_pre < efl);
+            U[4 * bs + node] = smax(U[4 * bs + node], efl);
             if (dg) {
               dg[DG_T6][node] += boris3 * idt_full;
               dg[DG_QRES][node] += dt_fac * qres;
@@ -1118,6 +1193,13 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
           }
         }
       free(mem);
+      if (c->bcount_on) {
+#pragma omp critical(orc_bcount)
+        {
+          for (int q = 0; q < MRM_BR_NCOUNT; q++) c->bcount[q] += bc.n[q];
+          for (int q = 0; q < MRM_BK_N; q++) c->bcount[MRM_BR_NCOUNT + q] += bc.h[q];
+        }
+      }
     }
     bounds[d1][0] = c->lbeg[d1];
     bounds[d1][1] = c->lend[d1];

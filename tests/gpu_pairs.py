@@ -1,0 +1,89 @@
+"""Helpers of the GPU parity tests: one seeded state stepped by the oracle
+and by the library (through the C ABI) side by side, with the limiter
+branch accounting on in both, and the comparison to the tests' bar."""
+from __future__ import annotations
+
+import contextlib
+import os
+
+import numpy as np
+
+from muram_main_amd import abi, config, eos_table, initial
+from muram_main_amd.mhd import MhdGrid
+from orc import Oracle, relerr
+
+TOL = 1e-12
+
+
+@contextlib.contextmanager
+def env(**kw):
+    """set MRM_* knobs (read at context creation) for the duration"""
+    saved = {k: os.environ.get(k) for k in kw}
+    for k, v in kw.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+    try:
+        yield
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def make_box_state(table, box):
+    tab = eos_table.load_tables(table)
+    return initial.make_state(tab, box)
+
+
+def start_oracle(table, box, phys, U, cfl=0.8, count=True):
+    g = config.grid_desc(box.gsize, box.dx)
+    o = Oracle(g, phys, table)
+    ro = config.run_desc(CFL=cfl)
+    o.set_state(U)
+    o.exchange_grid()
+    o.set_boundary_conditions(ro, 0)
+    if count:
+        o.branch_count(1)
+    return o, ro
+
+
+def start_gpu(table, box, phys, U, cfl=0.8, path="fused", fma="1", tvd_x=None, precision=abi.PREC_FP64,
+              count=True):
+    g = config.grid_desc(box.gsize, box.dx)
+    with env(MRM_STEP_PATH=path, MRM_FMA=fma, MRM_TVD_X=tvd_x):
+        m = MhdGrid(g, phys, precision)
+    m.eos_init(table)
+    r = config.run_desc(CFL=cfl)
+    m.set_state(U)
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
+    if count and precision == abi.PREC_FP64:
+        m.branch_count(True)
+    return m, r
+
+
+def field_errs(Ug, Uo):
+    return [relerr(Ug[v], Uo[v]) for v in range(8)]
+
+
+def check_pair(m, r, o, ro, tol=TOL, counts=True):
+    """fields to tol (max|gpu-ref|/max|ref| per field over the buffer), dt and
+    time to tol, and every branch count and checksum equal"""
+    errs = field_errs(m.get_state(), o.state(0))
+    print("per-field rel err:", ["%.2e" % e for e in errs], "dt", r.dt, ro.dt, flush=True)
+    assert max(errs) <= tol, errs
+    assert abs(r.dt - ro.dt) <= tol * abs(ro.dt)
+    assert abs(r.time - ro.time) <= tol * abs(ro.time)
+    if counts:
+        bg, bo = m.branch_counts(), o.branch_counts()
+        mis = abi.branch_mismatch(bg, bo)
+        print("branch mismatch:", mis, flush=True)
+        if mis["slots_differing"] or mis["checksums_differing"]:
+            print("gpu:", abi.branch_dict(bg), "\noracle:", abi.branch_dict(bo), flush=True)
+        assert mis == {"slots_differing": 0, "count_diff": 0, "checksums_differing": 0}
+        assert int(np.sum(bo[:len(abi.BRANCH_COUNTS)])) > 0
+    return errs

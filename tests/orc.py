@@ -10,7 +10,7 @@ import tempfile
 
 import numpy as np
 
-from muram_main_amd.abi import GridDesc, PhysDesc, RunDesc
+from muram_main_amd.abi import BR_NSLOT, GridDesc, PhysDesc, RunDesc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -65,6 +65,8 @@ class Oracle:
             L.orc_diag_zero.argtypes = [P]
             L.orc_diag.argtypes = [P, C.c_int]
             L.orc_diag.restype = dp
+            L.orc_branch_count.argtypes = [P, C.c_int]
+            L.orc_get_branch_counts.argtypes = [P, C.POINTER(C.c_ulonglong)]
             for n in ("orc_p_interp", "orc_s_interp", "orc_T_interp", "orc_d3_interp", "orc_eps3_interp"):
                 getattr(L, n).argtypes = [P, C.c_double, C.c_double]
                 getattr(L, n).restype = C.c_double
@@ -124,6 +126,12 @@ class Oracle:
             p = self.L.orc_diag(self.h, t)
             if p:
                 out[t] = np.ctypeslib.as_array(p, shape=(self.bs,)).reshape(self.shape)
+        return out
+
+    def branch_counts(self):
+        """the MRM_BR_NSLOT count and checksum slots (uint64)"""
+        out = np.zeros(BR_NSLOT, dtype=np.uint64)
+        self.L.orc_get_branch_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_ulonglong)))
         return out
 
     def clean_divb(self, run):

@@ -1,0 +1,80 @@
+"""GPU parity at the benchmark workloads' own sizes, with the generated
+512 x 512 EOS table the benchmark uses:
+
+* box64: 64^3 interior, dx = 32 km, one RK step;
+* box512: 256 x 512 x 512 interior, dx = 16 km, ten RK steps with the
+  Alfven limiter at 3000 km/s, on the drop-in sequence (the operations the
+  reference's solver.C:109-365 calls, one C-ABI call each) and on the fused
+  step with FMA contraction (mrm_step's default).
+
+Bar: every field within 1e-12 relative (max|gpu - oracle| / max|oracle|)
+of the oracle after the steps, the reduced dt equal to 1e-12 at every step,
+and every limiter branch count and position checksum equal (the choices of
+tvdlimit_SR.C:281-657 over every counted point, slope and interface of the
+ten steps).  The box512 oracle run takes about a minute on 16 host threads.
+"""
+import numpy as np
+import pytest
+
+from muram_main_amd import abi, config, eos_table
+from gpu_pairs import TOL, check_pair, start_gpu, start_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def table():
+    spec = eos_table.TableSpec(n_eps=512, n_lr=512, n_lp3=200, n_s3=150)
+    return eos_table.generate(spec)
+
+
+def state_of(table, wl):
+    from muram_main_amd import initial
+    return initial.make_state(eos_table.load_tables(table), wl.box)
+
+
+@pytest.mark.parametrize("path,fma", [("fused", "1"), ("fused", "0"), ("sequence", "0")])
+def test_box64_one_step(table, path, fma):
+    print(f"case: box64 one step, {path} fma={fma}", flush=True)
+    wl = config.workloads()["box64"]
+    U = state_of(table, wl)
+    o, ro = start_oracle(table, wl.box, wl.phys, U)
+    m, r = start_gpu(table, wl.box, wl.phys, U, path=path, fma=fma)
+    o.step(ro, config.DT_RAD_OFF, 1)
+    m.step(r, config.DT_RAD_OFF, 1)
+    check_pair(m, r, o, ro)
+
+
+@pytest.fixture(scope="module")
+def box512_oracle(table):
+    """the oracle's ten steps of box512 from the seeded state: final state,
+    dt of every step, branch slots"""
+    import time
+    wl = config.workloads()["box512"]
+    t = time.perf_counter()
+    U = state_of(table, wl)
+    print(f"box512 state in {time.perf_counter() - t:.1f}s", flush=True)
+    o, ro = start_oracle(table, wl.box, wl.phys, U)
+    dts = []
+    for s in range(wl.steps):
+        t = time.perf_counter()
+        o.step(ro, config.DT_RAD_OFF, 1)
+        dts.append(ro.dt)
+        print(f"oracle step {s + 1}: dt {ro.dt:.6e} in {time.perf_counter() - t:.1f}s", flush=True)
+    return dict(U0=U, o=o, ro=ro, dts=dts, wl=wl)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("path,fma", [("sequence", "0"), ("fused", "1")])
+def test_box512_ten_steps(table, box512_oracle, path, fma):
+    print(f"case: box512 ten steps, {path} fma={fma}", flush=True)
+    B = box512_oracle
+    wl = B["wl"]
+    m, r = start_gpu(table, wl.box, wl.phys, B["U0"], path=path, fma=fma)
+    for s, dt_o in enumerate(B["dts"]):
+        m.step(r, config.DT_RAD_OFF, 1)
+        print(f"gpu step {s + 1}: dt {r.dt:.6e} (oracle {dt_o:.6e})", flush=True)
+        assert abs(r.dt - dt_o) <= TOL * dt_o
+    errs = check_pair(m, r, B["o"], B["ro"])
+    m.close()
+    assert max(errs) <= TOL

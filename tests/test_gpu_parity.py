@@ -140,11 +140,15 @@ def test_recorded_res_zero(gold, table):
 @pytest.mark.parametrize("periodic", [True, False])
 def test_tvdlimit_ghost_planes(table, periodic):
     """The drop-in TVDlimit sweeps the interior dim-2 planes and copies the
-    ghost planes when they are exact periodic copies (checked on the device),
-    and sweeps every plane otherwise: both against the oracle, which sweeps
-    every plane, on a 12x10x14 box whose dim-2 ghost planes are, or are not,
-    the exchange's copies."""
+    ghost planes when they are exact periodic copies (checked on the device,
+    bit for bit, at every i including the vertical ghosts), and sweeps every
+    plane otherwise: both against the oracle, which sweeps every plane, on a
+    12x10x14 box whose ghosts come from exchange_grid + SetBoundaryConditions
+    (so the dim-2 ghost planes are exact copies everywhere), with or without
+    a perturbed ghost plane.  Which path ran is read from the launch count of
+    the exchange timer: the ghost-plane copy is one exchange launch."""
     print(f"case: drop-in TVDlimit, dim-2 ghost planes periodic={periodic}", flush=True)
+    import ctypes as CT
     tab = eos_table.load_tables(table)
     box = initial.BoxSpec(gsize=(12, 10, 14), dx=(3.2e6,) * 3, seed=21, v_rms=2.0e5)
     U = initial.make_state(tab, box)
@@ -152,10 +156,11 @@ def test_tvdlimit_ghost_planes(table, periodic):
     r = config.run_desc(CFL=0.8)
     m = make(g, p, table)
     o = Oracle(g, p, table)
+    ro = config.run_desc(CFL=0.8)
     for x in (m, o):
         x.set_state(U)
     m.exchange_grid()
-    o.exchange_grid()
+    m.SetBoundaryConditions(r, 0)
     V = m.get_state()
     if not periodic:
         # perturb a dim-2 ghost plane at each end (arrays are [v, k, j, i])
@@ -163,16 +168,25 @@ def test_tvdlimit_ghost_planes(table, periodic):
         V[4, -1] *= 1.0 - 1e-3
     m.set_state(V)
     o.set_state(V)
+    o.set_bc(*m.bc())
     # the reference's order: ConsToPrim and Adjust_Valf_Max (pressure and
     # limiter state), then TVDlimit on the conservative state
     m.ConsToPrim()
     o.cons_to_prim()
     m.Adjust_Valf_Max(r)
-    o.adjust_valf_max(r, None)
+    o.adjust_valf_max(ro, None)
     m.set_state(V)
     o.set_state(V)
+    m.lib.mrm_timers(m.h, 1)
     m.TVDlimit(r, 1.0)
-    o.tvdlimit(C.byref(r), 1.0)
+    ms = (CT.c_double * abi.NTIMER)()
+    cnt = (CT.c_long * abi.NTIMER)()
+    m.lib.mrm_get_timers(m.h, ms, cnt)
+    m.lib.mrm_timers(m.h, 0)
+    names = [m.lib.mrm_timer_name(i).decode() for i in range(abi.NTIMER)]
+    copies = cnt[names.index("exchange")]
+    assert copies == (1 if periodic else 0)
+    o.tvdlimit(C.byref(ro), 1.0)
     assert maxrel(m.get_state(), o.state(0)) <= TOL
 
 
