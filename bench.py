@@ -103,7 +103,7 @@ def oracle_run(table, phys, box, steps, threads, count=False):
     o.exchange_grid()
     o.set_boundary_conditions(r, 0)
     if count:
-        o.branch_count(1)
+        o.branch_count(1, abi.BRANCH_TIE)
     dts = []
     o.step(r, config.DT_RAD_OFF, 1)          # warm-up
     dts.append(r.dt)
@@ -128,7 +128,7 @@ def gpu_parity(table, wl, o, ro, dts, path):
     finally:
         os.environ["MRM_STEP_PATH"] = "fused"
     del U
-    m.branch_count(True)
+    m.branch_count(True, abi.BRANCH_TIE)
     ddt = 0.0
     for dt_o in dts:
         m.step(r, config.DT_RAD_OFF, 1)
@@ -137,15 +137,17 @@ def gpu_parity(table, wl, o, ro, dts, path):
     bg = m.branch_counts()
     m.close()
     B = o.state(0)
-    errs = {}
+    errs, l2 = {}, {}
     for v, name in enumerate(FIELDS):
-        d = np.abs(A[v] - B[v])
-        errs[name] = float(f"{float(np.max(d) / max(np.max(np.abs(B[v])), 1e-300)):.3e}")
+        d = A[v] - B[v]
+        errs[name] = float(f"{float(np.max(np.abs(d)) / max(np.max(np.abs(B[v])), 1e-300)):.3e}")
+        l2[name] = float(f"{float(np.sqrt(np.sum(d * d)) / max(np.sqrt(np.sum(B[v] * B[v])), 1e-300)):.3e}")
     del A
     bo = o.branch_counts()
     return {"path": path, "steps": len(dts), "field_rel_err_max": errs, "max_rel_err": max(errs.values()),
+            "field_rel_err_l2": l2, "max_rel_err_l2": max(l2.values()),
             "dt_rel_diff_max": float(f"{ddt:.3e}"), "branch_mismatch": abi.branch_mismatch(bg, bo),
-            "branch_counts": abi.branch_dict(bo)["counts"]}
+            "branch_tie": abi.BRANCH_TIE, "branch_counts": abi.branch_dict(bo)["counts"]}
 
 
 def cpu_baseline(table, wl, steps=3, parity_paths=("fused", "sequence")):

@@ -281,11 +281,27 @@ const char* mrm_timer_name(int id);
  *   CMAX_*    min(cmax, max(cm_i, cm_i+1)): cmax, cm_i or cm_i+1 (:457)
  *   RHO_FLOOR max(rho_min, rho) raised rho after the update (:653)
  *   E_FLOOR   the energy floor raised e after the update   (:656)
- * mrm_branch_count(ctx, 1) zeroes the slots and turns the accounting on
+ * Near ties: every evaluation has a margin (the distance of the deciding
+ * quantity from the decision boundary: |lower| and the gap between the two
+ * smallest of sl, sr, slm for a positive slope, |x - x0| for a switch x > x0,
+ * ...) and a scale (the magnitude of its operands: the variable's values at
+ * the points involved, the threshold, ...).  An evaluation whose margin is
+ * below tie x scale is a near tie: either choice gives the same value to
+ * within tie of the scale, and two implementations whose inputs differ in
+ * the last bits may choose differently.  A margin of exactly zero is an
+ * exact tie (identical operands: copies, mirrors, a limited slope equal to
+ * the difference it came from -- or values that happen to round alike).
+ * Near ties and exact ties are counted per branch point in slots
+ * MRM_BR_NCOUNT + MRM_BK_N + kind and MRM_BR_NCOUNT + 2 MRM_BK_N + kind and
+ * left out of the counts and checksums, which then hold the decisive choices
+ * only (tie = 0: no tie accounting, every evaluation is decisive).  The
+ * clip of an interface whose difference sl is itself tied with zero is tied
+ * with it (its ratio |sl_lim|/|sl| is then rounding noise).
+ * mrm_branch_count(ctx, 1, tie) zeroes the slots and turns the accounting on
  * (the TVD sweeps then run counting instantiations of the same kernels);
- * mrm_branch_count(ctx, 0) turns it off; mrm_get_branch_counts synchronizes
- * and copies the MRM_BR_NSLOT slots.  The diagnostics path (an output step's
- * collecting sweeps) is not counted. */
+ * mrm_branch_count(ctx, 0, 0) turns it off; mrm_get_branch_counts
+ * synchronizes and copies the MRM_BR_NSLOT slots.  The diagnostics path (an
+ * output step's collecting sweeps) is not counted. */
 enum {
   MRM_BR_CZ = 0, MRM_BR_CME,
   MRM_BR_SLOPE_POS_L, MRM_BR_SLOPE_POS_R, MRM_BR_SLOPE_POS_M,
@@ -302,8 +318,8 @@ enum {
   MRM_BK_CZ = 0, MRM_BK_CME, MRM_BK_SLOPE, MRM_BK_QRHO, MRM_BK_VFAST, MRM_BK_MM, MRM_BK_CLIP,
   MRM_BK_CMAX, MRM_BK_RHO_FLOOR, MRM_BK_E_FLOOR, MRM_BK_N
 };
-#define MRM_BR_NSLOT (MRM_BR_NCOUNT + MRM_BK_N)
-int mrm_branch_count(mrm_ctx* ctx, int on);
+#define MRM_BR_NSLOT (MRM_BR_NCOUNT + 3 * MRM_BK_N)
+int mrm_branch_count(mrm_ctx* ctx, int on, double tie);
 int mrm_get_branch_counts(mrm_ctx* ctx, unsigned long long out[MRM_BR_NSLOT]);
 
 #ifdef __cplusplus

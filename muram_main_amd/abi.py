@@ -46,23 +46,38 @@ BRANCH_COUNTS = ("cz", "cme", "slope_pos_l", "slope_pos_r", "slope_pos_m", "slop
                  "mm_neg_sl", "mm_zero", "clip", "cmax_limit", "cmax_left", "cmax_right", "rho_floor",
                  "e_floor")
 BRANCH_SUMS = ("cz", "cme", "slope", "qrho", "vfast", "minmod", "clip", "cmax", "rho_floor", "e_floor")
-BR_NSLOT = len(BRANCH_COUNTS) + len(BRANCH_SUMS)
+BR_NSLOT = len(BRANCH_COUNTS) + 3 * len(BRANCH_SUMS)
+# near-tie margin of the parity checks: choices within 1e-10 of their
+# operands' scale of the decision boundary
+BRANCH_TIE = 1e-10
 
 
 def branch_dict(slots):
-    """{'counts': {name: n}, 'checksums': {name: hex}} of MRM_BR_NSLOT slots"""
-    n = len(BRANCH_COUNTS)
-    return {"counts": {k: int(slots[i]) for i, k in enumerate(BRANCH_COUNTS)},
-            "checksums": {k: f"{int(slots[n + i]):016x}" for i, k in enumerate(BRANCH_SUMS)}}
+    """{'counts': {name: n}, 'checksums': {name: hex}, 'near_ties': {name: n},
+    'exact_ties': {name: n}} of MRM_BR_NSLOT slots"""
+    n, k = len(BRANCH_COUNTS), len(BRANCH_SUMS)
+    return {"counts": {c: int(slots[i]) for i, c in enumerate(BRANCH_COUNTS)},
+            "checksums": {c: f"{int(slots[n + i]):016x}" for i, c in enumerate(BRANCH_SUMS)},
+            "near_ties": {c: int(slots[n + k + i]) for i, c in enumerate(BRANCH_SUMS)},
+            "exact_ties": {c: int(slots[n + 2 * k + i]) for i, c in enumerate(BRANCH_SUMS)}}
 
 
 def branch_mismatch(a, b):
-    """number of count slots that differ, summed absolute count difference and
-    number of differing checksums between two slot arrays"""
-    n = len(BRANCH_COUNTS)
+    """differences between two slot arrays: of the decisive choices (count
+    slots that differ, summed absolute count difference, checksums that
+    differ) and of the ties (summed absolute differences of the near-tie and
+    exact-tie counts, and b's totals of each)"""
+    n, k = len(BRANCH_COUNTS), len(BRANCH_SUMS)
     diff = [abs(int(a[i]) - int(b[i])) for i in range(n)]
+    near = range(n + k, n + 2 * k)
+    exact = range(n + 2 * k, n + 3 * k)
     return {"slots_differing": sum(1 for d in diff if d), "count_diff": sum(diff),
-            "checksums_differing": sum(1 for i in range(n, BR_NSLOT) if int(a[i]) != int(b[i]))}
+            "checksums_differing": sum(1 for i in range(n, n + k) if int(a[i]) != int(b[i])),
+            "near_tie_diff": sum(abs(int(a[i]) - int(b[i])) for i in near),
+            "near_ties": sum(int(b[i]) for i in near),
+            "exact_tie_diff": sum(abs(int(a[i]) - int(b[i])) for i in exact),
+            "exact_ties": sum(int(b[i]) for i in exact),
+            "decisive": sum(int(b[i]) for i in range(n))}
 
 
 class GridDesc(C.Structure):
@@ -145,7 +160,7 @@ def load(path: str = LIB_PATH):
         "mrm_timers": ([P, C.c_int], C.c_int),
         "mrm_get_timers": ([P, dp, C.POINTER(C.c_long)], C.c_int),
         "mrm_timer_name": ([C.c_int], C.c_char_p),
-        "mrm_branch_count": ([P, C.c_int], C.c_int),
+        "mrm_branch_count": ([P, C.c_int, C.c_double], C.c_int),
         "mrm_get_branch_counts": ([P, C.POINTER(C.c_ulonglong)], C.c_int),
     }
     for name, (args, res) in sig.items():

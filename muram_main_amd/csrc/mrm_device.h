@@ -539,27 +539,46 @@ MRM_HD unsigned long long br_weight(unsigned long long x) {
 }
 struct TvdCnt {
   unsigned n[MRM_BR_NCOUNT];
+  unsigned nnear[MRM_BK_N];
+  unsigned nexact[MRM_BK_N];
   unsigned long long h[MRM_BK_N];
   unsigned long long key;
+  double tie;         // near-tie margin relative to the scale
   bool on;            // the line being swept is counted
-  MRM_HD void clear() {
+  MRM_HD void clear(double tie_) {
     for (int i = 0; i < MRM_BR_NCOUNT; i++) n[i] = 0;
-    for (int i = 0; i < MRM_BK_N; i++) h[i] = 0;
+    for (int i = 0; i < MRM_BK_N; i++) { h[i] = 0; nnear[i] = 0; nexact[i] = 0; }
     key = 0;
+    tie = tie_;
     on = false;
   }
+  // 0: decisive, 1: near tie (0 < margin < tie x scale), 2: exact tie
+  // (margin 0); with tie = 0 every evaluation is decisive
+  MRM_HD int cls(double margin, double scale) const {
+    return !(tie > 0.0) ? 0 : margin == 0.0 ? 2 : (margin < tie * fabs(scale) ? 1 : 0);
+  }
+  // ties are counted as such and nothing else
+  MRM_HD bool tied(int kind, int c) {
+    nnear[kind] += (on && c == 1) ? 1u : 0u;
+    nexact[kind] += (on && c == 2) ? 1u : 0u;
+    return c != 0;
+  }
   // a switch: counted and weighted when taken
-  MRM_HD void hit(int c, int kind, int q, bool taken) {
+  MRM_HD void hit(int c, int kind, int q, bool taken, int tc) {
+    if (tied(kind, tc)) return;
     taken = taken && on;
     n[c] += taken ? 1u : 0u;
     h[kind] += taken ? br_weight(key + q) : 0ull;
   }
   // one of ncat choices (cat = 0 .. ncat-1, slots c0 ..)
-  MRM_HD void cat(int c0, int ncat, int kind, int q, int cat) {
+  MRM_HD void cat(int c0, int ncat, int kind, int q, int cat, int tc) {
+    if (tied(kind, tc)) return;
     for (int i = 0; i < ncat; i++) n[c0 + i] += (on && cat == i) ? 1u : 0u;
     h[kind] += on ? br_weight(key + q) * (unsigned long long)(cat + 1) : 0ull;
   }
 };
+
+MRM_HD double fmax3_(double a, double b, double c) { return fmax(fmax(a, b), c); }
 
 // per-point quantities (tvdlimit_SR.C:269-385)
 MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, double tau,
@@ -588,8 +607,10 @@ MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, d
   double cfast = sqrt(va2 + cs2);
   double CME_mode = (1.0 - o.CZ) * ((double)(vv >= P.CME_thresh * cfast));
   if (cn) {
-    cn->hit(MRM_BR_CZ, MRM_BK_CZ, 0, o.CZ != 0.0);
-    cn->hit(MRM_BR_CME, MRM_BK_CME, 0, CME_mode != 0.0);
+    cn->hit(MRM_BR_CZ, MRM_BK_CZ, 0, o.CZ != 0.0, cn->cls(fabs(tau - 1.0e-5), 1.0e-5));
+    const double ct = P.CME_thresh * cfast;
+    // with CZ = 1 the switch is off whatever vv is
+    cn->hit(MRM_BR_CME, MRM_BK_CME, 0, CME_mode != 0.0, o.CZ != 0.0 ? 0 : cn->cls(fabs(vv - ct), fmax(vv, ct)));
   }
   dn = v0 * v0;
   double rf = smin(lf, dn / (dn + P.rho_lev));
@@ -634,18 +655,28 @@ MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const 
     double upper = smax(smax(sl, sr), slm);
     S[q] = smax(0.0, lower) + smin(0.0, upper);
     if (cn) {
-      // which of sl, sr, slm the min (positive) or max (negative) took
+      // which of sl, sr, slm the min (positive) or max (negative) took;
+      // margin: the sign's distance from zero and the gap to the runner-up
       int k;
+      double mg;
       if (lower > 0.0) {
         const double m1 = (sr < sl) ? sr : sl;
         k = (slm < m1) ? 2 : (sr < sl) ? 1 : 0;
+        const double x0 = k == 0 ? sl : k == 1 ? sr : slm;
+        const double gap = fmin(k == 0 ? 1e300 : sl - x0, fmin(k == 1 ? 1e300 : sr - x0, k == 2 ? 1e300 : slm - x0));
+        mg = fmin(lower, gap);
       } else if (upper < 0.0) {
         const double m1 = (sl < sr) ? sr : sl;
         k = (m1 < slm) ? 5 : (sl < sr) ? 4 : 3;
+        const double x0 = k == 3 ? sl : k == 4 ? sr : slm;
+        const double gap = fmin(k == 3 ? 1e300 : x0 - sl, fmin(k == 4 ? 1e300 : x0 - sr, k == 5 ? 1e300 : x0 - slm));
+        mg = fmin(-upper, gap);
       } else {
         k = 6;
+        mg = fmin(fabs(lower), fabs(upper));
       }
-      cn->cat(MRM_BR_SLOPE_POS_L, 7, MRM_BK_SLOPE, q, k);
+      cn->cat(MRM_BR_SLOPE_POS_L, 7, MRM_BK_SLOPE, q, k,
+              cn->cls(mg, fmax3_(fabs(a.v[q]), fabs(b.v[q]), fabs(c.v[q]))));
     }
   }
   if ((P.visc_slope_bot > 0.0) || (P.visc_slope_top > 0.0)) {
@@ -658,10 +689,12 @@ MRM_HD void tvd_slope(const TvdParams& P, const TvdPt& a, const TvdPt& b, const 
   }
   double qrho = fabs(b.lnr - c.lnr);
   if (qrho > P.q_rho_max) S[0] = 0.0;
-  if (cn) cn->hit(MRM_BR_QRHO, MRM_BK_QRHO, 0, qrho > P.q_rho_max);
+  if (cn) cn->hit(MRM_BR_QRHO, MRM_BK_QRHO, 0, qrho > P.q_rho_max,
+                  cn->cls(fabs(qrho - P.q_rho_max), fmax(fabs(b.lnr), fabs(c.lnr))));
   if (P.vmax_lim > 0.0) {
     if (b.vsqr > P.vsqr_diff) { S[1] = 0.0; S[2] = 0.0; S[3] = 0.0; }
-    if (cn) cn->hit(MRM_BR_VFAST, MRM_BK_VFAST, 0, b.vsqr > P.vsqr_diff);
+    if (cn)
+      cn->hit(MRM_BR_VFAST, MRM_BK_VFAST, 0, b.vsqr > P.vsqr_diff, cn->cls(fabs(b.vsqr - P.vsqr_diff), P.vsqr_diff));
   }
 }
 
@@ -689,14 +722,23 @@ MRM_HD double tvd_iface(const TvdParams& P, const TvdPt& a, const TvdPt& b, cons
       // negative (the larger), or zero
       const double mn = (sl < sl_lim) ? sl : sl_lim, mx = (sl_lim < sl) ? sl : sl_lim;
       const int k = (mn > 0.0) ? ((sl < sl_lim) ? 1 : 0) : (mx < 0.0) ? ((sl_lim < sl) ? 3 : 2) : 4;
-      cn->cat(MRM_BR_MM_POS_LIM, 5, MRM_BK_MM, q, k);
-      cn->hit(MRM_BR_CLIP, MRM_BK_CLIP, q, cl < 0.0);
+      // margin: the distance of the nearer of sl_lim, sl from zero (sl_lim
+      // lies between 0 and sl, so which of the two the minmod takes is fixed
+      // by the slopes' own choices)
+      const double sc = fmax(fabs(b.v[q]), fabs(c.v[q]));
+      cn->cat(MRM_BR_MM_POS_LIM, 5, MRM_BK_MM, q, k, cn->cls(fmin(fabs(mn), fabs(mx)), sc));
+      // the clip's ratio rf = |sl_lim| / |sl| is noise where sl is (near)
+      // zero: the clip is then tied with sl (the flux is ~0 either way)
+      const int tsl = cn->cls(fabs(sl), sc);
+      cn->hit(MRM_BR_CLIP, MRM_BK_CLIP, q, cl < 0.0, tsl ? tsl : cn->cls(fabs(cl), fmax(1.0, fabs(dn * (rf - 1.0)))));
     }
   }
   double tvd_faci = 0.5 * P.idxd1 * smin(P.cmax, smax(b.cm, c.cm));
   if (cn) {
     const double m = (b.cm < c.cm) ? c.cm : b.cm;
-    cn->cat(MRM_BR_CMAX_LIMIT, 3, MRM_BK_CMAX, 0, (P.cmax < m) ? 0 : (b.cm < c.cm) ? 2 : 1);
+    const int k = (P.cmax < m) ? 0 : (b.cm < c.cm) ? 2 : 1;
+    const double mg = k == 0 ? m - P.cmax : fmin(P.cmax - m, fabs(b.cm - c.cm));
+    cn->cat(MRM_BR_CMAX_LIMIT, 3, MRM_BK_CMAX, 0, k, cn->cls(mg, fmax3_(P.cmax, b.cm, c.cm)));
   }
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) F[q] *= tvd_faci;
@@ -762,7 +804,7 @@ MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_N
   r[4] += P.dt_tvd * qres;
 #pragma unroll
   for (int k = 0; k < MRM_NV; k++) u[k] += r[k];
-  if (cn) cn->hit(MRM_BR_RHO_FLOOR, MRM_BK_RHO_FLOOR, 0, u[0] < P.rho_min);
+  if (cn) cn->hit(MRM_BR_RHO_FLOOR, MRM_BK_RHO_FLOOR, 0, u[0] < P.rho_min, cn->cls(fabs(u[0] - P.rho_min), P.rho_min));
   u[0] = smax(P.rho_min, u[0]);
   if (DIAG) {
     dg[1] = boris3;
@@ -770,7 +812,7 @@ MRM_HD void tvd_update(const TvdParams& P, const TvdPt& b, const double Fm[MRM_N
     dg[3] = u[4];
   }
   const double efl = P.eps_min * u[0] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0];
-  if (cn) cn->hit(MRM_BR_E_FLOOR, MRM_BK_E_FLOOR, 0, u[4] < efl);
+  if (cn) cn->hit(MRM_BR_E_FLOOR, MRM_BK_E_FLOOR, 0, u[4] < efl, cn->cls(fabs(u[4] - efl), fmax(fabs(u[4]), fabs(efl))));
   u[4] = smax(u[4], efl);
 }
 

@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(MAXT) k_tvd_x_lines(TvdLineArgs A, double* __r
   const int t = threadIdx.x;
   TvdCnt cnt;
   if (CNT) {
-    cnt.clear();
+    cnt.clear(P.btie);
     cnt.on = j >= G.lb1 && j <= G.le1 && k >= G.lb2 && k <= G.le2;
     cnt.key = br_key(0, bs, base + t);
   }

@@ -577,6 +577,12 @@ __device__ inline void tvd_cnt_flush(const TvdCnt& c, unsigned long long* out) {
     unsigned long long v = c.h[i];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if (lead && v) atomicAdd(out + MRM_BR_NCOUNT + i, v);
+    unsigned long long w = c.nnear[i];
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+    if (lead && w) atomicAdd(out + MRM_BR_NCOUNT + MRM_BK_N + i, w);
+    unsigned long long x = c.nexact[i];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lead && x) atomicAdd(out + MRM_BR_NCOUNT + 2 * MRM_BK_N + i, x);
   }
 }
 
@@ -594,7 +600,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
   __shared__ double ring[3][MRM_NV][MRM_BLOCK];
   if (VALF) valf_lds_init(vacc + threadIdx.x, MRM_BLOCK);
   TvdCnt cnt;
-  if (CNT) cnt.clear();
+  if (CNT) cnt.clear(P.btie);
   TvdCnt* const cn = CNT ? &cnt : nullptr;
   const long bs = G.bs;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;

@@ -110,6 +110,7 @@ struct mrm_ctx {
   // limiter branch accounting (mrm_branch_count): MRM_BR_NSLOT slots
   unsigned long long* d_bcount = nullptr;
   bool bcount_on = false;
+  double btie = 0.0;
 
   unsigned long long* d_red = nullptr;   // RED_NSLOT slots
   double* d_part = nullptr;
@@ -1298,6 +1299,7 @@ static int tvd_params(mrm_ctx* c, double dt_tvd, TvdParams& P) {
   P.qft = c->d_prof + 2 * n0;
   P.tau = c->tau_set ? c->dF + MRM_TAU * c->G.bs : nullptr;
   P.bcount = c->bcount_on ? c->d_bcount : nullptr;
+  P.btie = c->btie;
   P.y_k0 = 0;        // every plane, ghosts included (tvdlimit_SR.C:199-205)
   P.y_nk = c->G.n2;
   return MRM_OK;
@@ -1418,8 +1420,9 @@ static int tvd_buf_diag(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, 
 // limiter branch accounting: the sweeps of tvd_buf run their counting
 // instantiations while it is on (the FP32-mixed sweeps and the diagnostics
 // sweeps are not counted)
-extern "C" int mrm_branch_count(mrm_ctx* c, int on) {
-  if (!c) return MRM_ERR_ARG;
+extern "C" int mrm_branch_count(mrm_ctx* c, int on, double tie) {
+  if (!c || !(tie >= 0.0)) return MRM_ERR_ARG;
+  c->btie = tie;
   if (on) {
     NEED_FP64(c, "mrm_branch_count");
     if (!c->d_bcount)

@@ -79,6 +79,7 @@ struct TvdParams {
   const double* qft;
   const double* tau;   // NULL: Tau == 0 everywhere
   unsigned long long* bcount;   // limiter branch slots (counting instantiations), MRM_BR_NSLOT
+  double btie;                  // near-tie margin relative to the operand scale
 };
 
 struct ResidParams {

@@ -146,9 +146,10 @@ class MhdGrid:
         """tvar1..tvar8, Qres, Qvis (grid.H:69-100) by name, each [nz, ny, nx]"""
         return {n: self.get_field(abi.TVAR1 + t) for t, n in enumerate(abi.DIAG_NAMES)}
 
-    def branch_count(self, on: bool = True):
-        """limiter branch accounting on (zeroed) or off"""
-        self._chk(self.lib.mrm_branch_count(self.h, 1 if on else 0), "mrm_branch_count")
+    def branch_count(self, on: bool = True, tie: float = 0.0):
+        """limiter branch accounting on (zeroed) or off; choices within tie
+        of their scale are counted as near ties"""
+        self._chk(self.lib.mrm_branch_count(self.h, 1 if on else 0, tie), "mrm_branch_count")
 
     def branch_counts(self) -> np.ndarray:
         """the MRM_BR_NSLOT count and checksum slots (uint64)"""

@@ -77,6 +77,7 @@ struct orc_ctx {
   double* diag[10];
   /* limiter branch accounting (orc_branch_count): MRM_BR_NSLOT slots */
   int bcount_on;
+  double btie;
   unsigned long long bcount[MRM_BR_NSLOT];
 };
 
@@ -89,20 +90,38 @@ static inline unsigned long long br_weight(unsigned long long x) {
   x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
   return (x ^ (x >> 31)) | 1ull;
 }
-typedef struct { unsigned long long n[MRM_BR_NCOUNT], h[MRM_BK_N]; } br_acc;
-static inline void br_hit(br_acc* a, int c, int kind, unsigned long long key, int taken) {
+typedef struct {
+  unsigned long long n[MRM_BR_NCOUNT], h[MRM_BK_N], nnear[MRM_BK_N], nexact[MRM_BK_N];
+  double tie;
+} br_acc;
+/* 0: decisive, 1: near tie (0 < margin < tie x scale), 2: exact tie (margin
+   0); with tie = 0 every evaluation is decisive */
+static inline int br_cls(const br_acc* a, double margin, double scale) {
+  return !(a->tie > 0.0) ? 0 : margin == 0.0 ? 2 : (margin < a->tie * fabs(scale) ? 1 : 0);
+}
+/* ties are counted as such and nothing else */
+static inline int br_tied(br_acc* a, int kind, int tc) {
+  if (tc == 1) a->nnear[kind]++;
+  if (tc == 2) a->nexact[kind]++;
+  return tc != 0;
+}
+static inline void br_hit(br_acc* a, int c, int kind, unsigned long long key, int taken, int tc) {
+  if (br_tied(a, kind, tc)) return;
   if (taken) { a->n[c]++; a->h[kind] += br_weight(key); }
 }
-static inline void br_cat(br_acc* a, int c0, int kind, unsigned long long key, int cat) {
+static inline void br_cat(br_acc* a, int c0, int kind, unsigned long long key, int cat, int tc) {
+  if (br_tied(a, kind, tc)) return;
   a->n[c0 + cat]++;
   a->h[kind] += br_weight(key) * (unsigned long long)(cat + 1);
 }
+static inline double fmax3(double a, double b, double c) { return fmax(fmax(a, b), c); }
 static inline unsigned long long br_key(int d, long bs, long node) {
   return ((unsigned long long)d * (unsigned long long)bs + (unsigned long long)node) << 3;
 }
-void orc_branch_count(orc_ctx* c, int on) {
+void orc_branch_count(orc_ctx* c, int on, double tie) {
   if (on) memset(c->bcount, 0, sizeof c->bcount);
   c->bcount_on = on;
+  c->btie = tie;
 }
 void orc_get_branch_counts(const orc_ctx* c, unsigned long long out[MRM_BR_NSLOT]) {
   memcpy(out, c->bcount, sizeof c->bcount);
@@ -935,6 +954,7 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
     {
       br_acc bc;
       memset(&bc, 0, sizeof bc);
+      bc.tie = c->btie;
       const int NA = 3 * NV + 16;
       double* mem = (double*)malloc(sizeof(double) * (size_t)sz * NA);
       double* var = mem;
@@ -984,8 +1004,11 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             double cfast = sqrt(va2 + cs2);
             double CME_mode = (1.0 - CZ[i]) * ((double)(vv >= CME_thresh * cfast));
             if (cnt) {
-              br_hit(&bc, MRM_BR_CZ, MRM_BK_CZ, br_key(d1, bs, node), CZ[i] != 0.0);
-              br_hit(&bc, MRM_BR_CME, MRM_BK_CME, br_key(d1, bs, node), CME_mode != 0.0);
+              br_hit(&bc, MRM_BR_CZ, MRM_BK_CZ, br_key(d1, bs, node), CZ[i] != 0.0,
+                     br_cls(&bc, fabs(Tau[node] - 1.0e-5), 1.0e-5));
+              const double ct = CME_thresh * cfast;
+              br_hit(&bc, MRM_BR_CME, MRM_BK_CME, br_key(d1, bs, node), CME_mode != 0.0,
+                     CZ[i] != 0.0 ? 0 : br_cls(&bc, fabs(vv - ct), fmax(vv, ct)));
             }
             dn = var[i] * var[i];
             double rf = smin(lf, dn / (dn + rho_lev));
@@ -1031,17 +1054,28 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
               S[i] = smax(0.0, lower) + smin(0.0, upper);
               if (cnt) {
                 /* which of sl, sr, slm the min (positive) or max (negative) took */
+                /* margin: the sign's distance from zero and the gap to the
+                   runner-up; scale: the variable at the three points */
                 int kc;
+                double mg;
                 if (lower > 0.0) {
                   double m1 = (sr < sl) ? sr : sl;
                   kc = (slm < m1) ? 2 : (sr < sl) ? 1 : 0;
+                  double x0 = kc == 0 ? sl : kc == 1 ? sr : slm;
+                  double gap = fmin(kc == 0 ? 1e300 : sl - x0, fmin(kc == 1 ? 1e300 : sr - x0, kc == 2 ? 1e300 : slm - x0));
+                  mg = fmin(lower, gap);
                 } else if (upper < 0.0) {
                   double m1 = (sl < sr) ? sr : sl;
                   kc = (m1 < slm) ? 5 : (sl < sr) ? 4 : 3;
+                  double x0 = kc == 3 ? sl : kc == 4 ? sr : slm;
+                  double gap = fmin(kc == 3 ? 1e300 : x0 - sl, fmin(kc == 4 ? 1e300 : x0 - sr, kc == 5 ? 1e300 : x0 - slm));
+                  mg = fmin(-upper, gap);
                 } else {
                   kc = 6;
+                  mg = fmin(fabs(lower), fabs(upper));
                 }
-                br_cat(&bc, MRM_BR_SLOPE_POS_L, MRM_BK_SLOPE, br_key(d1, bs, offset + i * str) + v, kc);
+                br_cat(&bc, MRM_BR_SLOPE_POS_L, MRM_BK_SLOPE, br_key(d1, bs, offset + i * str) + v, kc,
+                       br_cls(&bc, mg, fmax3(fabs(X[i - 1]), fabs(X[i]), fabs(X[i + 1]))));
               }
             }
           }
@@ -1059,12 +1093,18 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
           }
           for (int i = 1; i < sz - 1; i++) {
             if (qrho[i] > q_rho_max) slp[i] = 0.0;
-            if (cnt) br_hit(&bc, MRM_BR_QRHO, MRM_BK_QRHO, br_key(d1, bs, offset + i * str), qrho[i] > q_rho_max);
+            if (cnt) {
+              double lr0 = rho_log ? var[i] : log(var[i]), lr1 = rho_log ? var[i + 1] : log(var[i + 1]);
+              br_hit(&bc, MRM_BR_QRHO, MRM_BK_QRHO, br_key(d1, bs, offset + i * str), qrho[i] > q_rho_max,
+                     br_cls(&bc, fabs(qrho[i] - q_rho_max), fmax(fabs(lr0), fabs(lr1))));
+            }
           }
           if (vmax_lim > 0.0) {
             for (int i = 1; i < sz - 1; i++) {
               if (vsqr[i] > vsqr_diff) { slp[sz + i] = 0.0; slp[2 * sz + i] = 0.0; slp[3 * sz + i] = 0.0; }
-              if (cnt) br_hit(&bc, MRM_BR_VFAST, MRM_BK_VFAST, br_key(d1, bs, offset + i * str), vsqr[i] > vsqr_diff);
+              if (cnt)
+                br_hit(&bc, MRM_BR_VFAST, MRM_BK_VFAST, br_key(d1, bs, offset + i * str), vsqr[i] > vsqr_diff,
+                       br_cls(&bc, fabs(vsqr[i] - vsqr_diff), vsqr_diff));
             }
           }
           for (int v = 0; v < NV; v++) {
@@ -1085,8 +1125,13 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
                 double mn = (sl < sl_lim) ? sl : sl_lim, mx = (sl_lim < sl) ? sl : sl_lim;
                 int kc = (mn > 0.0) ? ((sl < sl_lim) ? 1 : 0) : (mx < 0.0) ? ((sl_lim < sl) ? 3 : 2) : 4;
                 unsigned long long key = br_key(d1, bs, offset + i * str) + v;
-                br_cat(&bc, MRM_BR_MM_POS_LIM, MRM_BK_MM, key, kc);
-                br_hit(&bc, MRM_BR_CLIP, MRM_BK_CLIP, key, cl < 0.0);
+                /* margins: the nearer of sl_lim, sl to zero; the clip's ratio
+                   is noise where sl is within a near tie of zero */
+                double sc = fmax(fabs(X[i]), fabs(X[i + 1]));
+                br_cat(&bc, MRM_BR_MM_POS_LIM, MRM_BK_MM, key, kc, br_cls(&bc, fmin(fabs(mn), fabs(mx)), sc));
+                int tsl = br_cls(&bc, fabs(sl), sc);
+                br_hit(&bc, MRM_BR_CLIP, MRM_BK_CLIP, key, cl < 0.0,
+                       tsl ? tsl : br_cls(&bc, fabs(cl), fmax(1.0, fabs(dn * (rf - 1.0)))));
               }
             }
           }
@@ -1094,8 +1139,10 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             double tvd_faci = 0.5 * idxd1 * smin(cmax, smax(cm[i], cm[i + 1]));
             if (cnt) {
               double m = (cm[i] < cm[i + 1]) ? cm[i + 1] : cm[i];
-              br_cat(&bc, MRM_BR_CMAX_LIMIT, MRM_BK_CMAX, br_key(d1, bs, offset + i * str),
-                     (cmax < m) ? 0 : (cm[i] < cm[i + 1]) ? 2 : 1);
+              int kc = (cmax < m) ? 0 : (cm[i] < cm[i + 1]) ? 2 : 1;
+              double mg = kc == 0 ? m - cmax : fmin(cmax - m, fabs(cm[i] - cm[i + 1]));
+              br_cat(&bc, MRM_BR_CMAX_LIMIT, MRM_BK_CMAX, br_key(d1, bs, offset + i * str), kc,
+                     br_cls(&bc, mg, fmax3(cmax, cm[i], cm[i + 1])));
             }
             for (int v = 0; v < NV; v++) flx[v * sz + i] *= tvd_faci;
             flx[(5 + d1) * sz + i] *= B_par_diff;
@@ -1177,12 +1224,16 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
             double qres = 0.5 * (qrho[i] + qrho[i - 1]) * qft1[i];
             r[4] += dt_tvd * qres;
             for (int v = 0; v < NV; v++) U[v * bs + node] += r[v];
-            if (cnt) br_hit(&bc, MRM_BR_RHO_FLOOR, MRM_BK_RHO_FLOOR, br_key(d1, bs, node), U[node] < rho_min);
+            if (cnt)
+              br_hit(&bc, MRM_BR_RHO_FLOOR, MRM_BK_RHO_FLOOR, br_key(d1, bs, node), U[node] < rho_min,
+                     br_cls(&bc, fabs(U[node] - rho_min), rho_min));
             U[node] = smax(rho_min, U[node]);
             double mx = U[bs + node], my = U[2 * bs + node], mz = U[3 * bs + node];
             const double e_pre = U[4 * bs + node];
             const double efl = eps_min * U[node] + 0.5 * (mx * mx + my * my + mz * mz) / U[node];
-            if (cnt) br_hit(&bc, MRM_BR_E_FLOOR, MRM_BK_E_FLOOR, br_key(d1, bs, node), e_pre < efl);
+            if (cnt)
+              br_hit(&bc, MRM_BR_E_FLOOR, MRM_BK_E_FLOOR, br_key(d1, bs, node), e_pre < efl,
+                     br_cls(&bc, fabs(e_pre - efl), fmax(fabs(e_pre), fabs(efl))));
             U[4 * bs + node] = smax(U[4 * bs + node], efl);
             if (dg) {
               dg[DG_T6][node] += boris3 * idt_full;
@@ -1197,7 +1248,11 @@ void orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd) {
 #pragma omp critical(orc_bcount)
         {
           for (int q = 0; q < MRM_BR_NCOUNT; q++) c->bcount[q] += bc.n[q];
-          for (int q = 0; q < MRM_BK_N; q++) c->bcount[MRM_BR_NCOUNT + q] += bc.h[q];
+          for (int q = 0; q < MRM_BK_N; q++) {
+            c->bcount[MRM_BR_NCOUNT + q] += bc.h[q];
+            c->bcount[MRM_BR_NCOUNT + MRM_BK_N + q] += bc.nnear[q];
+            c->bcount[MRM_BR_NCOUNT + 2 * MRM_BK_N + q] += bc.nexact[q];
+          }
         }
       }
     }

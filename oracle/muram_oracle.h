@@ -52,9 +52,9 @@ int    orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps);
    added to by every operation called with run->need_diagnostics */
 void   orc_diag_zero(orc_ctx* c);
 double* orc_diag(orc_ctx* c, int t);
-/* limiter branch accounting with the library's categories and checksums
-   (MRM_BR_* in muram_step.h): on = 1 zeroes and starts, 0 stops */
-void   orc_branch_count(orc_ctx* c, int on);
+/* limiter branch accounting with the library's categories, checksums and
+   near-tie rule (MRM_BR_* in muram_step.h): on = 1 zeroes and starts, 0 stops */
+void   orc_branch_count(orc_ctx* c, int on, double tie);
 void   orc_get_branch_counts(const orc_ctx* c, unsigned long long out[MRM_BR_NSLOT]);
 
 /* scalar EOS interpolations of eos_init.C:216-416 */

@@ -65,7 +65,7 @@ class Oracle:
             L.orc_diag_zero.argtypes = [P]
             L.orc_diag.argtypes = [P, C.c_int]
             L.orc_diag.restype = dp
-            L.orc_branch_count.argtypes = [P, C.c_int]
+            L.orc_branch_count.argtypes = [P, C.c_int, C.c_double]
             L.orc_get_branch_counts.argtypes = [P, C.POINTER(C.c_ulonglong)]
             for n in ("orc_p_interp", "orc_s_interp", "orc_T_interp", "orc_d3_interp", "orc_eps3_interp"):
                 getattr(L, n).argtypes = [P, C.c_double, C.c_double]

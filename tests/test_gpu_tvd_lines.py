@@ -83,7 +83,7 @@ def test_branch_counts_off_by_default(table):
     m, r = start_gpu(table, box, config.stratified_phys(), U, count=False)
     m.step(r, config.DT_RAD_OFF, 1)
     assert not m.branch_counts().any()
-    m.branch_count(True)
+    m.branch_count(True, 0.0)
     m.step(r, config.DT_RAD_OFF, 1)
     first = m.branch_counts().copy()
     assert first.any()

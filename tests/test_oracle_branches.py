@@ -73,7 +73,7 @@ def test_every_evaluation_counted_once(table):
     o.set_boundary_conditions(r, 0)
     U = o.state(0).copy()
     r, dt = prepared(o, U)
-    o.branch_count(1)
+    o.branch_count(1, 0.0)
     o.tvdlimit(r, dt)
     c = counts_of(o)
     L = lines_and_lengths(o)
@@ -98,13 +98,27 @@ def test_every_evaluation_counted_once(table):
     # the checksums are reproducible and off when the accounting is off
     first = o.branch_counts().copy()
     o.set_state(U)
-    o.branch_count(1)
+    o.branch_count(1, 0.0)
     o.tvdlimit(r, dt)
     assert np.array_equal(o.branch_counts(), first)
-    o.branch_count(0)
+    o.branch_count(0, 0.0)
     o.set_state(U)
     o.tvdlimit(r, dt)
     assert np.array_equal(o.branch_counts(), first)
+    # with a near-tie margin, every evaluation is decisive, a near tie or an
+    # exact tie: the categories and the ties of each branch point add up again
+    o.set_state(U)
+    o.branch_count(1, 1e-3)
+    o.tvdlimit(r, dt)
+    t = [int(x) for x in o.branch_counts()]
+    k = len(abi.BRANCH_SUMS)
+    near = {c: t[NB + k + i] + t[NB + 2 * k + i] for i, c in enumerate(abi.BRANCH_SUMS)}
+    assert near["slope"] > 0 and near["minmod"] > 0
+    assert sum(t[IDX[k]] for k in ("slope_pos_l", "slope_pos_r", "slope_pos_m", "slope_neg_l", "slope_neg_r",
+                                   "slope_neg_m", "slope_zero")) + near["slope"] == 8 * slopes
+    assert sum(t[IDX[k]] for k in ("mm_pos_lim", "mm_pos_sl", "mm_neg_lim", "mm_neg_sl", "mm_zero")) \
+        + near["minmod"] == 8 * ifaces
+    assert sum(t[IDX[k]] for k in ("cmax_limit", "cmax_left", "cmax_right")) + near["cmax"] == ifaces
 
 
 def test_tau_counted_once_per_sweep(table):
@@ -121,7 +135,7 @@ def test_tau_counted_once_per_sweep(table):
         tau[cell] = 1.0
     o.field(abi.TAU)[...] = tau
     r, dt = prepared(o, U)
-    o.branch_count(1)
+    o.branch_count(1, 0.0)
     o.tvdlimit(r, dt)
     assert counts_of(o)[IDX["cz"]] == sum(cells.values())
 
@@ -181,7 +195,7 @@ def test_designed_profile_choices(table):
     assert len(k) == len(b1) == o.shape[2]
     U = designed_state(o, k, b1)
     r, dt = prepared(o, U)
-    o.branch_count(1)
+    o.branch_count(1, 0.0)
     o.tvdlimit(r, dt)
     c = counts_of(o)
     lg = np.asarray(k)
@@ -222,7 +236,7 @@ def test_density_floor_counted_per_update(table):
     tab = eos_table.load_tables(table)
     U = initial.make_state(tab, initial.BoxSpec(gsize=(8, 6, 6), dx=(3.2e6,) * 3, seed=3))
     r, dt = prepared(o, U)
-    o.branch_count(1)
+    o.branch_count(1, 0.0)
     o.tvdlimit(r, dt)
     (L0, s0), _, _ = lines_and_lengths(o)
     assert counts_of(o)[IDX["rho_floor"]] == L0 * (s0 - 4)
