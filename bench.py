@@ -90,7 +90,7 @@ def oracle_run(table, phys, box, steps, threads, count=False):
     timed steps.  Returns the oracle, its run descriptor, the dt of every
     step and the seconds per timed step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from muram_main_amd import config, eos_table, initial
+    from muram_main_amd import abi, config, eos_table, initial
     from orc import Oracle
     tab = eos_table.load_tables(table)
     U = initial.make_state(tab, box)
@@ -138,13 +138,16 @@ def gpu_parity(table, wl, o, ro, dts, path):
     m.close()
     B = o.state(0)
     errs, l2 = {}, {}
+    inner = (slice(2, -2),) * 3
     for v, name in enumerate(FIELDS):
-        d = A[v] - B[v]
-        errs[name] = float(f"{float(np.max(np.abs(d)) / max(np.max(np.abs(B[v])), 1e-300)):.3e}")
-        l2[name] = float(f"{float(np.sqrt(np.sum(d * d)) / max(np.sqrt(np.sum(B[v] * B[v])), 1e-300)):.3e}")
+        a, b = A[v][inner], B[v][inner]
+        d = a - b
+        errs[name] = float(f"{float(np.max(np.abs(d)) / max(np.max(np.abs(b)), 1e-300)):.3e}")
+        l2[name] = float(f"{float(np.sqrt(np.sum(d * d)) / max(np.sqrt(np.sum(b * b)), 1e-300)):.3e}")
     del A
     bo = o.branch_counts()
-    return {"path": path, "steps": len(dts), "field_rel_err_max": errs, "max_rel_err": max(errs.values()),
+    return {"path": path, "steps": len(dts), "cells": "interior",
+            "field_rel_err_max": errs, "max_rel_err": max(errs.values()),
             "field_rel_err_l2": l2, "max_rel_err_l2": max(l2.values()),
             "dt_rel_diff_max": float(f"{ddt:.3e}"), "branch_mismatch": abi.branch_mismatch(bg, bo),
             "branch_tie": abi.BRANCH_TIE, "branch_counts": abi.branch_dict(bo)["counts"]}

@@ -92,17 +92,24 @@ def check_branches(m, o):
 
 def check_pair(m, r, o, ro, tol=TOL, counts=True, linf_tol=None):
     """decisive branch choices equal (first, so that a field failure still
-    reports them), fields to tol (relative L2 per field over the buffer, and
-    max|gpu-ref|/max|ref| to linf_tol, tol unless given), dt and time to tol"""
+    reports them); over the interior cells, every field within tol in
+    relative L2 (||gpu - ref|| / ||ref||) and within linf_tol (tol unless
+    given) in max|gpu - ref| / max|ref|; dt and time to tol.  The ghost
+    layers, which the boundary rebuilds from the interior every stage, are
+    reported beside them."""
     mis = check_branches(m, o) if counts else None
     Ug, Uo = m.get_state(), o.state(0)
-    errs = field_errs(Ug, Uo)
-    l2 = [rel_l2(Ug[v], Uo[v]) for v in range(8)]
-    print("per-field rel err (max):", ["%.2e" % e for e in errs], flush=True)
-    print("per-field rel err (L2): ", ["%.2e" % e for e in l2], "dt", r.dt, ro.dt, flush=True)
-    w = int(np.argmax(errs))
+    inner = (slice(2, -2),) * 3
+    errs = [relerr(Ug[v][inner], Uo[v][inner]) for v in range(8)]
+    l2 = [rel_l2(Ug[v][inner], Uo[v][inner]) for v in range(8)]
+    ebuf = field_errs(Ug, Uo)
+    print("interior, per-field rel err (max):", ["%.2e" % e for e in errs], flush=True)
+    print("interior, per-field rel err (L2): ", ["%.2e" % e for e in l2], "dt", r.dt, ro.dt, flush=True)
+    print("whole buffer, per-field rel err (max):", ["%.2e" % e for e in ebuf], flush=True)
+    w = int(np.argmax(ebuf))
     d = np.abs(Ug[w] - Uo[w])
-    print("largest difference: field", w, "at (k, j, i)", np.unravel_index(int(np.argmax(d)), d.shape), flush=True)
+    print("largest difference in the buffer: field", w, "at (k, j, i)", np.unravel_index(int(np.argmax(d)), d.shape),
+          flush=True)
     assert max(l2) <= tol, l2
     assert max(errs) <= (tol if linf_tol is None else linf_tol), errs
     assert abs(r.dt - ro.dt) <= tol * abs(ro.dt)
