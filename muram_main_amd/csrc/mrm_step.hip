@@ -217,6 +217,23 @@ static inline double bits_to_double(unsigned long long b) {
 }
 
 // ------------------------------------------------------------------ lifecycle
+// buffer geometry and cell sizes of a grid descriptor (GridData::Init,
+// grid.C:147-264, on one rank)
+static void geom_of(const mrm_grid_desc& grid, Geom& G, double dx[3]) {
+  int n[3];
+  for (int d = 0; d < 3; d++) {
+    n[d] = grid.gsize[d] + 2 * grid.ghosts[d];
+    dx[d] = (grid.gxmax[d] - grid.gxmin[d]) / grid.gsize[d];
+  }
+  G.n0 = n[0]; G.n1 = n[1]; G.n2 = n[2];
+  G.g0 = grid.ghosts[0]; G.g1 = grid.ghosts[1]; G.g2 = grid.ghosts[2];
+  G.lb0 = G.g0; G.lb1 = G.g1; G.lb2 = G.g2;
+  G.le0 = grid.gsize[0] + G.g0 - 1; G.le1 = grid.gsize[1] + G.g1 - 1; G.le2 = grid.gsize[2] + G.g2 - 1;
+  G.s1 = n[0];
+  G.s2 = (long)n[0] * n[1];
+  G.bs = (long)n[0] * n[1] * n[2];
+}
+
 extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, int precision, mrm_ctx** out) {
   if (!grid || !phys || !out) return MRM_ERR_ARG;
   *out = nullptr;
@@ -237,22 +254,14 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   c->precision = precision;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MRM_ERR_HIP; }
   Geom& G = c->G;
-  int n[3];
+  geom_of(*grid, G, c->dx);
   for (int d = 0; d < 3; d++) {
     c->lsize[d] = grid->gsize[d];
-    n[d] = grid->gsize[d] + 2 * grid->ghosts[d];
-    c->vsize = n[d] > c->vsize ? n[d] : c->vsize;
-    c->dx[d] = (grid->gxmax[d] - grid->gxmin[d]) / grid->gsize[d];
+    const int n = grid->gsize[d] + 2 * grid->ghosts[d];
+    c->vsize = n > c->vsize ? n : c->vsize;
   }
   c->dxmin = c->dx[0];
   for (int d = 1; d < 3; d++) c->dxmin = sminh(c->dxmin, c->dx[d]);
-  G.n0 = n[0]; G.n1 = n[1]; G.n2 = n[2];
-  G.g0 = grid->ghosts[0]; G.g1 = grid->ghosts[1]; G.g2 = grid->ghosts[2];
-  G.lb0 = G.g0; G.lb1 = G.g1; G.lb2 = G.g2;
-  G.le0 = grid->gsize[0] + G.g0 - 1; G.le1 = grid->gsize[1] + G.g1 - 1; G.le2 = grid->gsize[2] + G.g2 - 1;
-  G.s1 = n[0];
-  G.s2 = (long)n[0] * n[1];
-  G.bs = (long)n[0] * n[1] * n[2];
   const size_t bs = (size_t)G.bs;
   int rc = MRM_OK;
   hipError_t e;
@@ -286,11 +295,11 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
     rc = fail(c, MRM_ERR_HIP, "hipMalloc P2", e);
   if (!rc && (e = hipMalloc(&c->dR, 8 * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc Res", e);
   if (!rc && (e = hipMalloc(&c->dF, MRM_NFIELD * bs * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc fields", e);
-  c->npart = 8 * (long)nblk((long)n[1] * n[2]) + 64;
+  c->npart = 8 * (long)nblk((long)G.n1 * G.n2) + 64;
   if (!rc && (e = hipMalloc(&c->d_part, c->npart * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc part", e);
   if (!rc && (e = hipMalloc(&c->d_red, RED_NSLOT * sizeof(unsigned long long))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc red", e);
   if (!rc && (e = hipHostMalloc(&c->h_red, RED_NSLOT * sizeof(unsigned long long), hipHostMallocDefault)) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipHostMalloc", e);
-  if (!rc && (e = hipMalloc(&c->d_prof, 3 * n[0] * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc prof", e);
+  if (!rc && (e = hipMalloc(&c->d_prof, 3 * G.n0 * sizeof(double))) != hipSuccess) rc = fail(c, MRM_ERR_HIP, "hipMalloc prof", e);
   // Res starts at zero (the reference relies on it, grid.C:279); all arrays defined
   for (int q = 0; q < 3 && !rc; q++) {
     if (mixed_prec) {
