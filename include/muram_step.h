@@ -167,6 +167,27 @@ int mrm_adjust_valf_max(mrm_ctx* ctx, const mrm_run_desc* run, double stats[5]);
  * force over the interior; returns dt_cfl = dxmin / cmax. */
 int mrm_mhd_residual(mrm_ctx* ctx, const mrm_run_desc* run, double* dt_cfl);
 
+/* MHD_Residual on caller-owned device arrays, allocating nothing.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  It needs no context: U (8 x bufsize doubles, SoA,
+ * primitive as ConsToPrim leaves it, ghosts filled), pres (bufsize) and res
+ * (8 x bufsize, added to over the interior as mrm_mhd_residual adds to Res)
+ * are device pointers of the grid's buffer, dt_cfl a device pointer that
+ * receives dxmin / cmax (limited as mrm_mhd_residual limits it with
+ * resistivity; run may be NULL without it).  Every kernel is enqueued on
+ * `stream` (a hipStream_t, NULL for the default stream) and nothing is
+ * synchronized.  The characteristic-speed reduction lives in a 256-byte
+ * device array of the library, one per GPU, held until the process ends and
+ * cleared by each call: calls that overlap on one GPU race.  The arithmetic
+ * and the kernels are those of mrm_mhd_residual (bit-identical results).
+ * Limits: grids of at most 2^29 buffer cells (MRM_ERR_ARG beyond), ghosts 2;
+ * Spitzer conduction and ambipolar diffusion, whose residual terms have
+ * kernels of their own, return MRM_ERR_UNSUPPORTED. */
+int mrm_mhd_residual_noalloc(const mrm_grid_desc* grid, const mrm_phys_desc* phys, const mrm_run_desc* run,
+                             double inv_va2max, const double* U, const double* pres, double* res,
+                             double* dt_cfl, void* stream);
+
 /* Sync_timestep (solver.C:487-548), single rank: updates run->dt if autostep,
  * writes cfl; MRM_ERR_INVALID_DT / MRM_ERR_CFL on the reference's aborts. */
 int mrm_sync_timestep(mrm_ctx* ctx, mrm_run_desc* run, double dt_rad, double dt_mhd,

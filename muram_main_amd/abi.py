@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libmuram_mi355x.so")
 MRM_OK = 0
 ERRORS = {-1: "MRM_ERR_ARG", -2: "MRM_ERR_HIP", -3: "MRM_ERR_EOS", -4: "MRM_ERR_INVALID_DT",
           -5: "MRM_ERR_CFL", -6: "MRM_ERR_STATE", -7: "MRM_ERR_UNSUPPORTED"}
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
 PREC_FP64, PREC_FP32_MIXED = 0, 1
 U, U0, RES = 0, 1, 2
 PRES, TEMP, NE, RHOI, AMB, QTOT, TAU, DIVB, PHI, SFLX = range(10)
@@ -35,7 +36,7 @@ EXPORTS = [
     "mrm_io_write_backup", "mrm_io_read_backup", "mrm_io_read_var", "mrm_io_write_var",
     "mrm_adjoint_forward", "mrm_adjoint_backward",
     "mrm_timers", "mrm_get_timers", "mrm_timer_name",
-    "mrm_branch_count", "mrm_get_branch_counts",
+    "mrm_branch_count", "mrm_get_branch_counts", "mrm_mhd_residual_noalloc",
 ]
 NTIMER = 24
 
@@ -161,6 +162,8 @@ def load(path: str = LIB_PATH):
         "mrm_get_timers": ([P, dp, C.POINTER(C.c_long)], C.c_int),
         "mrm_timer_name": ([C.c_int], C.c_char_p),
         "mrm_branch_count": ([P, C.c_int, C.c_double], C.c_int),
+        "mrm_mhd_residual_noalloc": ([C.POINTER(GridDesc), C.POINTER(PhysDesc), C.POINTER(RunDesc), C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
         "mrm_get_branch_counts": ([P, C.POINTER(C.c_ulonglong)], C.c_int),
     }
     for name, (args, res) in sig.items():

@@ -14,6 +14,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <mutex>
 
 #include "../../include/muram_step.h"
 #include "mrm_launch.h"
@@ -217,6 +218,8 @@ static inline double bits_to_double(unsigned long long b) {
 }
 
 // ------------------------------------------------------------------ lifecycle
+static int scratch_addr(unsigned long long** out);
+
 // buffer geometry and cell sizes of a grid descriptor (GridData::Init,
 // grid.C:147-264, on one rank)
 static void geom_of(const mrm_grid_desc& grid, Geom& G, double dx[3]) {
@@ -253,6 +256,11 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   c->phys = *phys;
   c->precision = precision;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MRM_ERR_HIP; }
+  {
+    // resolve the no-allocation entry point's scratch on this device now
+    unsigned long long* sp = nullptr;
+    if (scratch_addr(&sp) != MRM_OK) { delete c; return MRM_ERR_HIP; }
+  }
   Geom& G = c->G;
   geom_of(*grid, G, c->dx);
   for (int d = 0; d < 3; d++) {
@@ -857,6 +865,108 @@ extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_
   if (rc) return rc;
   if (dt_cfl) *dt_cfl = eta_dt_limit(c, run, c->dxmin / cmax);   // mhdres_SR.C:795-819
   return MRM_OK;
+}
+
+// ------------------------------------------------------------------ MHD_Residual without allocation
+// mrm_mhd_residual_noalloc runs MHD_Residual's kernels (k_resid_tile,
+// k_cmax_faces) on caller-owned device arrays.  The only device memory it
+// needs beyond its inputs and outputs, the characteristic-speed reduction
+// slot, lives in `scratch`, a file-scope device array (one copy per GPU that
+// loads the library, held until the process ends); its address on the
+// current device is resolved once per device with hipGetSymbolAddress, at
+// mrm_create or at the entry point's first call on that device, and only
+// that address is ever passed to kernels.  Every call on a GPU shares that
+// GPU's array: calls that overlap on one GPU race.
+#define MRM_NOALLOC_SLOTS 32     // 256 bytes; RED_CMAX is the slot these kernels use
+#define MRM_MAX_DEVICES 64
+static __device__ __align__(256) unsigned long long scratch[MRM_NOALLOC_SLOTS];
+static unsigned long long* scratch_dev[MRM_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+// the device address of `scratch` on the current device (resolved once)
+static int scratch_addr(unsigned long long** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MRM_MAX_DEVICES) return MRM_ERR_HIP;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return MRM_ERR_HIP;
+    scratch_dev[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = scratch_dev[dev];
+  return MRM_OK;
+}
+
+// zeroes n slots (a kernel, not hipMemsetAsync, so that the call can be
+// captured in a HIP graph)
+__global__ void k_slots_zero(unsigned long long* p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+
+// dt_cfl = dxmin / cmax, limited by the resistive dt (mhdres_SR.C:795-819),
+// written to device memory
+__global__ void k_dt_from_cmax(const unsigned long long* red, double dxmin, double dt_res, double* dt_out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double dt = dxmin / __longlong_as_double((long long)red[RED_CMAX]);
+    *dt_out = (dt_res < dt) ? dt_res : dt;
+  }
+}
+
+extern "C" int mrm_mhd_residual_noalloc(const mrm_grid_desc* grid, const mrm_phys_desc* phys,
+                                        const mrm_run_desc* run, double inv_va2max, const double* U,
+                                        const double* pres, double* res, double* dt_cfl, void* stream) {
+  if (!grid || !phys || !U || !pres || !res || !dt_cfl) return MRM_ERR_ARG;
+  for (int d = 0; d < 3; d++) {
+    if (grid->gsize[d] < 1 || grid->ghosts[d] != 2 || !(grid->gxmax[d] > grid->gxmin[d])) return MRM_ERR_ARG;
+  }
+  // conduction and ambipolar diffusion add residual terms of their own
+  if (phys->params[3] > 0.0 || phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
+  Geom G;
+  double dx[3];
+  geom_of(*grid, G, dx);
+  // the kernels address cells with 32-bit byte offsets of 8-byte values
+  if (8 * (size_t)G.bs >= (size_t(1) << 32)) return MRM_ERR_ARG;
+  unsigned long long* red = nullptr;
+  int rc = scratch_addr(&red);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  StageArgs A;
+  A.G = G;
+  for (int d = 0; d < 3; d++) {
+    A.RP.w1[d] = 8. / (12. * dx[d]);
+    A.RP.w2[d] = -1. / (12. * dx[d]);
+  }
+  A.RP.inv_va2max = inv_va2max;
+  A.RP.eta = phys->params[4];
+  for (int d = 0; d < 3; d++) {
+    A.RP.ww0[d] = -30. / (12. * dx[d] * dx[d]);
+    A.RP.ww1[d] = 16. / (12. * dx[d] * dx[d]);
+    A.RP.ww2[d] = -1. / (12. * dx[d] * dx[d]);
+  }
+  const int n[3] = {grid->gsize[0], grid->gsize[1], grid->gsize[2]};
+  A.ntx = (n[0] + ST_SX - 1) / ST_SX;
+  A.nty = (n[1] + ST_SY - 1) / ST_SY;
+  A.nkc = (n[2] + ST_KC - 1) / ST_KC;
+  k_slots_zero<<<1, 64, 0, st>>>(red, MRM_NOALLOC_SLOTS);
+  const unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
+  if (A.RP.eta > 0.0) k_resid_tile<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, U, pres, res, red);
+  else k_resid_tile<false><<<nb, ST_SX * ST_SY, 0, st>>>(A, U, pres, res, red);
+  long nf = 2L * G.g0 * n[1] * n[2] + 2L * G.g1 * n[0] * n[2] + 2L * G.g2 * n[0] * n[1];
+  unsigned nbf = nblk(nf);
+  if (nbf > 2048) nbf = 2048;
+  k_cmax_faces<<<nbf, MRM_BLOCK, 0, st>>>(G, inv_va2max, U, pres, red);
+  double dxmin = dx[0];
+  for (int d = 1; d < 3; d++) dxmin = sminh(dxmin, dx[d]);
+  double dt_res = INFINITY;
+  if (A.RP.eta > 0.0) {
+    const double dt_diff_fac = 0.5 / (1. / (dx[0] * dx[0]) + 1. / (dx[1] * dx[1]) + 1. / (dx[2] * dx[2]));
+    dt_res = dt_diff_fac / A.RP.eta;
+    if (run && run->CFL > 1) dt_res *= 0.5;
+  }
+  k_dt_from_cmax<<<1, 64, 0, st>>>(red, dxmin, dt_res, dt_cfl);
+  return hipGetLastError() == hipSuccess ? MRM_OK : MRM_ERR_HIP;
 }
 
 // ------------------------------------------------------------------ Sync_timestep
