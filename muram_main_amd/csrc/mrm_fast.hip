@@ -1,6 +1,6 @@
 // mrm_fast.hip -- the fused-step kernels built with FMA contraction
 // (-ffp-contract=fast): same source and operation order as the exact build,
-// with a*b+c allowed to fuse.  Selected at run time (MRM_FMA, default on).
+// with a*b+c allowed to fuse.  Selected at run time (MRM_FMA=1; off by default).
 #define MRM_NS fast
 #include "mrm_launch.h"
 

@@ -62,7 +62,7 @@ struct mrm_ctx {
   bool tvd_x_march = false;                      // dim-0 TVD always by the marching kernel
   size_t lds_max = 64 * 1024;                    // LDS bytes one block may use (device attribute)
   bool stage_zwin = false;                       // stage kernel variant
-  bool fma = true;                               // fused step uses the contracted build
+  bool fma = false;                              // fused step uses the contracted build (MRM_FMA=1)
   double last_dt = 0.0;                          // dt and TVD split of the last fused step
   bool last_split = false;
   double* dF = nullptr;   // MRM_NFIELD arrays of bs doubles
@@ -326,7 +326,7 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   const char* zw = getenv("MRM_STAGE_ZWIN");
   c->stage_zwin = !(zw && zw[0] == '0');
   const char* fm = getenv("MRM_FMA");
-  c->fma = !(fm && fm[0] == '0');
+  c->fma = fm && fm[0] == '1';      // contracted build on request only (the exact one is the parity build)
   const char* tx = getenv("MRM_TVD_X");
   c->tvd_x_march = tx && strcmp(tx, "march") == 0;
   // LDS one block may use: the larger of the default and the opt-in limit

@@ -51,7 +51,7 @@ def start_oracle(table, box, phys, U, cfl=0.8, count=True):
     return o, ro
 
 
-def start_gpu(table, box, phys, U, cfl=0.8, path="fused", fma="1", tvd_x=None, precision=abi.PREC_FP64,
+def start_gpu(table, box, phys, U, cfl=0.8, path="fused", fma="0", tvd_x=None, precision=abi.PREC_FP64,
               count=True):
     g = config.grid_desc(box.gsize, box.dx)
     with env(MRM_STEP_PATH=path, MRM_FMA=fma, MRM_TVD_X=tvd_x):

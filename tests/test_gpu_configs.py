@@ -4,8 +4,9 @@
 * box64: 64^3 interior, dx = 32 km, one RK step;
 * box512: 256 x 512 x 512 interior, dx = 16 km, ten RK steps with the
   Alfven limiter at 3000 km/s, on the drop-in sequence (the operations the
-  reference's solver.C:109-365 calls, one C-ABI call each) and on the fused
-  step with FMA contraction (mrm_step's default).
+  reference's solver.C:109-365 calls, one C-ABI call each), on the fused
+  step (mrm_step's default, no FMA contraction) and on the fused step built
+  with contraction (MRM_FMA=1).
 
 Bar: every decisive limiter choice equal in count and position checksum
 (the choices of tvdlimit_SR.C:281-657 over every counted point, slope and
@@ -20,7 +21,10 @@ differ from the unchanged run by up to 7e-11 of max|B| (in the tenuous top
 layers, where the Alfven limiter is strongest) -- tests/ulp_sensitivity.py
 measures it -- so two FP64 implementations whose exp and log differ in the
 last bit cannot agree closer than that.  The box512 oracle run takes about a
-minute on 16 host threads.
+minute on 16 host threads.  The contracted build rounds every a*b+c once
+instead of twice, a difference at every operation rather than once in the
+initial state; its bar is 5e-12 in relative L2 (measured 1.9e-12 in B1, B2,
+from the top layers) with every decisive limiter choice still equal.
 """
 import numpy as np
 import pytest
@@ -74,7 +78,7 @@ def box512_oracle(table):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("path,fma", [("sequence", "0"), ("fused", "1")])
+@pytest.mark.parametrize("path,fma", [("sequence", "0"), ("fused", "0"), ("fused", "1")])
 def test_box512_ten_steps(table, box512_oracle, path, fma):
     print(f"case: box512 ten steps, {path} fma={fma}", flush=True)
     B = box512_oracle
@@ -84,5 +88,5 @@ def test_box512_ten_steps(table, box512_oracle, path, fma):
         m.step(r, config.DT_RAD_OFF, 1)
         print(f"gpu step {s + 1}: dt {r.dt:.6e} (oracle {dt_o:.6e})", flush=True)
         assert abs(r.dt - dt_o) <= TOL * dt_o
-    check_pair(m, r, B["o"], B["ro"], linf_tol=1e-9)
+    check_pair(m, r, B["o"], B["ro"], tol=TOL if fma == "0" else 5e-12, linf_tol=1e-9)
     m.close()
