@@ -457,7 +457,8 @@ def main():
                     help="param_spitzer (Spitzer heat conduction); default: the workload's (0)")
     ap.add_argument("--divb", action="store_true",
                     help="Clean_divB on (divB_switch 1, at most 5 passes, error target 0.2)")
-    ap.add_argument("--stage-zwin", default=None, choices=["0", "1"])
+    ap.add_argument("--stage-zwin", default=None, choices=["0", "1", "2"],
+                    help="stage kernel: 2 fluxes shared through LDS (default), 1 register window, 0 re-read")
     ap.add_argument("--fma", default=None, choices=["0", "1"])
     ap.add_argument("--calibrate", action="store_true",
                     help="run a 512 MiB device copy first (reference kernel for PMC byte counters)")
@@ -573,7 +574,7 @@ def main():
                    "path": args.path, "va_adjust": wl.phys.params[config.I_PARAM_VA_ADJUST],
                    "divb": bool(wl.phys.divB[0] > 0), "eta": wl.phys.params[config.I_PARAM_ETA],
                    "spitzer": wl.phys.params[config.I_PARAM_SPITZER],
-                   "stage_zwin": os.environ.get("MRM_STAGE_ZWIN", "1"),
+                   "stage_kernel": os.environ.get("MRM_STAGE_ZWIN", "2"),
                    "fma": os.environ.get("MRM_FMA", "0")},
         "hbm_frac_model": R["hbm_frac_model"],
         "roofline": R["roofline"],

@@ -370,6 +370,59 @@ MRM_HD void eta_sweep(double dB2, double dB3, const double c[3], const double s1
   for (int q = 0; q < 3; q++) res[5 + q] += eta * (ww0 * c[q] + ww1 * s1[q] + ww2 * s2[q]);
 }
 
+template <int D, bool ETA>
+MRM_HD void sweep_tail(const double e1[5], const double e2[5], const double bs1[3], const double bs2[3],
+                       const double c[9], double w1, double w2, double lf, double res[MRM_NV],
+                       const ResidParams* rp, double* J);
+
+// Everything a residual sweep along D does after its flux differences
+// (mhdres_SR.C:361-412): d1 = F(+1) - F(-1), d2 = F(+2) - F(-2) of the 8
+// fluxes, 3 stresses, B_D2 and B_D3; bs1, bs2 (ETA) the sums of B at +-1,
+// +-2; c the centre point's primitives
+template <int D, bool ETA = false>
+MRM_HD void sweep_finish(const double d1[13], const double d2[13], const double bs1[3], const double bs2[3],
+                         const double c[9], double w1, double w2, double lf, double res[MRM_NV],
+                         const ResidParams* rp, double* J) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    res[q] = 0.0;
+    res[q] -= w1 * d1[q] + w2 * d2[q];
+  }
+  sweep_tail<D, ETA>(d1 + 8, d2 + 8, bs1, bs2, c, w1, w2, lf, res, rp, J);
+}
+
+// the part of sweep_finish after the flux terms: e1 = d1 + 8, e2 = d2 + 8
+// (the stresses' and B_D2, B_D3's differences)
+template <int D, bool ETA>
+MRM_HD void sweep_tail(const double e1[5], const double e2[5], const double bs1[3], const double bs2[3],
+                       const double c[9], double w1, double w2, double lf, double res[MRM_NV],
+                       const ResidParams* rp, double* J) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  const double* d1 = e1 - 8;
+  const double* d2 = e2 - 8;
+  double dB2 = w1 * d1[11] + w2 * d2[11];
+  double dB3 = w1 * d1[12] + w2 * d2[12];
+  double cb[3];
+  cb[D] = -c[5 + D2] * dB2 - c[5 + D3] * dB3;
+  cb[D2] = c[5 + D] * dB2;
+  cb[D3] = c[5 + D] * dB3;
+  double lrt0 = w1 * d1[8] + w2 * d2[8];
+  double lrt1 = w1 * d1[9] + w2 * d2[9];
+  double lrt2 = w1 * d1[10] + w2 * d2[10];
+  lrt0 = lf * lrt0 + (1.0 - lf) * cb[0];
+  lrt1 = lf * lrt1 + (1.0 - lf) * cb[1];
+  lrt2 = lf * lrt2 + (1.0 - lf) * cb[2];
+  res[1] += lrt0;
+  res[2] += lrt1;
+  res[3] += lrt2;
+  res[4] += c[1] * lrt0 + c[2] * lrt1 + c[3] * lrt2;
+  if constexpr (ETA)
+    eta_sweep<D>(dB2, dB3, c + 5, bs1, bs2, rp->eta, rp->ww0[D], rp->ww1[D], rp->ww2[D], J, res);
+}
+
 // The same residual as sweep_residual with a lower register peak: the four
 // neighbour points are fetched through get(s, w[9]) (s = 0,1,3,4 for offsets
 // -2,-1,+1,+2) one pair at a time, and only the differences
@@ -421,29 +474,60 @@ MRM_HD void sweep_residual_lr(GET get, const double c[9], double w1, double w2,
       for (int q = 0; q < 3; q++) bs2[q] = pb[5 + q] + pa[5 + q];
     }
   }
+  sweep_finish<D, ETA>(d1, d2, bs1, bs2, c, w1, w2, lf, res, rp, J);
+}
+
+// A flux point as the residual sweeps along D use it: the 8 fluxes, the 3
+// magnetic stresses, B_D2 and B_D3, and (ETA) B_D -- NFP_(ETA) values
+template <bool ETA> constexpr int nfp_() { return ETA ? 14 : 13; }
+template <int D, bool ETA>
+MRM_HD void flux_pack(const double p[9], double o[14]) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  PtFlux f;
+  point_flux<D>(p, p[8], f);
+#pragma unroll
+  for (int q = 0; q < 8; q++) o[q] = f.f[q];
+#pragma unroll
+  for (int q = 0; q < 3; q++) o[8 + q] = f.ms[q];
+  o[11] = p[5 + D2];
+  o[12] = p[5 + D3];
+  if (ETA) o[13] = p[5 + D];
+}
+
+// sweep_residual_lr from packed flux points, one value at a time: getq(s, f)
+// gives value f of the packed point at offset s - 2 (s = 0, 1, 3, 4), so
+// that only the four values of one quantity are live at once; each flux
+// term's residual is formed as sweep_finish forms it, the stresses' and B's
+// differences go to sweep_tail
+template <int D, bool ETA = false, class GETQ>
+MRM_HD void sweep_residual_packed(GETQ getq, const double c[9], double w1, double w2, double lf,
+                                  double res[MRM_NV], const ResidParams* rp = nullptr, double* J = nullptr) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
+    const double d1 = getq(3, q) - getq(1, q);
+    const double d2 = getq(4, q) - getq(0, q);
     res[q] = 0.0;
-    res[q] -= w1 * d1[q] + w2 * d2[q];
+    res[q] -= w1 * d1 + w2 * d2;
   }
-  double dB2 = w1 * d1[11] + w2 * d2[11];
-  double dB3 = w1 * d1[12] + w2 * d2[12];
-  double cb[3];
-  cb[D] = -c[5 + D2] * dB2 - c[5 + D3] * dB3;
-  cb[D2] = c[5 + D] * dB2;
-  cb[D3] = c[5 + D] * dB3;
-  double lrt0 = w1 * d1[8] + w2 * d2[8];
-  double lrt1 = w1 * d1[9] + w2 * d2[9];
-  double lrt2 = w1 * d1[10] + w2 * d2[10];
-  lrt0 = lf * lrt0 + (1.0 - lf) * cb[0];
-  lrt1 = lf * lrt1 + (1.0 - lf) * cb[1];
-  lrt2 = lf * lrt2 + (1.0 - lf) * cb[2];
-  res[1] += lrt0;
-  res[2] += lrt1;
-  res[3] += lrt2;
-  res[4] += c[1] * lrt0 + c[2] * lrt1 + c[3] * lrt2;
-  if constexpr (ETA)
-    eta_sweep<D>(dB2, dB3, c + 5, bs1, bs2, rp->eta, rp->ww0[D], rp->ww1[D], rp->ww2[D], J, res);
+  double e1[5], e2[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    e1[q] = getq(3, 8 + q) - getq(1, 8 + q);
+    e2[q] = getq(4, 8 + q) - getq(0, 8 + q);
+  }
+  double bs1[3], bs2[3];
+  if constexpr (ETA) {
+    bs1[D2] = getq(3, 11) + getq(1, 11);
+    bs1[D3] = getq(3, 12) + getq(1, 12);
+    bs1[D] = getq(3, 13) + getq(1, 13);
+    bs2[D2] = getq(4, 11) + getq(0, 11);
+    bs2[D3] = getq(4, 12) + getq(0, 12);
+    bs2[D] = getq(4, 13) + getq(0, 13);
+  }
+  sweep_tail<D, ETA>(e1, e2, bs1, bs2, c, w1, w2, lf, res, rp, J);
 }
 
 // ------------------------------------------------------------------ sources

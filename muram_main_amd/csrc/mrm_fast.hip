@@ -5,7 +5,7 @@
 #include "mrm_launch.h"
 
 namespace fast {
-hipError_t (*p_launch_stage)(hipStream_t, unsigned, bool, const StageArgs&, const double*, const double*,
+hipError_t (*p_launch_stage)(hipStream_t, unsigned, int, const StageArgs&, const double*, const double*,
                              const double*, double*, double*, const double*) = launch_stage;
 hipError_t (*p_launch_tvd_x_lines)(hipStream_t, unsigned, unsigned, size_t, const TvdLineArgs&, double*,
                                    const double*) = launch_tvd_x_lines;

@@ -27,7 +27,7 @@ using namespace exact;
 
 // the contracted build of the fused-step kernels (mrm_fast.hip)
 namespace fast {
-extern hipError_t (*p_launch_stage)(hipStream_t, unsigned, bool, const StageArgs&, const double*, const double*,
+extern hipError_t (*p_launch_stage)(hipStream_t, unsigned, int, const StageArgs&, const double*, const double*,
                                     const double*, double*, double*, const double*);
 extern hipError_t (*p_launch_tvd_x_lines)(hipStream_t, unsigned, unsigned, size_t, const TvdLineArgs&, double*,
                                           const double*);
@@ -61,7 +61,7 @@ struct mrm_ctx {
   bool force_sequence = false;
   bool tvd_x_march = false;                      // dim-0 TVD always by the marching kernel
   size_t lds_max = 64 * 1024;                    // LDS bytes one block may use (device attribute)
-  bool stage_zwin = false;                       // stage kernel variant
+  int stage_variant = 2;                         // stage kernel: 2 shared fluxes, 1 register window, 0 re-read
   bool fma = false;                              // fused step uses the contracted build (MRM_FMA=1)
   double last_dt = 0.0;                          // dt and TVD split of the last fused step
   bool last_split = false;
@@ -324,7 +324,7 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   const char* path = getenv("MRM_STEP_PATH");
   c->force_sequence = path && strcmp(path, "sequence") == 0;
   const char* zw = getenv("MRM_STAGE_ZWIN");
-  c->stage_zwin = !(zw && zw[0] == '0');
+  c->stage_variant = (zw && zw[0] >= '0' && zw[0] <= '2') ? zw[0] - '0' : 2;
   const char* fm = getenv("MRM_FMA");
   c->fma = fm && fm[0] == '1';      // contracted build on request only (the exact one is the parity build)
   const char* tx = getenv("MRM_TVD_X");
@@ -1684,8 +1684,8 @@ static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const do
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
   unsigned nb = (unsigned)((long)A.ntx * A.nty * A.nkc);
   TScope ts(c, T_NAMED);
-  hipError_t e = c->fma ? fast::p_launch_stage(c->stream, nb, c->stage_zwin, A, Uin, Pin, U0, Uout, Pout, Q)
-                        : exact::launch_stage(c->stream, nb, c->stage_zwin, A, Uin, Pin, U0, Uout, Pout, Q);
+  hipError_t e = c->fma ? fast::p_launch_stage(c->stream, nb, c->stage_variant, A, Uin, Pin, U0, Uout, Pout, Q)
+                        : exact::launch_stage(c->stream, nb, c->stage_variant, A, Uin, Pin, U0, Uout, Pout, Q);
   if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "k_stage", e);
   return MRM_OK;
 }
