@@ -210,9 +210,37 @@ int mrm_clean_divB(mrm_ctx* ctx, const mrm_run_desc* run, double err[2]);
 /* exchange_grid_acc (exchange.H:8, exchange.C:532-782) on one rank:
  * periodic ghost layers of U in dims 1 and 2, filled on the device. */
 int mrm_exchange_grid(mrm_ctx* ctx);
-/* SetBoundaryConditions (icbc.H:10-12, boundary_pdmp_1_fftw3.C:81-493) with
- * bnd_pot = 0, plus set_p_s_bc (:496-565) on the first call. */
+/* SetBoundaryConditions (icbc.H:10-12, boundary_pdmp_1_fftw3.C:81-493),
+ * plus set_p_s_bc (:496-565) on the first call.  The top boundary holds the
+ * vertical field (phys.bnd[1] = bnd_pot = 0) or the potential field
+ * (bnd_pot > 0, FP64 contexts, after mrm_potential_kernels): B of the two
+ * top ghost layers and B_y, B_z of the top interior layer from b_ext
+ * (:402-474).  b_ext is recomputed from B_x of the top interior layer on
+ * every call with bnd_pot = 1, on calls with pt_update = 1 with bnd_pot > 1
+ * (:335-381), and on the first call after the kernels were loaded.
+ * mrm_set_boundary_conditions takes pt_update from stage as solver.C passes
+ * it: 1 at stage 0 (:81), 0 at stages 1-4 (:320), at stage 5 (the end of
+ * the step) 1 with bnd_pot <= 2 or run->iteration % (bnd_pot - 1) == 0
+ * (:355-363); mrm_set_boundary_conditions_pt takes the reference's argument. */
 int mrm_set_boundary_conditions(mrm_ctx* ctx, const mrm_run_desc* run, int stage);
+int mrm_set_boundary_conditions_pt(mrm_ctx* ctx, const mrm_run_desc* run, int stage, int pt_update);
+
+/* Kernels of the potential-field extrapolation (potential_sd_fftw3.C:83-183,
+ * read there from PSF-kernel-<c>.dat): 8 planes of gsize[2] x gsize[1]
+ * floats, dim 1 fastest, in the order z-1, z-2, x-0, x-1, x-2, y-0, y-1,
+ * y-2.  b_ext[q] is the circular convolution of B_x(lend) over the
+ * horizontal plane with kernel q (potential_sd_fftw3.C:178-216), formed
+ * through 2D DFTs on the device.  Needs dx[1] = dx[2] (:71-75), gsize[1]
+ * and gsize[2] in ghosts..2048 and products of 2, 3, 5 and 7.  FP64
+ * contexts.  Replaces the kernel reading and spectra of potential_ext_fftw3;
+ * mrm_potential_kernels_load reads the eight files of dir (each a 3-float
+ * header gsize[1], gsize[2], dx[0]/dx[1], checked against the grid as at
+ * :140-152, then the plane).  mrm_get_potential_bext copies b_ext (8 planes
+ * of n1 x n2 doubles with their periodic ghosts, boundary_pdmp_1_fftw3.C:
+ * 124-125) to the host. */
+int mrm_potential_kernels(mrm_ctx* ctx, const float* k8);
+int mrm_potential_kernels_load(mrm_ctx* ctx, const char* dir);
+int mrm_get_potential_bext(mrm_ctx* ctx, double* out);
 
 /* nsteps iterations of the ComputeSolution loop body (solver.C:109-365) with
  * RT and IO off: SaveCons, 4 x (ConsToPrim, Adjust_Valf_Max at stage 1,
@@ -247,6 +275,10 @@ int mrm_io_read_header(const char* dir, int iter, int gsize[3], double dx[3], do
                        double* va_max);
 int mrm_io_write_backup(const char* file, int iter, double time, double p_bc, double s_bc);
 int mrm_io_read_backup(const char* file, int* iter, double* time, double* p_bc, double* s_bc);
+/* The eight PSF-kernel-<c>.dat files of dir (potential_sd_fftw3.C:77-86,
+ * 129-163) into k8 in the order of mrm_potential_kernels; MRM_ERR_ARG when
+ * a file is missing or short, or a header differs from gx, gy, dx_ratio. */
+int mrm_io_read_psf_kernels(const char* dir, int gx, int gy, float dx_ratio, float* k8);
 
 /* Reverse mode of mrm_step.  The reference has no adjoint; these entry points
  * differentiate the step it defines (solver.C:109-365, as mrm_step runs it)

@@ -37,7 +37,13 @@ EXPORTS = [
     "mrm_adjoint_forward", "mrm_adjoint_backward",
     "mrm_timers", "mrm_get_timers", "mrm_timer_name",
     "mrm_branch_count", "mrm_get_branch_counts", "mrm_mhd_residual_noalloc",
+    "mrm_set_boundary_conditions_pt", "mrm_potential_kernels", "mrm_potential_kernels_load",
+    "mrm_get_potential_bext", "mrm_io_read_psf_kernels",
 ]
+
+# potential-field kernels in the order mrm_potential_kernels takes them
+# (potential_sd_fftw3.C:77-86)
+PSF_KERNELS = ("z-1", "z-2", "x-0", "x-1", "x-2", "y-0", "y-1", "y-2")
 NTIMER = 24
 
 # limiter branch accounting: names of the count slots (MRM_BR_*) and of the
@@ -165,6 +171,11 @@ def load(path: str = LIB_PATH):
         "mrm_mhd_residual_noalloc": ([C.POINTER(GridDesc), C.POINTER(PhysDesc), C.POINTER(RunDesc), C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
         "mrm_get_branch_counts": ([P, C.POINTER(C.c_ulonglong)], C.c_int),
+        "mrm_set_boundary_conditions_pt": ([P, C.POINTER(RunDesc), C.c_int, C.c_int], C.c_int),
+        "mrm_potential_kernels": ([P, C.POINTER(C.c_float)], C.c_int),
+        "mrm_potential_kernels_load": ([P, C.c_char_p], C.c_int),
+        "mrm_get_potential_bext": ([P, dp], C.c_int),
+        "mrm_io_read_psf_kernels": ([C.c_char_p, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -22,10 +22,10 @@ FLAGS = [
 # plain IEEE arithmetic in the reference's operation order (no FMA); the
 # fused-step kernels are also built with contraction in mrm_fast.hip; the
 # FP32-mixed kernels (mrm_mixed.hip) are contracted; the adjoint kernels
-# (mrm_adjoint.hip) are not
+# (mrm_adjoint.hip) and the potential-field FFTs (mrm_potential.hip) are not
 UNITS = [("mrm_step.hip", "-ffp-contract=off"), ("mrm_fast.hip", "-ffp-contract=fast"),
          ("mrm_mixed.hip", "-ffp-contract=fast"), ("mrm_adjoint.hip", "-ffp-contract=off"),
-         ("mrm_io.cpp", "-ffp-contract=off")]
+         ("mrm_potential.hip", "-ffp-contract=off"), ("mrm_io.cpp", "-ffp-contract=off")]
 
 
 def _run(cmd, verbose):

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(DIVB_TX * DIVB_TY) k_divb_update(DivbArgs A, d
         double m1 = uu[1], m2 = uu[2], m3 = uu[3], ei = uu[4], pr[5];
         cons_to_prim_cell<2>(X.E, uu[0], m1, m2, m3, ei, pr);
         X.Pnew[n] = pr[0];
-        cm[0] = smax(cm[0], point_cmax_cons(uu, pr[0], X.inv_va2max));
+        if (!(X.skip_top_cmax && t.c0 % s1 == G.le0)) cm[0] = smax(cm[0], point_cmax_cons(uu, pr[0], X.inv_va2max));
         if (X.valf) valf_cell(va, uu[0], m1, m2, m3, ei, pr[0], X.vlim95, X.elim95);
       }
 #pragma unroll

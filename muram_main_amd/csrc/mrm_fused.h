@@ -631,13 +631,24 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_fused(Geom G, EosDev E, BcPara
         cm[0] = smax(cm[0], point_cmax_cons(u2, pg2, inv_va2max));
       }
     }
-    // ---- top (boundary_pdmp_1_fftw3.C:402-474), vertical field
+    // ---- top (boundary_pdmp_1_fftw3.C:402-474), vertical or potential field
     else {
       long n1 = G.le0 + base, n2 = n1 - 1;
       double u1[MRM_NV], u2[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
+      double u0[MRM_NV];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) u0[q] = u1[q];
       bc_top_cells(B, u1, u2);
+      if (B.bext) {
+        // the top interior cell's new B_y, B_z; its speed enters cmax here
+        // (the last TVD sweep leaves it out, skip_top_cmax)
+        bc_top_potential(B, t, &u0[6], &u0[7], u1, u2);
+        U[6 * bs + n1] = u0[6];
+        U[7 * bs + n1] = u0[7];
+        if (cmax_out && inner) cm[0] = smax(cm[0], point_cmax_cons(u0, P[n1], inv_va2max));
+      }
       long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }

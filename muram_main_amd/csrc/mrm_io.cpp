@@ -202,3 +202,24 @@ extern "C" int mrm_io_read_backup(const char* file, int* iter, double* time, dou
   if (s_bc) *s_bc = in ? s : -1.0;
   return MRM_OK;
 }
+
+// PSF-kernel-<c>.dat (potential_sd_fftw3.C:77-86, 129-163): a header of
+// three floats (gsize[1], gsize[2], dx[0]/dx[1]), then the plane, dim 1
+// fastest; the files in the order z-1, z-2, x-0, x-1, x-2, y-0, y-1, y-2
+extern "C" int mrm_io_read_psf_kernels(const char* dir, int gx, int gy, float dx_ratio, float* k8) {
+  if (!dir || !k8 || gx < 1 || gy < 1) return MRM_ERR_ARG;
+  static const char* const names[8] = {"z-1", "z-2", "x-0", "x-1", "x-2", "y-0", "y-1", "y-2"};
+  std::string d(dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  const size_t plane = (size_t)gx * gy;
+  for (int q = 0; q < 8; q++) {
+    std::ifstream in(d + "PSF-kernel-" + names[q] + ".dat", std::ios::binary);
+    if (!in) return MRM_ERR_ARG;
+    float h[3];
+    in.read(reinterpret_cast<char*>(h), sizeof h);
+    if (!in || (int)h[0] != gx || (int)h[1] != gy || h[2] != dx_ratio) return MRM_ERR_ARG;
+    in.read(reinterpret_cast<char*>(k8 + q * plane), (std::streamsize)(plane * sizeof(float)));
+    if (!in) return MRM_ERR_ARG;
+  }
+  return MRM_OK;
+}

@@ -504,6 +504,24 @@ MRM_HD void bc_top_cells(const BcParams& B, double u1[MRM_NV], double u2[MRM_NV]
   u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
 }
 
+// potential-field top boundary (boundary_pdmp_1_fftw3.C:422-436), after
+// bc_top_cells: B_y, B_z of the top interior cell (*by, *bz) and the whole B
+// of both ghost cells from b_ext; il = j + k n1 is the column's offset in
+// each b_ext plane
+MRM_HD void bc_top_potential(const BcParams& B, long il, double* by, double* bz, double u1[MRM_NV],
+                             double u2[MRM_NV]) {
+  const double* b = B.bext + il;
+  const long p = B.bplane;
+  *by = b[2 * p];
+  *bz = b[5 * p];
+  u1[5] = b[0];
+  u1[6] = b[3 * p];
+  u1[7] = b[6 * p];
+  u2[5] = b[p];
+  u2[6] = b[4 * p];
+  u2[7] = b[7 * p];
+}
+
 // bottom ghost layers, all (j,k) incl. ghosts; psum[0..1] are the layer sums
 // of p written by k_sum_partials
 __global__ void __launch_bounds__(MRM_BLOCK) k_bc_bottom(Geom G, EosDev E, BcParams B, const double* __restrict__ psum,
@@ -536,6 +554,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) { u1[q] = U[q * bs + n1]; u2[q] = U[q * bs + n2]; }
   bc_top_cells(B, u1, u2);
+  if (B.bext) bc_top_potential(B, t, &U[6 * bs + n1], &U[7 * bs + n1], u1, u2);
   long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
@@ -687,7 +706,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
               valf_lds_cell(vacc + threadIdx.x, MRM_BLOCK, uu[0], v1, v2, v3, ei, pn, X.vlim95, X.elim95);
             }
             X.Pnew[nd] = pn;
-            cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
+            if (!(X.skip_top_cmax && lineX == G.le0)) cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
           }
 #pragma unroll
           for (int k = 0; k < MRM_NV; k++) U[k * bs + nd] = uu[k];

@@ -22,6 +22,7 @@
 #include "mrm_spitzer.h"
 #include "mrm_mixed_api.h"
 #include "mrm_adjoint_api.h"
+#include "mrm_potential_api.h"
 
 using namespace exact;
 
@@ -91,6 +92,11 @@ struct mrm_ctx {
   int bnd_ini = 1;
 
   double* d_prof = nullptr;  // hfb, hft, qft (3 * n0)
+  // potential-field top boundary (bnd_pot > 0): the FFT plan with the
+  // kernels' spectra, and b_ext (8 planes of n1 x n2), valid once updated
+  pot::Plan* pot = nullptr;
+  double* d_bext = nullptr;
+  bool bext_valid = false;
   int tvd_ini = 1;
 
   // Clean_divB (clean_divB.C:50): passes of the next call; the last call's
@@ -152,10 +158,11 @@ enum {
 static const char* const timer_names[MRM_NTIMER] = {
   "cons_to_prim", "valf_reduce", "residual", "cmax_faces", "source_integrate", "save_cons", "tcheck",
   "exchange", "bc_sums", "bc_bottom", "bc_top", "tvd_sweep_x", "tvd_sweep_y", "tvd_sweep_z",
-  "stage_fused", "eos_p", "clean_divb", "spitzer", "eos_temp", "t19", "t20", "t21", "t22", "t23"};
+  "stage_fused", "eos_p", "clean_divb", "spitzer", "eos_temp", "potential", "t20", "t21", "t22", "t23"};
 static const int T_DIVB = T_NAMED + 2;
 static const int T_SPT = T_NAMED + 3;
 static const int T_EOST = T_NAMED + 4;
+static const int T_POT = T_NAMED + 5;
 
 // brackets the launches of one timer id with events on the context's stream
 struct TScope {
@@ -243,7 +250,10 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
   if (phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
-  if (phys->dmp[0] != 0.0 || (int)phys->bnd[1] != 0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
+  if (phys->dmp[0] != 0.0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
+  // the potential-field top boundary: FP64 contexts
+  if ((int)phys->bnd[1] < 0) return MRM_ERR_ARG;
+  if ((int)phys->bnd[1] > 0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   if (phys->divB[0] > 0.0 && phys->divB[1] < 1.0) return MRM_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; d++) {
     if (grid->gsize[d] < 1 || grid->ghosts[d] != 2) return MRM_ERR_ARG;
@@ -362,6 +372,8 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   for (int q = 0; q < 2; q++) hipFree(c->Pm[q]);
   hipFree(c->dR); hipFree(c->dF); hipFree(c->dP2); hipFree(c->d_spt); hipFree(c->d_diag);
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part); hipFree(c->d_bcount);
+  pot::plan_destroy(c->pot);
+  hipFree(c->d_bext);
   for (double* p : c->adj_ckpt) hipFree(p);
   hipFree(c->adj_mem);
   if (c->h_red) hipHostFree(c->h_red);
@@ -1284,12 +1296,45 @@ static BcParams bc_params(const mrm_ctx* c) {
   B.eps_top = c->phys.bnd[3];
   B.cc = 1.0 / ((double)(c->grid.gsize[1] * c->grid.gsize[2]));
   B.open_top = (int)c->phys.bnd[0];
+  if ((int)c->phys.bnd[1] > 0) {
+    B.bext = c->d_bext;
+    B.bplane = (long)c->G.n1 * c->G.n2;
+  }
   return B;
+}
+
+// pt_update as solver.C passes it: 1 on the first call (stage 0), 0 inside
+// the step (stages 1-4), at the end of a step 1 with bnd_pot <= 2 and on
+// every (bnd_pot - 1)-th iteration otherwise (solver.C:81, 320, 355-363)
+static int pt_update_rule(const mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  const int pot = (int)c->phys.bnd[1];
+  if (stage == 0) return 1;
+  if (stage <= 4) return 0;
+  if (pot <= 2 || !run) return 1;
+  return !(run->iteration % (pot - 1));
+}
+
+// the potential-field extrapolation of the top boundary when it is due
+// (boundary_pdmp_1_fftw3.C:335-381): on every call with bnd_pot = 1, on
+// calls with pt_update with bnd_pot > 1, and whenever b_ext has not been
+// formed yet
+static int potential_due(mrm_ctx* c, const double* U, int pt_update) {
+  const int pot = (int)c->phys.bnd[1];
+  if (pot <= 0) return MRM_OK;
+  if (!c->pot || !c->pot->kernels)
+    return fail(c, MRM_ERR_STATE, "SetBoundaryConditions: bnd_pot > 0 needs the potential-field kernels "
+                                  "(mrm_potential_kernels)");
+  if (!(pot == 1 || pt_update == 1 || !c->bext_valid)) return MRM_OK;
+  TScope ts(c, T_POT);
+  hipError_t e = pot::plan_update(c->pot, c->stream, c->G, U, c->d_bext);
+  if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "potential field update", e);
+  c->bext_valid = true;
+  return MRM_OK;
 }
 
 // SetBoundaryConditions on buffer U.  With P the ghost pressures are written
 // too (fused step); with cmax the dim-0 ghost faces enter the cmax reduction.
-static int bc_buf(mrm_ctx* c, double* U, double* P, unsigned long long* cmax) {
+static int bc_buf(mrm_ctx* c, double* U, double* P, unsigned long long* cmax, int pt_update) {
   int rc;
   if (c->bnd_ini) {
     if ((c->p_bc < 0.0) || (c->s_bc < 0.0)) {
@@ -1301,6 +1346,8 @@ static int bc_buf(mrm_ctx* c, double* U, double* P, unsigned long long* cmax) {
   double* dsum = (double*)(c->d_red + RED_SUM0);
   rc = bottom_sums(c, 0, U, dsum);
   if (rc) return rc;
+  // bz0 is B_x(lend) before this call's top boundary (boundary_pdmp_1_fftw3.C:339-349)
+  if ((rc = potential_due(c, U, pt_update))) return rc;
   BcParams B = bc_params(c);
   const long nyz = (long)c->G.n1 * c->G.n2;
   if (P) {
@@ -1344,17 +1391,68 @@ static int bc_mx(mrm_ctx* c, MxState M, float* P, unsigned long long* cmax) {
 }
 
 extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  if (!c) return MRM_ERR_ARG;
+  return mrm_set_boundary_conditions_pt(c, run, stage, pt_update_rule(c, run, stage));
+}
+
+extern "C" int mrm_set_boundary_conditions_pt(mrm_ctx* c, const mrm_run_desc* run, int stage, int pt_update) {
   (void)stage;
   if (!c) return MRM_ERR_ARG;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
   if (int rc = diag_track(c, run)) return rc;
   if (c->precision == MRM_PREC_FP32_MIXED) return bc_mx(c, c->M[c->cur], nullptr, nullptr);
-  int rc = bc_buf(c, c->dU, nullptr, nullptr);
+  int rc = bc_buf(c, c->dU, nullptr, nullptr, pt_update);
   if (rc || !c->d_spt) return rc;
   const long nyz = (long)c->G.n1 * c->G.n2;
   k_spitzer_bc<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.bnd[3] > 0.0, c->dF + MRM_SFLX * c->G.bs);
   LAUNCHCHK(c, "k_spitzer_bc");
+  return MRM_OK;
+}
+
+// ------------------------------------------------------------------ potential field
+// the kernels of the potential-field extrapolation (potential_sd_fftw3.C:83-183):
+// plan, device b_ext, spectra
+extern "C" int mrm_potential_kernels(mrm_ctx* c, const float* k8) {
+  if (!c || !k8) return MRM_ERR_ARG;
+  NEED_FP64(c, "mrm_potential_kernels");
+  const int gx = c->grid.gsize[1], gy = c->grid.gsize[2];
+  if (gy > 1 && c->dx[1] != c->dx[2])
+    return fail(c, MRM_ERR_UNSUPPORTED, "mrm_potential_kernels: dx[1] = dx[2] required for the potential field");
+  if (gx < c->G.g1 || gy < c->G.g2)
+    return fail(c, MRM_ERR_UNSUPPORTED, "mrm_potential_kernels: horizontal extents below the ghost width");
+  if (!c->pot) {
+    std::string err;
+    int rc = pot::plan_create(&c->pot, gx, gy, err);
+    if (rc) return fail(c, rc, err.c_str());
+  }
+  if (!c->d_bext) {
+    const size_t bytes = (size_t)8 * c->G.n1 * c->G.n2 * sizeof(double);
+    HIPCHK(c, hipMalloc(&c->d_bext, bytes));
+    HIPCHK(c, hipMemsetAsync(c->d_bext, 0, bytes, c->stream));
+  }
+  hipError_t e = pot::plan_kernels(c->pot, k8, c->stream);
+  if (e != hipSuccess) return fail(c, MRM_ERR_HIP, "mrm_potential_kernels", e);
+  c->bext_valid = false;
+  return MRM_OK;
+}
+
+// the eight PSF-kernel-<c>.dat files of dir, headers checked against the grid
+extern "C" int mrm_potential_kernels_load(mrm_ctx* c, const char* dir) {
+  if (!c || !dir) return MRM_ERR_ARG;
+  const int gx = c->grid.gsize[1], gy = c->grid.gsize[2];
+  std::vector<float> k8((size_t)8 * gx * gy);
+  const int rc = mrm_io_read_psf_kernels(dir, gx, gy, (float)(c->dx[0] / c->dx[1]), k8.data());
+  if (rc) return fail(c, rc, "mrm_potential_kernels_load: missing PSF-kernel file or header inconsistent with the grid");
+  return mrm_potential_kernels(c, k8.data());
+}
+
+extern "C" int mrm_get_potential_bext(mrm_ctx* c, double* out) {
+  if (!c || !out) return MRM_ERR_ARG;
+  if (!c->d_bext) return fail(c, MRM_ERR_STATE, "mrm_get_potential_bext: no potential-field kernels loaded");
+  HIPCHK(c, hipMemcpyAsync(out, c->d_bext, (size_t)8 * c->G.n1 * c->G.n2 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRM_OK;
 }
 
@@ -1761,7 +1859,7 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
                            spt ? c->d_spt + 3 * G.bs : nullptr)))
       return rc;
     if ((rc = exchange_buf(c, st[s].out, st[s].pout))) return rc;
-    if ((rc = bc_buf(c, st[s].out, st[s].pout, nullptr))) return rc;
+    if ((rc = bc_buf(c, st[s].out, st[s].pout, nullptr, 0))) return rc;
     if (spt && (rc = spitzer_ghosts(c, sflx[(s + 1) % 2]))) return rc;
   }
   // TVD on U4 with the pressure of the stage-4 input (tvdlimit_SR.C:332 reads
@@ -1780,6 +1878,7 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   X.valf = va_adj ? c->d_red : nullptr;
   X.vlim95 = 0.95 * c->v_lim;
   X.elim95 = 0.95 * c->e_lim;
+  X.skip_top_cmax = (int)c->phys.bnd[1] > 0;
   c->last_dt = run->dt;
   c->last_split = !(cfl <= run->CFL_tvd);
   // with Clean_divB (between TVDlimit and TCheck, solver.C:341-349) its last
@@ -1795,7 +1894,7 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   }
   if (divb && (rc = clean_divb_dev(c, Uf, &X))) return rc;
   if ((rc = exchange_buf(c, Uf, Pc))) return rc;
-  if ((rc = bc_buf(c, Uf, Pc, cmax))) return rc;
+  if ((rc = bc_buf(c, Uf, Pc, cmax, pt_update_rule(c, run, 5)))) return rc;
   c->cur = i2;
   c->dU = c->S[i2];
   c->dU0 = c->S[(i2 + 1) % 3];
@@ -2118,7 +2217,7 @@ static int adj_recompute(mrm_ctx* c, int n, const double* Un, AdjBufs& B, double
     double* Po = s < 4 ? B.P[s] : nullptr;
     if ((rc = stage_launch(c, &r, s, B.X[s - 1], B.P[s - 1], B.X[0], B.X[s], Po))) return rc;
     if ((rc = exchange_buf(c, B.X[s], Po))) return rc;
-    if ((rc = bc_buf(c, B.X[s], Po, nullptr))) return rc;
+    if ((rc = bc_buf(c, B.X[s], Po, nullptr, 0))) return rc;
     if ((rc = bottom_sums(c, 0, B.X[s], B.small + 2 * (s - 1)))) return rc;
   }
   HIPCHK(c, hipMemcpyAsync(B.W, B.X[4], f8b, hipMemcpyDeviceToDevice, c->stream));
@@ -2141,7 +2240,7 @@ static int adj_recompute(mrm_ctx* c, int n, const double* Un, AdjBufs& B, double
   if ((rc = bottom_sums(c, 0, B.Z, B.small + 8))) return rc;
   if (Unext) {
     HIPCHK(c, hipMemcpyAsync(Unext, B.Z, f8b, hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = bc_buf(c, Unext, nullptr, nullptr))) return rc;
+    if ((rc = bc_buf(c, Unext, nullptr, nullptr, 0))) return rc;
   }
   return MRM_OK;
 }
@@ -2202,10 +2301,10 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0 || c->phys.params[4] > 0.0 ||
-      c->d_spt)
+      c->d_spt || (int)c->phys.bnd[1] > 0)
     return fail(c, MRM_ERR_UNSUPPORTED,
-                "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0, eta = 0 and no "
-                "Clean_divB or Spitzer conduction");
+                "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0, eta = 0, the "
+                "vertical-field top boundary and no Clean_divB or Spitzer conduction");
   for (int d = 0; d < 3; d++)
     if (c->lsize[d] < 4) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: interior extents below 4");
   for (double* p : c->adj_ckpt) hipFree(p);

@@ -129,6 +129,10 @@ struct DivbArgs {
 struct BcParams {
   double p_bc, s_bc, B_crit, eps_top, cc;
   int open_top;
+  // potential-field top boundary: b_ext (8 planes of bplane = n1 n2 doubles,
+  // boundary_pdmp_1_fftw3.C:422-436), or null for the vertical field
+  const double* bext = nullptr;
+  long bplane = 0;
 };
 
 struct TvdEpilogue {
@@ -140,6 +144,9 @@ struct TvdEpilogue {
   // reduction slots red[RED_VMAX..RED_CNT_E], or null; 0.95 v_lim, 0.95 e_lim
   unsigned long long* valf;
   double vlim95, elim95;
+  // the top interior layer (i = lend) stays out of cmax: the potential-field
+  // boundary rewrites its B_y, B_z afterwards and reduces its speed then
+  int skip_top_cmax = 0;
 };
 
 struct StageArgs {

@@ -158,9 +158,33 @@ class MhdGrid:
                   "mrm_get_branch_counts")
         return out
 
-    def SetBoundaryConditions(self, run, stage: int):
-        self._chk(self.lib.mrm_set_boundary_conditions(self.h, C.byref(run), stage),
-                  "SetBoundaryConditions")
+    def SetBoundaryConditions(self, run, stage: int, pt_update: int | None = None):
+        """SetBoundaryConditions(Run, Grid, Physics, stage, pt_update); pt_update
+        None takes it from stage as solver.C passes it"""
+        if pt_update is None:
+            self._chk(self.lib.mrm_set_boundary_conditions(self.h, C.byref(run), stage), "SetBoundaryConditions")
+        else:
+            self._chk(self.lib.mrm_set_boundary_conditions_pt(self.h, C.byref(run), stage, int(pt_update)),
+                      "SetBoundaryConditions")
+
+    # ------------------------------------------------------------ potential field
+    def potential_kernels(self, k8: np.ndarray):
+        """the 8 potential-field kernels, shape (8, gsize[2], gsize[1]) float32,
+        in the order abi.PSF_KERNELS"""
+        k = np.ascontiguousarray(k8, dtype=np.float32)
+        self._chk(self.lib.mrm_potential_kernels(self.h, k.ctypes.data_as(C.POINTER(C.c_float))),
+                  "mrm_potential_kernels")
+
+    def potential_kernels_load(self, directory: str):
+        """the kernels from the PSF-kernel-<c>.dat files of a run directory"""
+        self._chk(self.lib.mrm_potential_kernels_load(self.h, directory.encode()), "mrm_potential_kernels_load")
+
+    def potential_bext(self) -> np.ndarray:
+        """b_ext of the last potential update, shape (8, n2, n1)"""
+        n2, n1, _ = self.shape
+        out = np.empty((8, n2, n1), dtype=np.float64)
+        self._chk(self.lib.mrm_get_potential_bext(self.h, out.ctypes.data_as(_dp)), "mrm_get_potential_bext")
+        return out
 
     def step(self, run, dt_rad: float = 1.0e30, nsteps: int = 1):
         self._chk(self.lib.mrm_step(self.h, C.byref(run), dt_rad, nsteps), "step")

@@ -75,6 +75,13 @@ struct orc_ctx {
   /* diagnostics tvar1..8, Qres, Qvis (grid.C:319-329, grid.H:69-100), allocated at the
      first orc_diag_zero or first call with need_diagnostics */
   double* diag[10];
+  /* potential-field top boundary (boundary_pdmp_1_fftw3.C:113-141, 335-442;
+     potential_sd_fftw3.C:27-257): the 8 kernels (float, [gy][gx], file
+     order z-1, z-2, x-0, x-1, x-2, y-0, y-1, y-2) and b_ext, 8 planes of
+     n1 x n2 with periodic ghosts, computed once valid */
+  float* pkern;
+  double* bext;
+  int bext_valid;
   /* limiter branch accounting (orc_branch_count): MRM_BR_NSLOT slots */
   int bcount_on;
   double btie;
@@ -196,6 +203,8 @@ void orc_destroy(orc_ctx* c) {
   free(c->curlB);
   free(c->sflx0); free(c->Rflx); free(c->BgradT);
   for (int t = 0; t < 10; t++) free(c->diag[t]);
+  free(c->pkern);
+  free(c->bext);
   free(c);
 }
 
@@ -1459,7 +1468,80 @@ static void set_p_s_bc(orc_ctx* c) {
 }
 
 /* SetBoundaryConditions (boundary_pdmp_1_fftw3.C:81-493) with bnd_pot = 0  */
+/* ---- potential-field extrapolation of the top boundary, one rank.
+   b_ext[q] = the circular convolution of bz0 = B_x(lend) over the
+   horizontal plane with kernel q (potential_sd_fftw3.C:178-216 forms it as
+   IFFT(FFT(bz0) FFT(K_q) / (gx gy)); here the sum is taken directly),
+   written into the interior of plane q, then periodic ghosts in dim 1
+   (:218-228) and in dim 2 over the whole dim-1 range (y_exchange,
+   :405-486). */
+int orc_potential_kernels(orc_ctx* c, const float* k8) {
+  const size_t n = (size_t)c->gsize[1] * c->gsize[2];
+  if (!c->pkern) c->pkern = (float*)malloc(sizeof(float) * 8 * n);
+  if (!c->bext) c->bext = (double*)calloc((size_t)8 * c->n[1] * c->n[2], sizeof(double));
+  if (!c->pkern || !c->bext) return -1;
+  memcpy(c->pkern, k8, sizeof(float) * 8 * n);
+  c->bext_valid = 0;
+  return 0;
+}
+
+const double* orc_potential_bext(const orc_ctx* c) { return c->bext; }
+
+static void potential_update(orc_ctx* c) {
+  const int gx = c->gsize[1], gy = c->gsize[2], g1 = c->ghosts[1], g2 = c->ghosts[2];
+  const int n1 = c->n[1], n2 = c->n[2];
+  const long bs = c->bufsize;
+  double* bz = (double*)malloc(sizeof(double) * (size_t)gx * gy);
+  /* bz0[j + k lsize1] = B_x at lend (boundary_pdmp_1_fftw3.C:339-349) */
+  for (int k = 0; k < gy; k++)
+    for (int j = 0; j < gx; j++)
+      bz[j + (size_t)k * gx] = c->U[5 * bs + c->lend[0] + (j + c->lbeg[1]) * c->stride[1] + (k + c->lbeg[2]) * c->stride[2]];
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int q = 0; q < 8; q++)
+    for (int y = 0; y < gy; y++)
+      for (int x = 0; x < gx; x++) {
+        const float* K = c->pkern + (size_t)q * gx * gy;
+        double sum = 0.0;
+        for (int m2 = 0; m2 < gy; m2++) {
+          const int yy = (y - m2 + gy) % gy;
+          for (int m1 = 0; m1 < gx; m1++) {
+            const int xx = (x - m1 + gx) % gx;
+            sum += bz[m1 + (size_t)m2 * gx] * (double)K[xx + (size_t)yy * gx];
+          }
+        }
+        c->bext[(size_t)q * n1 * n2 + (size_t)(y + g2) * n1 + (x + g1)] = sum;
+      }
+  for (int q = 0; q < 8; q++) {
+    double* b = c->bext + (size_t)q * n1 * n2;
+    for (int y = g2; y < gy + g2; y++)
+      for (int i = 0; i < g1; i++) {
+        b[(size_t)y * n1 + i] = b[(size_t)y * n1 + i + gx];
+        b[(size_t)y * n1 + i + gx + g1] = b[(size_t)y * n1 + i + g1];
+      }
+    for (int j = 0; j < g2; j++)
+      for (int i = 0; i < n1; i++) {
+        b[(size_t)(j + gy + g2) * n1 + i] = b[(size_t)(j + g2) * n1 + i];
+        b[(size_t)j * n1 + i] = b[(size_t)(j + gy) * n1 + i];
+      }
+  }
+  free(bz);
+  c->bext_valid = 1;
+}
+
+/* pt_update of the end-of-step call (solver.C:355-362) */
+static int pt_update_rule(const orc_ctx* c, const mrm_run_desc* run, int stage) {
+  const int pot = (int)c->ph.bnd[1];
+  if (stage == 0) return 1;                       /* solver.C:81 */
+  if (stage <= MAXSTAGE) return 0;                /* solver.C:320 */
+  if (pot <= 2) return 1;
+  return !(run->iteration % (pot - 1));
+}
+
 void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage) {
+  orc_set_boundary_conditions_pt(c, run, stage, pt_update_rule(c, run, stage));
+}
+
+void orc_set_boundary_conditions_pt(orc_ctx* c, const mrm_run_desc* run, int stage, int pt_update) {
   (void)run; (void)stage;
   const long bs = c->bufsize;
   double* U = c->U;
@@ -1532,7 +1614,15 @@ void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage)
       for (int v = 0; v < NV; v++) { U[v * bs + g1] = u1[v]; U[v * bs + g2] = u2[v]; }
       if (c->Rflx) { c->fld[MRM_SFLX][g1] = 0.0; c->fld[MRM_SFLX][g2] = 0.0; }   /* :306-310 */
     }
-  /* top ghost layers (:402-474), vertical field (bnd_pot = 0) */
+  /* potential field (:335-381): b_ext recomputed on every call with
+     bnd_pot = 1, on calls with pt_update with bnd_pot > 1 */
+  const int POT = (int)c->ph.bnd[1];
+  const int pot_on = POT > 0 && c->bext && c->pkern;
+  if (pot_on && ((POT == 1) || (POT > 1 && pt_update == 1) || !c->bext_valid)) potential_update(c);
+  const long lgs = (long)c->n[1] * c->n[2];
+  /* top ghost layers (:402-474): vertical field (bnd_pot = 0, B_y and B_z
+     mirrored) or the potential field of b_ext, which also sets B_y, B_z of
+     the top interior layer */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = c->lbeg[2] - c->ghosts[2]; k <= c->lend[2] + c->ghosts[2]; k++)
     for (int j = c->lbeg[1] - c->ghosts[1]; j <= c->lend[1] + c->ghosts[1]; j++) {
@@ -1549,7 +1639,16 @@ void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage)
         u1[4] = eps_top * u1[0] + 0.5 * (u1[1] * u1[1] + u1[2] * u1[2] + u1[3] * u1[3]) / u1[0];
         u2[4] = eps_top * u2[0] + 0.5 * (u2[1] * u2[1] + u2[2] * u2[2] + u2[3] * u2[3]) / u2[0];
       }
-      u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+      if (pot_on) {
+        const long il = j + (long)k * c->n[1];
+        const double* b = c->bext;
+        U[6 * bs + n1] = b[il + 2 * lgs];
+        U[7 * bs + n1] = b[il + 5 * lgs];
+        u1[5] = b[il + 0 * lgs]; u1[6] = b[il + 3 * lgs]; u1[7] = b[il + 6 * lgs];
+        u2[5] = b[il + 1 * lgs]; u2[6] = b[il + 4 * lgs]; u2[7] = b[il + 7 * lgs];
+      } else {
+        u1[6] *= -1; u2[6] *= -1; u1[7] *= -1; u2[7] *= -1;
+      }
       long g1 = c->lend[0] + 1 + base, g2 = c->lend[0] + 2 + base;
       for (int v = 0; v < NV; v++) { U[v * bs + g1] = u1[v]; U[v * bs + g2] = u2[v]; }
       if (c->Rflx) {   /* heat flux (:455-466): symmetric under a hot plate, else zero */

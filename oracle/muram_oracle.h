@@ -46,6 +46,14 @@ void   orc_tcheck(orc_ctx* c, const mrm_run_desc* run);
 void   orc_tvdlimit(orc_ctx* c, const mrm_run_desc* run, double dt_tvd);
 void   orc_exchange_grid(orc_ctx* c);
 void   orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage);
+/* with the reference's pt_update argument (the potential-field update);
+   orc_set_boundary_conditions takes it from the stage as solver.C passes it */
+void   orc_set_boundary_conditions_pt(orc_ctx* c, const mrm_run_desc* run, int stage, int pt_update);
+/* potential-field kernels: 8 x gsize[2] x gsize[1] floats in the order
+   z-1, z-2, x-0, x-1, x-2, y-0, y-1, y-2 (the PSF-kernel-*.dat files without
+   their headers); b_ext: 8 planes of n1 x n2 doubles */
+int    orc_potential_kernels(orc_ctx* c, const float* k8);
+const double* orc_potential_bext(const orc_ctx* c);
 void   orc_clean_divb(orc_ctx* c, const mrm_run_desc* run, double err[2]);
 int    orc_step(orc_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps);
 /* diagnostics tvar1..8, Qres, Qvis (t = 0..9): zeroed by orc_diag_zero,

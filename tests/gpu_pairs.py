@@ -39,10 +39,13 @@ def make_box_state(table, box):
     return initial.make_state(tab, box)
 
 
-def start_oracle(table, box, phys, U, cfl=0.8, count=True):
+def start_oracle(table, box, phys, U, cfl=0.8, count=True, kernels=None):
+    """kernels: the potential-field kernels (8, gsize[2], gsize[1]) for bnd_pot > 0"""
     g = config.grid_desc(box.gsize, box.dx)
     o = Oracle(g, phys, table)
     ro = config.run_desc(CFL=cfl)
+    if kernels is not None:
+        o.potential_kernels(kernels)
     o.set_state(U)
     o.exchange_grid()
     o.set_boundary_conditions(ro, 0)
@@ -52,11 +55,17 @@ def start_oracle(table, box, phys, U, cfl=0.8, count=True):
 
 
 def start_gpu(table, box, phys, U, cfl=0.8, path="fused", fma="0", tvd_x=None, precision=abi.PREC_FP64,
-              count=True):
+              count=True, kernels=None):
+    """kernels: the potential-field kernels as an array, or the directory of
+    their PSF-kernel-<c>.dat files"""
     g = config.grid_desc(box.gsize, box.dx)
     with env(MRM_STEP_PATH=path, MRM_FMA=fma, MRM_TVD_X=tvd_x):
         m = MhdGrid(g, phys, precision)
     m.eos_init(table)
+    if isinstance(kernels, str):
+        m.potential_kernels_load(kernels)
+    elif kernels is not None:
+        m.potential_kernels(kernels)
     r = config.run_desc(CFL=cfl)
     m.set_state(U)
     m.exchange_grid()

@@ -60,6 +60,10 @@ class Oracle:
             L.orc_tvdlimit.argtypes = [P, C.POINTER(RunDesc), C.c_double]
             L.orc_exchange_grid.argtypes = [P]
             L.orc_set_boundary_conditions.argtypes = [P, C.POINTER(RunDesc), C.c_int]
+            L.orc_set_boundary_conditions_pt.argtypes = [P, C.POINTER(RunDesc), C.c_int, C.c_int]
+            L.orc_potential_kernels.argtypes = [P, C.POINTER(C.c_float)]
+            L.orc_potential_bext.argtypes = [P]
+            L.orc_potential_bext.restype = dp
             L.orc_step.argtypes = [P, C.POINTER(RunDesc), C.c_double, C.c_int]
             L.orc_clean_divb.argtypes = [P, C.POINTER(RunDesc), dp]
             L.orc_diag_zero.argtypes = [P]
@@ -109,6 +113,21 @@ class Oracle:
     def set_limits(self, lim):
         a = np.ascontiguousarray(lim, dtype=np.float64)
         self.L.orc_set_limits(self.h, _as_dp(a))
+
+    def potential_kernels(self, k8):
+        """the 8 kernels, (8, gsize[2], gsize[1]) float32"""
+        self._k8 = np.ascontiguousarray(k8, dtype=np.float32)
+        rc = self.L.orc_potential_kernels(self.h, self._k8.ctypes.data_as(C.POINTER(C.c_float)))
+        assert rc == 0, rc
+
+    def potential_bext(self):
+        """b_ext, (8, n2, n1) copy"""
+        n1, n2 = self.shape[1], self.shape[0]
+        p = self.L.orc_potential_bext(self.h)
+        return np.ctypeslib.as_array(p, shape=(8 * n1 * n2,)).reshape(8, n2, n1).copy()
+
+    def set_boundary_conditions_pt(self, run, stage, pt_update):
+        self.L.orc_set_boundary_conditions_pt(self.h, C.byref(run), stage, pt_update)
 
     def bc(self):
         p, s = C.c_double(), C.c_double()
