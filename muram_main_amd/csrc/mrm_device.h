@@ -551,16 +551,17 @@ MRM_HD void tcheck_vals(double u[MRM_NV], double rho_min, double eps_min, double
 
 // Sources, Boris correction, RK update and Tcheck of one cell
 // (Add_Sources_Integrate_SR.C:331-465).  up: primitive U at the cell; R: its
-// residual (modified); u0: U0.  Returns the new conservative state in out.
+// residual (modified); u0: U0; dmpi: the damping profile at the cell's
+// layer (0 with the damping off).  Returns the new conservative state in out.
 // DIAG: dg[0..3] = efaci, e_dmpi, boris3 and the energy before the clamp
 // (Add_Sources_Integrate_SR.C:168-170, 218, 292)
 template <bool DIAG = false>
 MRM_HD void source_integrate_cell(const SrcParams& P, const double up[MRM_NV],
                                                       double R[MRM_NV], const double u0[MRM_NV], double Qrad,
-                                                      double out[MRM_NV], double* dg = nullptr) {
+                                                      double dmpi, double out[MRM_NV], double* dg = nullptr) {
   double dn = P.invRdt * up[4];
   double efaci = dn / fmax(dn, fabs(Qrad));
-  dn = P.g * up[0] - 0.0;
+  dn = P.g * up[0] - dmpi;
   double e_dmpi = dn * up[1];
   R[1] += dn;
   R[4] += e_dmpi + Qrad * efaci;

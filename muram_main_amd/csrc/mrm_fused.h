@@ -218,7 +218,8 @@ k_stage(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ 
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) u0[q] = ld32(U0 + q * bs, node);
       double Qrad = Qtot ? ld32(Qtot, node) : 0.0;
-      source_integrate_cell(A.SP, w[2], acc, u0, Qrad, out);
+      const double dmpi = A.SP.dmp ? A.SP.dmp[i] : 0.0;
+      source_integrate_cell(A.SP, w[2], acc, u0, Qrad, dmpi, out);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) st32(Uout + q * bs, node, out[q]);
       if (A.write_p) {
@@ -389,7 +390,8 @@ k_stage_f(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) u0[q] = ld32(U0 + q * bs, node);
       double Qrad = Qtot ? ld32(Qtot, node) : 0.0;
-      source_integrate_cell(A.SP, w[2], acc, u0, Qrad, out);
+      const double dmpi = A.SP.dmp ? A.SP.dmp[i] : 0.0;
+      source_integrate_cell(A.SP, w[2], acc, u0, Qrad, dmpi, out);
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) st32(Uout + q * bs, node, out[q]);
       if (A.write_p) {

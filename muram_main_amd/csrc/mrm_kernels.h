@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_source_integrate(Geom G, SrcParam
   }
   double Qrad = Qtot ? Qtot[node] : 0.0;
   double dg[4];
-  source_integrate_cell<DIAG>(SP, up, r, u0, Qrad, out, dg);
+  source_integrate_cell<DIAG>(SP, up, r, u0, Qrad, SP.dmp ? SP.dmp[i] : 0.0, out, dg);
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) {
     U[q * bs + node] = out[q];
@@ -558,6 +558,59 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_bc_top(Geom G, BcParams B, double
   long g1 = G.le0 + 1 + base, g2 = G.le0 + 2 + base;
 #pragma unroll
   for (int q = 0; q < MRM_NV; q++) { U[q * bs + g1] = u1[q]; U[q * bs + g2] = u2[q]; }
+}
+
+// ---------------------------------------------------------------- damping
+// get_damping (Add_Sources_Integrate_SR.C:628-761) on one rank.  Layer sums
+// of rho v_x / hsize in two passes: k_damp_partials gives each wave 64
+// consecutive layers (coalesced in i) and one of DMP_SETS column sets
+// (columns set, set + DMP_SETS, ...), k_damp_finish adds the sets of a layer
+// in a fixed order and forms dmp[i] = sum hh, hh = min(1/tau_min,
+// (1/tau_ref) (sum / (vel_ref rho_max))^4).  PRIM: U holds velocities (after
+// ConsToPrim), else momenta, whose velocity is M (1/rho) as ConsToPrim forms
+// it.  FIRST also reduces max rho into *rmax (the first call's rho_max).
+constexpr int DMP_SETS = 1024;
+
+template <bool PRIM, bool FIRST>
+__global__ void __launch_bounds__(256) k_damp_partials(Geom G, const double* __restrict__ U,
+                                                       double* __restrict__ part, unsigned long long* rmax) {
+  const int nx = G.le0 - G.lb0 + 1, ny = G.le1 - G.lb1 + 1, nz = G.le2 - G.lb2 + 1;
+  const int lane = threadIdx.x & 63;
+  const int set = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int ii = blockIdx.x * 64 + lane;
+  const long ncol = (long)ny * nz;
+  const double hsize = (double)ncol;
+  double acc = 0.0, rm = 0.0;
+  if (ii < nx) {
+    const long i = G.lb0 + ii;
+    for (long c = set; c < ncol; c += DMP_SETS) {
+      const long node = i + (G.lb1 + c % ny) * G.s1 + (G.lb2 + c / ny) * G.s2;
+      const double d = U[node];
+      double v = U[G.bs + node];
+      if (!PRIM) v = v * (1.0 / d);
+      acc += v * d / hsize;
+      if (FIRST) rm = fmax(rm, d);
+    }
+    part[(long)set * G.n0 + i] = acc;
+  }
+  if (FIRST) {
+    for (int o = 32; o > 0; o >>= 1) rm = fmax(rm, __shfl_xor(rm, o, 64));
+    if (lane == 0) atomic_max_nonneg(rmax, rm);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_damp_finish(Geom G, const double* __restrict__ part,
+                                                     const unsigned long long* __restrict__ rmax, double tau_ref,
+                                                     double vel_ref, double tau_min, double* __restrict__ dmp) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ii > G.le0 - G.lb0) return;
+  const long i = G.lb0 + ii;
+  double sum = 0.0;
+  for (int set = 0; set < DMP_SETS; set++) sum += part[(long)set * G.n0 + i];
+  const double rho_max = __longlong_as_double((long long)*rmax);
+  double hh = tau_ref * pow(sum / (vel_ref * rho_max), 4.0);
+  hh = fmin(tau_min, hh);
+  dmp[i] = sum * hh;
 }
 
 // ---------------------------------------------------------------- TVD sweeps

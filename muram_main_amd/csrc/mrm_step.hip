@@ -97,6 +97,10 @@ struct mrm_ctx {
   pot::Plan* pot = nullptr;
   double* d_bext = nullptr;
   bool bext_valid = false;
+  // damping (dmp_switch): the profile dmp[i] (n0), rho_max's bits (one
+  // slot), DMP_SETS x n0 partial layer sums; rho_max is taken at the first call
+  double* d_damp = nullptr;
+  bool damp_ini = true;
   int tvd_ini = 1;
 
   // Clean_divB (clean_divB.C:50): passes of the next call; the last call's
@@ -158,11 +162,12 @@ enum {
 static const char* const timer_names[MRM_NTIMER] = {
   "cons_to_prim", "valf_reduce", "residual", "cmax_faces", "source_integrate", "save_cons", "tcheck",
   "exchange", "bc_sums", "bc_bottom", "bc_top", "tvd_sweep_x", "tvd_sweep_y", "tvd_sweep_z",
-  "stage_fused", "eos_p", "clean_divb", "spitzer", "eos_temp", "potential", "t20", "t21", "t22", "t23"};
+  "stage_fused", "eos_p", "clean_divb", "spitzer", "eos_temp", "potential", "damping", "t21", "t22", "t23"};
 static const int T_DIVB = T_NAMED + 2;
 static const int T_SPT = T_NAMED + 3;
 static const int T_EOST = T_NAMED + 4;
 static const int T_POT = T_NAMED + 5;
+static const int T_DAMP = T_NAMED + 6;
 
 // brackets the launches of one timer id with events on the context's stream
 struct TScope {
@@ -250,7 +255,9 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   if (precision != MRM_PREC_FP64 && precision != MRM_PREC_FP32_MIXED) return MRM_ERR_UNSUPPORTED;
   // physics outside the implemented path
   if (phys->params[6] > 0.0) return MRM_ERR_UNSUPPORTED;
-  if (phys->dmp[0] != 0.0 || phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
+  if (phys->rt_ext_cor != 0) return MRM_ERR_UNSUPPORTED;
+  // the damping: FP64 contexts
+  if ((int)phys->dmp[0] != 0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
   // the potential-field top boundary: FP64 contexts
   if ((int)phys->bnd[1] < 0) return MRM_ERR_ARG;
   if ((int)phys->bnd[1] > 0 && precision != MRM_PREC_FP64) return MRM_ERR_UNSUPPORTED;
@@ -374,6 +381,7 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   hipFree(c->d_eos); hipFree(c->d_prof); hipFree(c->d_red); hipFree(c->d_part); hipFree(c->d_bcount);
   pot::plan_destroy(c->pot);
   hipFree(c->d_bext);
+  hipFree(c->d_damp);
   for (double* p : c->adj_ckpt) hipFree(p);
   hipFree(c->adj_mem);
   if (c->h_red) hipHostFree(c->h_red);
@@ -1019,13 +1027,51 @@ static SrcParams src_params(const mrm_ctx* c, const mrm_run_desc* run, int stage
   SP.eps_min = c->phys.tchk[0];
   SP.vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   SP.eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  SP.dmp = (int)c->phys.dmp[0] ? c->d_damp : nullptr;
   return SP;
+}
+
+// get_damping at stage 1 (Add_Sources_Integrate_SR.C:105-108): the profile
+// from U (velocities with prim, else momenta); the first call also takes
+// rho_max.  A no-op with the damping off.
+static int damping_update(mrm_ctx* c, const double* U, bool prim) {
+  if (!(int)c->phys.dmp[0]) return MRM_OK;
+  const Geom& G = c->G;
+  if (!c->d_damp) {
+    const size_t bytes = ((size_t)G.n0 * (1 + DMP_SETS) + 1) * sizeof(double);
+    HIPCHK(c, hipMalloc(&c->d_damp, bytes));
+    HIPCHK(c, hipMemsetAsync(c->d_damp, 0, bytes, c->stream));
+  }
+  double* prof = c->d_damp;
+  unsigned long long* rmax = (unsigned long long*)(c->d_damp + G.n0);
+  double* part = c->d_damp + G.n0 + 1;
+  const int nx = G.le0 - G.lb0 + 1;
+  const dim3 grid((nx + 63) / 64, DMP_SETS / 4);
+  TScope ts(c, T_DAMP);
+  if (c->damp_ini) {
+    if (prim) k_damp_partials<true, true><<<grid, 256, 0, c->stream>>>(G, U, part, rmax);
+    else k_damp_partials<false, true><<<grid, 256, 0, c->stream>>>(G, U, part, rmax);
+    c->damp_ini = false;
+  } else {
+    if (prim) k_damp_partials<true, false><<<grid, 256, 0, c->stream>>>(G, U, part, rmax);
+    else k_damp_partials<false, false><<<grid, 256, 0, c->stream>>>(G, U, part, rmax);
+  }
+  LAUNCHCHK(c, "k_damp_partials");
+  k_damp_finish<<<nblk(nx), MRM_BLOCK, 0, c->stream>>>(G, part, rmax, 1.0 / c->phys.dmp[1], c->phys.dmp[2],
+                                                       1.0 / c->phys.dmp[3], prof);
+  LAUNCHCHK(c, "k_damp_finish");
+  return MRM_OK;
 }
 
 extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_source_integrate_tcheck");
   c->fast_valid = false;
+  // the damping profile of the step, from the primitive state of stage 1
+  if (stage == 1) {
+    int rc = damping_update(c, c->dU, true);
+    if (rc) return rc;
+  }
   SrcParams SP = src_params(c, run, stage);
   const double* Q = c->qtot_set ? c->dF + MRM_QTOT * c->G.bs : nullptr;
   DiagArgs DA;
@@ -1839,6 +1885,8 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   if ((rc = mrm_sync_timestep(c, run, dt_rad, eta_dt_limit(c, run, c->dxmin / cm), &cfl))) return rc;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
   double* U0 = c->S[i0];
+  // the damping profile of the step from U^n (stage 1's state)
+  if ((rc = damping_update(c, U0, false))) return rc;
   struct { double *in, *pin, *out, *pout; } st[4] = {
       {c->S[i0], Pc, c->S[i1], Pa}, {c->S[i1], Pa, c->S[i2], Pc},
       {c->S[i2], Pc, c->S[i1], Pa}, {c->S[i1], Pa, c->S[i2], nullptr}};
@@ -2301,10 +2349,10 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0 || c->phys.params[4] > 0.0 ||
-      c->d_spt || (int)c->phys.bnd[1] > 0)
+      c->d_spt || (int)c->phys.bnd[1] > 0 || (int)c->phys.dmp[0] != 0)
     return fail(c, MRM_ERR_UNSUPPORTED,
                 "mrm_adjoint_forward: the adjoint follows the fused step with va_adjust = 0, eta = 0, the "
-                "vertical-field top boundary and no Clean_divB or Spitzer conduction");
+                "vertical-field top boundary and no damping, Clean_divB or Spitzer conduction");
   for (int d = 0; d < 3; d++)
     if (c->lsize[d] < 4) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_adjoint_forward: interior extents below 4");
   for (double* p : c->adj_ckpt) hipFree(p);

@@ -63,6 +63,9 @@ struct EosDev {
 struct SrcParams {
   double wdt, invRdt, g, inv_va2max;
   double rho_min, eps_min, eps_max, vmax;
+  // damping profile dmp[i] of the step (Add_Sources_Integrate_SR.C:92-108),
+  // n0 doubles, or null with the damping off
+  const double* dmp = nullptr;
 };
 
 struct TvdParams {

@@ -82,6 +82,12 @@ struct orc_ctx {
   float* pkern;
   double* bext;
   int bext_valid;
+  /* damping (Add_Sources_Integrate_SR.C:92-108, 628-761): the vertical
+     profile dmp (n0, zero outside the interior) formed at stage 1, and
+     rho_max of the first call */
+  double* dmp;
+  double damp_rho_max;
+  int damp_ini;
   /* limiter branch accounting (orc_branch_count): MRM_BR_NSLOT slots */
   int bcount_on;
   double btie;
@@ -205,6 +211,7 @@ void orc_destroy(orc_ctx* c) {
   for (int t = 0; t < 10; t++) free(c->diag[t]);
   free(c->pkern);
   free(c->bext);
+  free(c->dmp);
   free(c);
 }
 
@@ -765,9 +772,45 @@ static inline void tcheck_cell(double* U, long bs, long node, double rho_min, do
   U[bs + node] = mx * s; U[2 * bs + node] = my * s; U[3 * bs + node] = mz * s;
 }
 
+/* get_damping (Add_Sources_Integrate_SR.C:628-761) on one rank: rho_max over
+   the interior at the first call; then per layer i the sum over the
+   horizontal plane of M_x rho / hsize (k outer, j inner, as the loop nest
+   accumulates), damped by hh = min(1/tau_min, (1/tau_ref) (mean / (vel_ref
+   rho_max))^4) */
+static void get_damping(orc_ctx* c) {
+  const long bs = c->bufsize;
+  const double* U = c->U;
+  const int hsize = c->gsize[1] * c->gsize[2];
+  const double tau_ref = 1.0 / c->ph.dmp[1], vel_ref = c->ph.dmp[2], tau_min = 1.0 / c->ph.dmp[3];
+  if (!c->damp_ini) {
+    double hh = 0.0;
+    for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
+      for (int j = c->lbeg[1]; j <= c->lend[1]; j++)
+        for (int i = c->lbeg[0]; i <= c->lend[0]; i++)
+          hh = fmax(hh, U[i + j * c->stride[1] + k * c->stride[2]]);
+    c->damp_rho_max = hh;
+    c->damp_ini = 1;
+  }
+  const int n0 = c->n[0];
+  double* lbuf = (double*)calloc(n0, sizeof(double));
+  for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
+    for (int j = c->lbeg[1]; j <= c->lend[1]; j++)
+      for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+        const long node = i + j * c->stride[1] + k * c->stride[2];
+        lbuf[i] += U[bs + node] * U[node] / hsize;
+      }
+  for (int i = c->lbeg[0]; i <= c->lend[0]; i++) {
+    double hh = tau_ref * pow(lbuf[i] / (vel_ref * c->damp_rho_max), 4);
+    hh = fmin(tau_min, hh);
+    c->dmp[i] = lbuf[i] * hh;
+  }
+  free(lbuf);
+}
+
 /* Source_Integrate_Tcheck, non-diagnostic branch                             */
-/* (Add_Sources_Integrate_SR.C:38-110 and :299-467), damping off, RT source   */
-/* from Qtot (+Qthin when ext_cor, which is outside the path).               */
+/* (Add_Sources_Integrate_SR.C:38-110 and :299-467) with the damping profile  */
+/* of stage 1 (:92-108), RT source from Qtot (+Qthin when ext_cor, which is   */
+/* outside the path).                                                         */
 void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage) {
   const long bs = c->bufsize;
   const double wdt = run->dt / (double)(MAXSTAGE + 1 - stage);
@@ -781,6 +824,9 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
   const double* Qtot = c->fld[MRM_QTOT];
   double** dg = diag_of(c, run);   /* the diagnostic branch, :122-296 */
   const double invdt = 1.0 / run->dt;
+  if (!c->dmp) c->dmp = (double*)calloc(c->n[0], sizeof(double));
+  if (stage == 1 && (int)c->ph.dmp[0]) get_damping(c);
+  const double* dmp = c->dmp;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = c->lbeg[2]; k <= c->lend[2]; k++)
     for (int j = c->lbeg[1]; j <= c->lend[1]; j++) {
@@ -790,7 +836,7 @@ void orc_source_integrate_tcheck(orc_ctx* c, const mrm_run_desc* run, int stage)
         double Qrad = Qtot[node];
         double dn = invRdt * U[4 * bs + node];
         double efaci = dn / fmax(dn, fabs(Qrad));
-        dn = g * U[node] - 0.0;   /* dmp[i] = 0 with damping off */
+        dn = g * U[node] - dmp[i];   /* gravity + damping (:164, :340) */
         double e_dmpi = dn * U[bs + node];
         R[bs + node] += dn;
         R[4 * bs + node] += e_dmpi + Qrad * efaci;
@@ -1467,7 +1513,6 @@ static void set_p_s_bc(orc_ctx* c) {
   }
 }
 
-/* SetBoundaryConditions (boundary_pdmp_1_fftw3.C:81-493) with bnd_pot = 0  */
 /* ---- potential-field extrapolation of the top boundary, one rank.
    b_ext[q] = the circular convolution of bz0 = B_x(lend) over the
    horizontal plane with kernel q (potential_sd_fftw3.C:178-216 forms it as
@@ -1537,6 +1582,8 @@ static int pt_update_rule(const orc_ctx* c, const mrm_run_desc* run, int stage) 
   return !(run->iteration % (pot - 1));
 }
 
+/* SetBoundaryConditions (boundary_pdmp_1_fftw3.C:81-493) with pt_update as
+   solver.C passes it */
 void orc_set_boundary_conditions(orc_ctx* c, const mrm_run_desc* run, int stage) {
   orc_set_boundary_conditions_pt(c, run, stage, pt_update_rule(c, run, stage));
 }

@@ -275,7 +275,7 @@ static void run_all(const EosDev& E) {
     const double Qrad = r == 0 ? 0.0 : -1e-2 * w[4];
     check(r == 0 ? "source_integrate" : "source_integrate (Qrad)", x, sc, 8,
           [&](const Vec& xx) { double R[8], out[8]; for (int q = 0; q < 8; q++) R[q] = xx[8 + q];
-                               source_integrate_cell(SP, &xx[0], R, &xx[16], Qrad, out); return Vec(out, out + 8); },
+                               source_integrate_cell(SP, &xx[0], R, &xx[16], Qrad, 0.0, out); return Vec(out, out + 8); },
           [&](const Vec& xx, const Vec& l) { Vec lx(24, 0.0);
                                              source_integrate_vjp(SP, &xx[0], &xx[8], &xx[16], Qrad, l.data(), &lx[0], &lx[8], &lx[16]);
                                              return lx; });

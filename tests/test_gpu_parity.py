@@ -271,10 +271,10 @@ def test_error_codes(gold, table):
     r2.dt = 5.0
     with pytest.raises(abi.MrmError, match="CFL"):
         m.Sync_timestep(r2, 1e30, 1.0)
-    # physics outside the implemented set: ambipolar diffusion, damping; the
-    # potential-field top boundary on an FP32-mixed context
+    # physics outside the implemented set: ambipolar diffusion; the damping
+    # and the potential-field top boundary on an FP32-mixed context
     for setter, prec in ((lambda p: p.params.__setitem__(config.I_PARAM_AMBIPOLAR, 1.0), abi.PREC_FP64),
-                         (lambda p: p.dmp.__setitem__(0, 1.0), abi.PREC_FP64),
+                         (lambda p: p.dmp.__setitem__(0, 1.0), abi.PREC_FP32_MIXED),
                          (lambda p: p.bnd.__setitem__(config.I_BND_POT, 2.0), abi.PREC_FP32_MIXED)):
         bad = config.stratified_phys()
         setter(bad)
