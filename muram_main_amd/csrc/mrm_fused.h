@@ -412,6 +412,230 @@ k_stage_f(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
   }
 }
 
+// ------------------------------------------------------------------ stage kernel, two wave groups
+// k_stage_f's work split over two groups of four waves on the same tile, so
+// that each group needs about half of the registers and the block runs two
+// waves per SIMD (k_stage_f: one):
+//   xy group (threads 0 .. 255), plane kbeg + m in iteration m: packs the
+//     dim-0 and dim-1 point fluxes of its point and ring point into LDS and
+//     forms the x and y residual terms (and their part of J), which it hands
+//     to the z group in LDS; it also forms the EOS pressure of the new state
+//     of plane kbeg + m - 2, which the z group handed over;
+//   z group (threads 256 .. 511), plane kbeg + n - 1 in iteration n: keeps
+//     the five-plane dim-2 window, adds the dim-2 terms to the xy group's sum
+//     and applies sources, Boris correction, RK update and Tcheck.
+// Each iteration has two barriers; the handover buffers are double
+// buffered by iteration parity.  The sums are formed in k_stage_f's order
+// ((0 + R_x) + R_y) + R_z, so the result is bit for bit k_stage_f's.
+template <bool ETA = false, bool ERES = false>
+__global__ void __launch_bounds__(2 * ST_SX * ST_SY, 1)
+k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ Pin, const double* __restrict__ U0,
+          double* __restrict__ Uout, double* __restrict__ Pout, const double* __restrict__ Qtot) {
+  constexpr int NFP = nfp_<ETA>();
+  constexpr int NT = ST_SX * ST_SY;
+  constexpr int NA = ETA ? MRM_NV + 3 : MRM_NV;   // x + y sums, and J with ETA
+  static_assert(4 * ST_SX + 4 * ST_SY <= NT, "one ring point per thread at most");
+  __shared__ double fx[NFP][ST_SY][ST_LX];   // dim-0 fluxes of row ty, columns i0-2 .. i0+SX+1
+  __shared__ double fy[NFP][ST_LY][ST_SX];   // dim-1 fluxes of column tx, rows j0-2 .. j0+SY+1
+  __shared__ double axy[2][NA][NT];          // xy group -> z group
+  __shared__ double aout[2][5][NT];          // z group -> xy group: the new rho, M, e
+  const Geom& G = A.G;
+  const long bs = G.bs;
+  const long nblocks = (long)A.ntx * A.nty * A.nkc;
+  long lb = xcd_remap(blockIdx.x, nblocks);
+  const int tx_t = (int)(lb % A.ntx);
+  lb /= A.ntx;
+  const int ty_t = (int)(lb % A.nty);
+  const int kc = (int)(lb / A.nty);
+  const bool zg = threadIdx.x >= NT;
+  const int t = zg ? threadIdx.x - NT : threadIdx.x;
+  const int tx = t % ST_SX, ty = t / ST_SX;
+  const int i0 = G.lb0 + tx_t * ST_SX, j0 = G.lb1 + ty_t * ST_SY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = (i <= G.le0) && (j <= G.le1);
+  const int kbeg = G.lb2 + kc * ST_KC;
+  const int kend = min(kbeg + ST_KC, G.le2 + 1);     // exclusive
+  const int NK = kend - kbeg;
+  const bool own_ok = i < G.n0 && j < G.n1;
+  const unsigned col = (unsigned)i + (unsigned)j * (unsigned)G.s1;
+  const unsigned s2u = (unsigned)G.s2;
+  if (!zg) {
+    // ---- xy group: the ring point of this thread as in k_stage_f
+    int ring = 0;
+    int r_row = 0, r_col = 0, r_i = 0, r_j = 0;
+    if (t < 4 * ST_SX) {
+      const int rr_ = t / ST_SX;
+      r_row = rr_ < 2 ? rr_ : ST_SY + rr_;
+      r_col = t % ST_SX;
+      r_i = i0 + r_col;
+      r_j = j0 - 2 + r_row;
+      ring = 1;
+    } else if (t < 4 * ST_SX + 4 * ST_SY) {
+      const int u = t - 4 * ST_SX;
+      r_row = u / 4;
+      const int c4 = u % 4;
+      r_col = c4 < 2 ? c4 : ST_SX + c4;
+      r_i = i0 - 2 + r_col;
+      r_j = j0 + r_row;
+      ring = 2;
+    }
+    const bool ring_ok = ring != 0 && r_i < G.n0 && r_j < G.n1;
+    const unsigned rcol = (unsigned)r_i + (unsigned)r_j * (unsigned)G.s1;
+    double rc[ST_NF], rr[ST_NF];   // raw values of the own point and the ring point, plane kbeg + m
+    if (own_ok) load_raw(Uin, Pin, bs, col + (unsigned)kbeg * s2u, rc);
+    if (ring_ok) load_raw(Uin, Pin, bs, rcol + (unsigned)kbeg * s2u, rr);
+    for (int m = 0; m <= NK + 1; m++) {
+      double c[ST_NF];
+      if (m < NK) {
+        if (own_ok) prim_of_raw(rc, c);
+        else {
+#pragma unroll
+          for (int f = 0; f < ST_NF; f++) c[f] = 1.0;
+        }
+        double o[14];
+        flux_pack<0, ETA>(c, o);
+#pragma unroll
+        for (int f = 0; f < NFP; f++) fx[f][ty][tx + 2] = o[f];
+        flux_pack<1, ETA>(c, o);
+#pragma unroll
+        for (int f = 0; f < NFP; f++) fy[f][ty + 2][tx] = o[f];
+        if (ring != 0) {
+          double h[ST_NF];
+          if (ring_ok) prim_of_raw(rr, h);
+          else {
+#pragma unroll
+            for (int f = 0; f < ST_NF; f++) h[f] = 1.0;
+          }
+          if (ring == 1) {
+            flux_pack<1, ETA>(h, o);
+#pragma unroll
+            for (int f = 0; f < NFP; f++) fy[f][r_row][r_col] = o[f];
+          } else {
+            flux_pack<0, ETA>(h, o);
+#pragma unroll
+            for (int f = 0; f < NFP; f++) fx[f][r_row][r_col] = o[f];
+          }
+        }
+        if (m + 1 < NK) {
+          if (own_ok) load_raw(Uin, Pin, bs, col + (unsigned)(kbeg + m + 1) * s2u, rc);
+          if (ring_ok) load_raw(Uin, Pin, bs, rcol + (unsigned)(kbeg + m + 1) * s2u, rr);
+        }
+      }
+      // the pressure of plane kbeg + m - 2's new state (handed over in iteration m - 1)
+      const bool eos = A.write_p && valid && m >= 2;
+      EosLook<2> pg;
+      if (eos) {
+        const int sl = (m - 1) & 1;
+        double d = aout[sl][0][t], m1 = aout[sl][1][t], m2 = aout[sl][2][t], m3 = aout[sl][3][t],
+               e = aout[sl][4][t];
+        eos_issue<2>(A.E, d, m1, m2, m3, e, pg);
+      }
+      __syncthreads();
+      if (m < NK && valid) {
+        double acc[MRM_NV], res[MRM_NV];
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) acc[q] = 0.0;
+        const double lfc = lfac_of(c, A.RP.inv_va2max);
+        double J[3] = {0.0, 0.0, 0.0};
+        auto getx = [&](int s, int f) { return fx[f][ty][tx + s]; };
+        sweep_residual_packed<0, ETA>(getx, c, A.RP.w1[0], A.RP.w2[0], lfc, res, &A.RP, J);
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+        auto gety = [&](int s, int f) { return fy[f][ty + s][tx]; };
+        sweep_residual_packed<1, ETA>(gety, c, A.RP.w1[1], A.RP.w2[1], lfc, res, &A.RP, J);
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+        const int sl = m & 1;
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) axy[sl][q][t] = acc[q];
+        if (ETA) {
+#pragma unroll
+          for (int q = 0; q < 3; q++) axy[sl][(MRM_NV + q) % NA][t] = J[q];
+        }
+      }
+      if (eos) {
+        double pr[5];
+        eos_finish<2>(A.E, pg, pr);
+        st32(Pout, col + (unsigned)(kbeg + m - 2) * s2u, pr[0]);
+      }
+      __syncthreads();
+    }
+  } else {
+    // ---- z group
+    double w[5][ST_NF];   // planes k-2 .. k+2 of this thread's column
+    double rz[ST_NF];     // raw values of plane k + 3
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int kk = kbeg - 2 + q;
+      if (own_ok) { double r[ST_NF]; load_raw(Uin, Pin, bs, col + (unsigned)kk * s2u, r); prim_of_raw(r, w[q + 1]); }
+      else {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0;
+      }
+    }
+    if (own_ok) load_raw(Uin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
+    auto getz = [&](int s, double o[9]) {
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) o[f] = w[s][f];
+    };
+    for (int n = 0; n <= NK + 1; n++) {
+      const int k = kbeg + n - 1;
+      const bool act = n >= 1 && n <= NK;
+      double res[MRM_NV];
+      if (act) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
+        if (own_ok) {
+          prim_of_raw(rz, w[4]);     // plane k + 2
+          // k + 3 <= kend + 1 <= lend2 + 2 = n2 - 1: inside the buffer
+          if (k + 1 < kend) load_raw(Uin, Pin, bs, col + (unsigned)(k + 3) * s2u, rz);
+        }
+        // without resistivity the dim-2 terms do not depend on the xy
+        // group's sums: form them while that group packs its fluxes (with
+        // it, J passes through the three sweeps in order, so they wait)
+        if (!ETA && valid) {
+          const double lfc = lfac_of(w[2], A.RP.inv_va2max);
+          sweep_residual_lr<2, false>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res, &A.RP, nullptr);
+        }
+      }
+      __syncthreads();
+      if (act && valid) {
+        const int sl = (n - 1) & 1;
+        double acc[MRM_NV];
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) acc[q] = axy[sl][q][t];
+        double J[3] = {0.0, 0.0, 0.0};
+        if (ETA) {
+#pragma unroll
+          for (int q = 0; q < 3; q++) J[q] = axy[sl][(MRM_NV + q) % NA][t];
+          const double lfc = lfac_of(w[2], A.RP.inv_va2max);
+          sweep_residual_lr<2, ETA>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res, &A.RP, J);
+        }
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+        if (ETA) acc[4] += A.RP.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+        const unsigned node = col + (unsigned)k * s2u;
+        if (ERES) acc[4] += ld32(A.eres, node);
+        double u0[MRM_NV], out[MRM_NV];
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) u0[q] = ld32(U0 + q * bs, node);
+        double Qrad = Qtot ? ld32(Qtot, node) : 0.0;
+        const double dmpi = A.SP.dmp ? A.SP.dmp[i] : 0.0;
+        source_integrate_cell(A.SP, w[2], acc, u0, Qrad, dmpi, out);
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) st32(Uout + q * bs, node, out[q]);
+        if (A.write_p) {
+#pragma unroll
+          for (int q = 0; q < 5; q++) aout[n & 1][q][t] = out[q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ------------------------------------------------------------------ drop-in residual, tiled
 // MHD_Residual of the per-operation path (mhdres_SR.C:27-854) with the stage
 // kernel's tiling: U holds ConsToPrim's primitives, P the pressure; the

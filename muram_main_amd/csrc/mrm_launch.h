@@ -16,12 +16,20 @@ inline void launch_stage_z(hipStream_t st, unsigned nb, const StageArgs& A, cons
   else k_stage<ZWIN><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
 }
 
-// variant 2: fluxes shared through LDS (k_stage_f); 1: register window of
+// variant 3: k_stage_f's work over two wave groups (k_stage_w); 2: fluxes
+// shared through LDS (k_stage_f); 1: register window of
 // primitives (k_stage<true>); 0: dim-2 neighbours re-read (k_stage<false>)
 inline hipError_t launch_stage(hipStream_t st, unsigned nb, int variant, const StageArgs& A, const double* Uin,
                                const double* Pin, const double* U0, double* Uout, double* Pout,
                                const double* Q) {
-  if (variant == 2) {
+  if (variant == 3) {
+    const bool eta = A.RP.eta > 0.0, eres = A.eres != nullptr;
+    const unsigned nt = 2 * ST_SX * ST_SY;
+    if (eta && eres) k_stage_w<true, true><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+    else if (eta) k_stage_w<true><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+    else if (eres) k_stage_w<false, true><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+    else k_stage_w<><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  } else if (variant == 2) {
     const bool eta = A.RP.eta > 0.0, eres = A.eres != nullptr;
     if (eta && eres) k_stage_f<true, true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
     else if (eta) k_stage_f<true><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);

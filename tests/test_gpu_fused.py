@@ -66,7 +66,8 @@ def compare(o, ro, m, r):
     assert r.time == pytest.approx(ro.time, rel=TOL)
 
 
-@pytest.mark.parametrize("zwin,fma", [("0", "0"), ("1", "0"), ("2", "0"), ("0", "1"), ("1", "1"), ("2", "1")])
+@pytest.mark.parametrize("zwin,fma", [("0", "0"), ("1", "0"), ("2", "0"), ("3", "0"), ("0", "1"), ("1", "1"), ("2", "1"),
+                                      ("3", "1")])
 def test_fused_steps_match_oracle(table, zwin, fma):
     print(f"case: fused 40x36x28 zwin={zwin} fma={fma}", flush=True)
     o, ro, m, r = setup_pair(table, (40, 36, 28), seed=31, zwin=zwin, fma=fma)
