@@ -164,7 +164,20 @@ int mrm_cons_to_prim(mrm_ctx* ctx);
 int mrm_adjust_valf_max(mrm_ctx* ctx, const mrm_run_desc* run, double stats[5]);
 
 /* MHD_Residual (mhd_tvd.H:5, mhdres_SR.C:27-854): Res += -div F + Lorentz
- * force over the interior; returns dt_cfl = dxmin / cmax. */
+ * force over the interior; returns dt_cfl = dxmin / cmax (dt_cfl may be
+ * NULL: no host round trip then).
+ *
+ * Deferred stage: SaveCons, ConsToPrim and MHD_Residual are recorded rather
+ * than run when the next calls allow it (FP64, no diagnostics, no Spitzer
+ * conduction; ConsToPrim stays recorded through Adjust_Valf_Max only with
+ * va_adjust = 0 and stats NULL).  MHD_Residual then forms ConsToPrim's EOS
+ * fields and cmax (what it returns), and the next Source_Integrate_Tcheck
+ * runs the whole stage -- residual, sources, RK update, Tcheck -- in one
+ * kernel from the conservative state.  The result is that of the calls run
+ * one by one, to rounding; any other call (a state or field read included)
+ * first carries the recorded operations out, so every call sees what it
+ * would see without the deferral.  MRM_DEFER=0 (read at mrm_create) runs
+ * every call as it comes. */
 int mrm_mhd_residual(mrm_ctx* ctx, const mrm_run_desc* run, double* dt_cfl);
 
 /* MHD_Residual on caller-owned device arrays, allocating nothing.  It has no

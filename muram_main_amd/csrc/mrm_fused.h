@@ -940,6 +940,75 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p(Geom G, EosDev E, const dou
   }
 }
 
+// ConsToPrim's five EOS outputs (p, T, ne, rhoi, amb) of the conservative
+// state U written to F without converting U, and the characteristic speed of
+// the points MHD_Residual's sweeps visit (k_eos_p's set) into cmax_out: what
+// the host must see of a ConsToPrim + MHD_Residual pair before the stage
+// kernel does the rest (the drop-in path's deferred stage)
+__global__ void __launch_bounds__(MRM_BLOCK) k_eos_fields(Geom G, EosDev E, const double* __restrict__ U,
+                                                          double* __restrict__ F, double inv_va2max,
+                                                          unsigned long long* cmax_out) {
+  __shared__ double lds[MRM_BLOCK / 64];
+  const long bs = G.bs;
+  double cm[1] = {0.0};
+  for (long n = blockIdx.x * (long)blockDim.x + threadIdx.x; n < bs; n += (long)gridDim.x * blockDim.x) {
+    double u[MRM_NV];
+#pragma unroll
+    for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + n];
+    double m1 = u[1], m2 = u[2], m3 = u[3], ei = u[4], pr[5];
+    cons_to_prim_cell<5>(E, u[0], m1, m2, m3, ei, pr);
+#pragma unroll
+    for (int q = 0; q < 5; q++) F[q * bs + n] = pr[q];
+    bool inner, vis;
+    cmax_cell_class(G, n, inner, vis);
+    if (vis) cm[0] = smax(cm[0], point_cmax_cons(u, pr[0], inv_va2max));
+  }
+  block_reduce<OpMax, 1>(cm, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
+}
+
+// every ghost cell (outside the interior) of the conservative state src
+// written to dst in ConsToPrim's primitive form (v = M (1/rho),
+// e_int = e - rho v^2 / 2, cons_to_prim.C:110-118; rho and B as they are): the
+// ghost cells a ConsToPrim followed by an in-place Source_Integrate_Tcheck
+// leaves.  The ghost width is 2 in every dimension.  Index space: the dim-0
+// ghost layers over all (j, k), then the dim-1 ghost rows over all (i, k),
+// then the dim-2 ghost planes over all (i, j); edges and corners are written
+// more than once, with the same values.
+__global__ void __launch_bounds__(MRM_BLOCK) k_ghosts_prim(Geom G, const double* __restrict__ src,
+                                                           double* __restrict__ dst) {
+  const long na = 4L * G.n1 * G.n2, nb = 4L * G.n0 * G.n2, nc = 4L * G.n0 * G.n1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t < na) {
+    const int l = (int)(t % 4);
+    const long r = t / 4;
+    j = (int)(r % G.n1); k = (int)(r / G.n1); i = l < 2 ? l : G.n0 - 4 + l;
+  } else if ((t -= na) < nb) {
+    const int l = (int)(t % 4);
+    const long r = t / 4;
+    i = (int)(r % G.n0); k = (int)(r / G.n0); j = l < 2 ? l : G.n1 - 4 + l;
+  } else if ((t -= nb) < nc) {
+    const int l = (int)(t % 4);
+    const long r = t / 4;
+    i = (int)(r % G.n0); j = (int)(r / G.n0); k = l < 2 ? l : G.n2 - 4 + l;
+  } else {
+    return;
+  }
+  const long n = i + j * G.s1 + k * G.s2, bs = G.bs;
+  const double d = src[n];
+  const double dn = 1.0 / d;
+  const double m1 = src[bs + n] * dn, m2 = src[2 * bs + n] * dn, m3 = src[3 * bs + n] * dn;
+  const double vv = (m1 * m1 + m2 * m2 + m3 * m3);
+  dst[n] = d;
+  dst[bs + n] = m1;
+  dst[2 * bs + n] = m2;
+  dst[3 * bs + n] = m3;
+  dst[4 * bs + n] = src[4 * bs + n] - 0.5 * vv * d;
+#pragma unroll
+  for (int q = 5; q < MRM_NV; q++) dst[q * bs + n] = src[q * bs + n];
+}
+
 // cmax alone over the same points, from a state and its pressure: the
 // stage-1 speed bound after Adjust_Valf_Max has changed inv_va2max
 __global__ void __launch_bounds__(MRM_BLOCK) k_cmax_visited(Geom G, const double* __restrict__ U,

@@ -51,7 +51,16 @@ struct mrm_ctx {
   double* S[3] = {nullptr, nullptr, nullptr};   // state buffers (fused step rotates them)
   int cur = 0;                                   // S[cur] is the current state U
   double* dU = nullptr;                          // = S[cur]
-  double* dU0 = nullptr;                         // = S[(cur+1)%3]
+  double* dU0 = nullptr;                         // = S[c0]
+  int c0 = 1;
+  // the drop-in path's deferred stage (MRM_DEFER=0 turns it off): a
+  // ConsToPrim recorded but not run (U still holds the conservative state),
+  // an MHD_Residual recorded after forming only what it returns (the EOS
+  // outputs and cmax; the residual is left to the stage kernel that
+  // Source_Integrate_Tcheck then runs), a SaveCons recorded (U0 = U, no copy)
+  bool defer = true;
+  bool c2p_def = false, res_def = false, u0_alias = false;
+  double def_inv_va2max = 0.0;                   // inv_va2max of the recorded MHD_Residual
   double* dR = nullptr;
   double* dP2 = nullptr;                         // second pressure buffer of the fused step
   bool fast_valid = false;                       // pres and cmax describe S[cur]
@@ -343,6 +352,8 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   c->force_sequence = path && strcmp(path, "sequence") == 0;
   const char* zw = getenv("MRM_STAGE_ZWIN");
   c->stage_variant = (zw && zw[0] >= '0' && zw[0] <= '3') ? zw[0] - '0' : 3;
+  const char* df = getenv("MRM_DEFER");
+  c->defer = !(df && df[0] == '0');
   const char* fm = getenv("MRM_FMA");
   c->fma = fm && fm[0] == '1';      // contracted build on request only (the exact one is the parity build)
   const char* tx = getenv("MRM_TVD_X");
@@ -410,8 +421,11 @@ extern "C" int mrm_synchronize(mrm_ctx* c) {
 // eos_init (eos_init.C:57-212): axes in double from the float header; the
 // forward tables transposed to [i_eps][i_rho] and divided by del_eps*del_lr in
 // double before rounding to float (:170-175); inverse tables likewise (:195-196).
+static int drop_settle(mrm_ctx* c);
+
 extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   if (!c || !bytes) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   const float* buf = (const float*)bytes;
   c->fast_valid = false;
   if (nbytes < 14 * sizeof(float)) return fail(c, MRM_ERR_EOS, "EOS table: short header");
@@ -483,11 +497,13 @@ static double* state_ptr(mrm_ctx* c, int which) {
 }
 
 static int res_settle(mrm_ctx* c);
+static int drop_settle(mrm_ctx* c);
 
 // In FP32-mixed contexts U is staged through dR (FP64) and converted on the
 // device: rho, M, B rounded to FP32, e kept in FP64.
 extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layout) {
   if (!c || !host) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_set_state: FP32-mixed contexts hold U only");
   double* d = mx ? c->dR : state_ptr(c, which);
@@ -513,6 +529,7 @@ extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layo
 
 extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
   if (!c || !host) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_get_state: FP32-mixed contexts hold U only");
   double* d = mx ? c->dR : state_ptr(c, which);
@@ -542,6 +559,7 @@ static int diag_ensure(mrm_ctx* c);
 
 extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
   if (!c || !host || f < 0 || f >= MRM_TVAR1 + DG_N) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   const size_t bs = c->G.bs;
   if (f >= MRM_TVAR1) {
     int rc = diag_ensure(c);
@@ -561,6 +579,7 @@ extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
 
 extern "C" int mrm_get_field(mrm_ctx* c, int f, double* host) {
   if (!c || !host || f < 0 || f >= MRM_TVAR1 + DG_N) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   const size_t bs = c->G.bs;
   if (f >= MRM_TVAR1) {
     // diagnostics never collected read as zero
@@ -610,17 +629,24 @@ extern "C" int mrm_set_bc(mrm_ctx* c, double p_bc, double s_bc) {
 }
 
 // ------------------------------------------------------------------ ConsToPrim
+static int c2p_launch(mrm_ctx* c);
+
 extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_cons_to_prim");
   c->fast_valid = false;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_cons_to_prim: no EOS table loaded");
-  unsigned nb = nblk(c->G.bs);
-  if (nb > 65536u * 4) nb = 65536u * 4;
-  TScope ts(c, T_C2P);
-  k_cons_to_prim<5><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF);
-  LAUNCHCHK(c, "k_cons_to_prim");
-  return MRM_OK;
+  // recorded: the next MHD_Residual and Source_Integrate_Tcheck read the
+  // conservative state themselves, anything else runs it first
+  if (c->c2p_def || c->res_def) {
+    int rc = drop_settle(c);
+    if (rc) return rc;
+  }
+  if (c->defer) {
+    c->c2p_def = true;
+    return MRM_OK;
+  }
+  return c2p_launch(c);
 }
 
 // ------------------------------------------------------------------ Adjust_Valf_Max
@@ -629,12 +655,21 @@ static void valf_apply(mrm_ctx* c, double stats[5]);
 // device reductions + the host-side update of Adjust_Valf_Max.C:97-144
 static int diag_track(mrm_ctx* c, const mrm_run_desc* run);
 
+static void valf_constants(mrm_ctx* c);
+
 extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_adjust_valf_max");
   c->fast_valid = false;
   int rc = diag_track(c, run);
   if (rc) return rc;
+  // va_adjust = 0 with no statistics asked for: the limits are constants
+  // (Adjust_Valf_Max.C:136-144), and a recorded ConsToPrim can stay recorded
+  if (c->c2p_def && !c->res_def && !(c->phys.params[2] > 0.0) && !stats) {
+    valf_constants(c);
+    return MRM_OK;
+  }
+  if ((rc = drop_settle(c))) return rc;
   // slots RED_CMAX..RED_CNT_E only: a fused step's Clean_divB maxima may
   // still wait in RED_DIVB0/1 for their host update
   HIPCHK(c, hipMemsetAsync(c->d_red, 0, (RED_CNT_E + 1) * sizeof(unsigned long long), c->stream));
@@ -762,6 +797,7 @@ static int diag_args(mrm_ctx* c, const mrm_run_desc* run, DiagArgs& DA, double d
 // the zeroing of solver.C:271-296 at the start of the stage that collects them
 extern "C" int mrm_diag_zero(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   NEED_FP64(c, "mrm_diag_zero");
   c->diag_last = 1;
   if (!c->d_diag) return diag_ensure(c);
@@ -780,13 +816,14 @@ static int res_settle(mrm_ctx* c) {
   return MRM_OK;
 }
 
-static int launch_residual(mrm_ctx* c, const mrm_run_desc* run) {
+// va: the inv_va2max to use (the context's when negative)
+static int launch_residual(mrm_ctx* c, const mrm_run_desc* run, double va = -1.0) {
   ResidParams RP;
   for (int d = 0; d < 3; d++) {
     RP.w1[d] = 8. / (12. * c->dx[d]);
     RP.w2[d] = -1. / (12. * c->dx[d]);
   }
-  RP.inv_va2max = c->inv_va2max;
+  RP.inv_va2max = va < 0.0 ? c->inv_va2max : va;
   eta_params(c, RP);
   DiagArgs DA;
   int rc = diag_args(c, run, DA);
@@ -876,15 +913,79 @@ static int read_cmax(mrm_ctx* c, double* cmax) {
   return MRM_OK;
 }
 
+static int c2p_launch(mrm_ctx* c) {
+  unsigned nb = nblk(c->G.bs);
+  if (nb > 65536u * 4) nb = 65536u * 4;
+  TScope ts(c, T_C2P);
+  k_cons_to_prim<5><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF);
+  LAUNCHCHK(c, "k_cons_to_prim");
+  return MRM_OK;
+}
+
+// the recorded drop-in operations carried out, in the order they were
+// called: SaveCons's copy, ConsToPrim, MHD_Residual's residual.  Every entry
+// point that reads or changes the state outside the deferred stage calls
+// this first.
+static int drop_settle(mrm_ctx* c) {
+  int rc;
+  if (c->u0_alias) {
+    TScope ts(c, T_SAVECONS);
+    k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
+    LAUNCHCHK(c, "k_save_cons");
+    c->u0_alias = false;
+  }
+  if (c->c2p_def) {
+    c->c2p_def = false;
+    if ((rc = c2p_launch(c))) return rc;
+  }
+  if (c->res_def) {
+    c->res_def = false;
+    mrm_run_desc r{};
+    r.need_diagnostics = 0;
+    if ((rc = launch_residual(c, &r, c->def_inv_va2max))) return rc;
+  }
+  return MRM_OK;
+}
+
+// whether MHD_Residual may be recorded: FP64, a recorded ConsToPrim, no
+// diagnostics and no Spitzer conduction (whose terms need ConsToPrim's
+// temperature and the stored residual)
+static bool res_deferrable(const mrm_ctx* c, const mrm_run_desc* run) {
+  return c->defer && c->c2p_def && !c->res_def && !(run && run->need_diagnostics) && !c->d_spt &&
+         c->precision == MRM_PREC_FP64;
+}
+
 extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_cfl) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_mhd_residual");
-  int rc = launch_residual(c, run);
-  if (rc) return rc;
+  int rc;
+  if (res_deferrable(c, run)) {
+    // the EOS outputs of the recorded ConsToPrim and cmax; the residual is
+    // formed by the stage kernel of the next Source_Integrate_Tcheck
+    if ((rc = diag_track(c, run))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->d_red + RED_CMAX, 0, sizeof(unsigned long long), c->stream));
+    unsigned nb = nblk(c->G.bs);
+    if (nb > 65536u) nb = 65536u;
+    {
+      TScope ts(c, T_C2P);
+      k_eos_fields<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF, c->inv_va2max, c->d_red + RED_CMAX);
+    }
+    LAUNCHCHK(c, "k_eos_fields");
+    c->res_def = true;
+    c->def_inv_va2max = c->inv_va2max;
+    if (!dt_cfl) return MRM_OK;
+    double cmax;
+    if ((rc = read_cmax(c, &cmax))) return rc;
+    *dt_cfl = eta_dt_limit(c, run, c->dxmin / cmax);   // mhdres_SR.C:795-819
+    return MRM_OK;
+  }
+  if ((rc = drop_settle(c))) return rc;
+  rc = launch_residual(c, run);
+  if (rc || !dt_cfl) return rc;
   double cmax;
   rc = read_cmax(c, &cmax);
   if (rc) return rc;
-  if (dt_cfl) *dt_cfl = eta_dt_limit(c, run, c->dxmin / cmax);   // mhdres_SR.C:795-819
+  *dt_cfl = eta_dt_limit(c, run, c->dxmin / cmax);   // mhdres_SR.C:795-819
   return MRM_OK;
 }
 
@@ -1064,10 +1165,48 @@ static int damping_update(mrm_ctx* c, const double* U, bool prim) {
   return MRM_OK;
 }
 
+static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const double* Uin, const double* Pin,
+                        const double* U0, double* Uout, double* Pout, const double* eres = nullptr,
+                        double rp_va = -1.0);
+
+// Source_Integrate_Tcheck after a recorded ConsToPrim and MHD_Residual: the
+// stage kernel forms the residual from the conservative state and the
+// recorded EOS pressure and applies sources, Boris correction, RK update and
+// Tcheck, into the spare state buffer, whose ghost cells then take the
+// input's in ConsToPrim's primitive form (what ConsToPrim and the in-place
+// update leave there); the buffers change roles, and Res := 0 is recorded
+static int srcint_deferred(mrm_ctx* c, const mrm_run_desc* run, int stage) {
+  int rc;
+  if (stage == 1 && (rc = damping_update(c, c->dU, false))) return rc;
+  const int spare = 3 - c->cur - c->c0;
+  double* Uin = c->dU;
+  double* Uout = c->S[spare];
+  const double* U0 = c->u0_alias ? c->dU : c->dU0;
+  if ((rc = stage_launch(c, run, stage, Uin, c->dF + MRM_PRES * c->G.bs, U0, Uout, nullptr, nullptr,
+                         c->def_inv_va2max)))
+    return rc;
+  const long ng = 4L * ((long)c->G.n1 * c->G.n2 + (long)c->G.n0 * c->G.n2 + (long)c->G.n0 * c->G.n1);
+  k_ghosts_prim<<<nblk(ng), MRM_BLOCK, 0, c->stream>>>(c->G, Uin, Uout);
+  LAUNCHCHK(c, "k_ghosts_prim");
+  if (c->u0_alias) {
+    c->c0 = c->cur;     // the input state is U0
+    c->u0_alias = false;
+  }
+  c->cur = spare;
+  c->dU = c->S[c->cur];
+  c->dU0 = c->S[c->c0];
+  c->c2p_def = false;
+  c->res_def = false;
+  c->res_zero = true;
+  return MRM_OK;
+}
+
 extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_source_integrate_tcheck");
   c->fast_valid = false;
+  if (c->res_def && !run->need_diagnostics) return srcint_deferred(c, run, stage);
+  if (int rc = drop_settle(c)) return rc;
   // the damping profile of the step, from the primitive state of stage 1
   if (stage == 1) {
     int rc = damping_update(c, c->dU, true);
@@ -1102,9 +1241,14 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_save_cons");
   c->fast_valid = false;
-  TScope ts(c, T_SAVECONS);
-  k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
-  LAUNCHCHK(c, "k_save_cons");
+  if (int rc = drop_settle(c)) return rc;
+  if (c->defer) {
+    c->u0_alias = true;   // copied only if U changes in place before a deferred stage
+  } else {
+    TScope ts(c, T_SAVECONS);
+    k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
+    LAUNCHCHK(c, "k_save_cons");
+  }
   if (c->d_spt) {
     k_spitzer_save<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dF + MRM_SFLX * c->G.bs, c->d_spt);
     LAUNCHCHK(c, "k_spitzer_save");
@@ -1114,6 +1258,7 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
 
 extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
   if (!c) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   NEED_FP64(c, "mrm_tcheck");
   c->fast_valid = false;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
@@ -1152,6 +1297,7 @@ static int exchange_arrays(mrm_ctx* c, double* base, int na);
 
 extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   c->fast_valid = false;
   if (c->precision == MRM_PREC_FP32_MIXED) return exchange_mx(c, c->M[c->cur], nullptr);
   int rc = exchange_buf(c, c->dU, nullptr);
@@ -1285,6 +1431,7 @@ static int clean_divb_mx(mrm_ctx* c, MxState M, const MxEpilogue* epi) {
 
 extern "C" int mrm_clean_divB(mrm_ctx* c, const mrm_run_desc* run, double err[2]) {
   if (!c) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   NEED_FP64(c, "mrm_clean_divB");
   if (c->phys.divB[1] < 1.0) return fail(c, MRM_ERR_ARG, "mrm_clean_divB: divB_itmax < 1");
   c->fast_valid = false;
@@ -1445,6 +1592,7 @@ extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, 
 extern "C" int mrm_set_boundary_conditions_pt(mrm_ctx* c, const mrm_run_desc* run, int stage, int pt_update) {
   (void)stage;
   if (!c) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
   if (int rc = diag_track(c, run)) return rc;
@@ -1711,6 +1859,7 @@ extern "C" int mrm_get_branch_counts(mrm_ctx* c, unsigned long long out[MRM_BR_N
 
 extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
   if (!c || !run) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   NEED_FP64(c, "mrm_tvdlimit");
   c->fast_valid = false;
   DiagArgs DA;
@@ -1744,7 +1893,7 @@ static int step_sequence(mrm_ctx* c, mrm_run_desc* run, double dt_rad, bool diag
       STEPCHK(mrm_mhd_residual(c, run, &dt_mhd));
       STEPCHK(mrm_sync_timestep(c, run, dt_rad, dt_mhd, &cfl));
     } else {
-      STEPCHK(launch_residual(c, run));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
+      STEPCHK(mrm_mhd_residual(c, run, nullptr));   // dt_cfl of stages 2-4 is not used (solver.C:302-309)
     }
     STEPCHK(mrm_source_integrate_tcheck(c, run, stage));
     STEPCHK(mrm_exchange_grid(c));
@@ -1810,7 +1959,7 @@ static int spitzer_ghosts(mrm_ctx* c, double* f) {
 }
 
 static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const double* Uin, const double* Pin,
-                        const double* U0, double* Uout, double* Pout, const double* eres = nullptr) {
+                        const double* U0, double* Uout, double* Pout, const double* eres, double rp_va) {
   StageArgs A;
   A.eres = eres;
   A.G = c->G;
@@ -1819,7 +1968,7 @@ static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const do
     A.RP.w1[d] = 8. / (12. * c->dx[d]);
     A.RP.w2[d] = -1. / (12. * c->dx[d]);
   }
-  A.RP.inv_va2max = c->inv_va2max;
+  A.RP.inv_va2max = rp_va < 0.0 ? c->inv_va2max : rp_va;
   eta_params(c, A.RP);
   A.SP = src_params(c, run, stage);
   A.ntx = (c->lsize[0] + ST_SX - 1) / ST_SX;
@@ -1945,8 +2094,9 @@ static int step_fused(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   if ((rc = exchange_buf(c, Uf, Pc))) return rc;
   if ((rc = bc_buf(c, Uf, Pc, cmax, pt_update_rule(c, run, 5)))) return rc;
   c->cur = i2;
+  c->c0 = (i2 + 1) % 3;
   c->dU = c->S[i2];
-  c->dU0 = c->S[(i2 + 1) % 3];
+  c->dU0 = c->S[c->c0];
   c->fast_valid = true;
   return MRM_OK;
 }
@@ -2162,6 +2312,7 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
 
 extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
   if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   const int want_diag = run->need_diagnostics;
@@ -2346,6 +2497,7 @@ static int adj_step_back(mrm_ctx* c, int n, AdjBufs& B) {
 
 extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps, int every) {
   if (!c || !run || nsteps < 1 || every < 1) return MRM_ERR_ARG;
+  if (int rc = drop_settle(c)) return rc;
   NEED_FP64(c, "mrm_adjoint_forward");
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
