@@ -178,7 +178,8 @@ def tape_gradient(prob):
     return U0.grad.numpy(), float(Jt)
 
 
-@pytest.mark.parametrize("gsize,seed,nsteps,every", [((16, 12, 12), 51, 2, 1), ((12, 10, 14), 55, 3, 2)])
+@pytest.mark.parametrize("gsize,seed,nsteps,every", [((16, 12, 12), 51, 2, 1), ((12, 10, 14), 55, 3, 2),
+                                                     ((40, 32, 24), 57, 4, 2)])
 def test_adjoint_matches_fp64_tape_gradient(gsize, seed, nsteps, every):
     print(f"case: adjoint vs FP64 tape gradient {gsize}, {nsteps} steps", flush=True)
     prob = Problem(gsize, seed=seed, nsteps=nsteps, every=every)
