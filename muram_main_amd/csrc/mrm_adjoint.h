@@ -46,10 +46,10 @@ MRM_HD double eos_p_vjp(const EosDev& E, const double u[MRM_NV], double lp, doub
   const double e = u[4] - 0.5 * vv * d;
   const double eps = e * dn + E.eps_off;
   const double lgr = log(d), ep = log(eps), lr = lgr;
-  int i_d = cvt_int((lr - E.xlr[0]) * E.inv_del_lr);
+  int i_d = cvt_int((lr - E.xlr0) * E.inv_del_lr);
   i_d = imax_(0, i_d);
   i_d = imin_(E.N_lr - 2, i_d);
-  int i_e = cvt_int((ep - E.xeps[0]) * E.inv_del_eps);
+  int i_e = cvt_int((ep - E.xeps0) * E.inv_del_eps);
   i_e = imax_(0, i_e);
   i_e = imin_(E.N_eps - 2, i_e);
   const double ae = ep - E.xeps[i_e], be = E.xeps[i_e + 1] - ep;
@@ -66,7 +66,7 @@ MRM_HD double eos_p_vjp(const EosDev& E, const double u[MRM_NV], double lp, doub
   tt += be * ar * (double)E.tab[NT + o01];
   tt += be * br * (double)E.tab[NT + o00];
   const double p0 = exp(v);
-  const bool lowr = lgr < E.xlr[0];
+  const bool lowr = lgr < E.xlr0;
   const bool hot = eps > 0.8 * E.eps_max;
   double pres = p0, temp;
   if (!lowr && !hot && tt > -700.0 && tt < 700.0) temp = 1.0;
@@ -826,10 +826,10 @@ MRM_HD void tvd_update_vjp(const TvdParams& P, const TvdPt& b, const double Fm[M
 MRM_HD double eos_fwd_raw_d(const EosDev& E, int t, double ee, double dd, double& dee, double& ddd) {
   const double ee1 = log(ee + E.eps_off);
   const double logr = log(dd);
-  int i = cvt_int((ee1 - E.xeps[0]) * E.inv_del_eps);
+  int i = cvt_int((ee1 - E.xeps0) * E.inv_del_eps);
   if (i < 0) i = 0;
   if (i > E.N_eps - 2) i = E.N_eps - 2;
-  int j = cvt_int((logr - E.xlr[0]) * E.inv_del_lr);
+  int j = cvt_int((logr - E.xlr0) * E.inv_del_lr);
   if (j < 0) j = 0;
   if (j > E.N_lr - 2) j = E.N_lr - 2;
   const float* T = E.tab + (size_t)t * E.N_eps * E.N_lr;
@@ -847,10 +847,10 @@ MRM_HD double eos_fwd_raw_d(const EosDev& E, int t, double ee, double dd, double
 MRM_HD double inv_raw_d(const EosDev& E, int t, double pp, double ss, double& dpp, double& dss) {
   const double ss1 = ss + E.ss_off;
   const double logp = log(pp);
-  int i = cvt_int((logp - E.xlp3[0]) * E.inv_del_lp3);
+  int i = cvt_int((logp - E.xlp30) * E.inv_del_lp3);
   if (i < 0) i = 0;
   if (i > E.N_lp3 - 2) i = E.N_lp3 - 2;
-  int j = cvt_int((ss1 - E.xs3[0]) * E.inv_del_s3);
+  int j = cvt_int((ss1 - E.xs30) * E.inv_del_s3);
   if (j < 0) j = 0;
   if (j > E.N_s3 - 2) j = E.N_s3 - 2;
   const float* T = E.itab + (size_t)t * E.N_lp3 * E.N_s3;

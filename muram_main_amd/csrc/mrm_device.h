@@ -51,10 +51,10 @@ MRM_HD int cvt_int(double x) {
 MRM_HD double eos_fwd_raw(const EosDev& E, int t, double ee, double dd) {
   double ee1 = log(ee + E.eps_off);
   double logr = log(dd);
-  int i = cvt_int((ee1 - E.xeps[0]) * E.inv_del_eps);
+  int i = cvt_int((ee1 - E.xeps0) * E.inv_del_eps);
   if (i < 0) i = 0;
   if (i > E.N_eps - 2) i = E.N_eps - 2;
-  int j = cvt_int((logr - E.xlr[0]) * E.inv_del_lr);
+  int j = cvt_int((logr - E.xlr0) * E.inv_del_lr);
   if (j < 0) j = 0;
   if (j > E.N_lr - 2) j = E.N_lr - 2;
   const float* T = E.tab + (size_t)t * E.N_eps * E.N_lr;
@@ -74,10 +74,10 @@ MRM_HD double s_interp(const EosDev& E, double ee, double dd) {
 MRM_HD double inv_raw(const EosDev& E, int t, double pp, double ss) {
   double ss1 = ss + E.ss_off;
   double logp = log(pp);
-  int i = cvt_int((logp - E.xlp3[0]) * E.inv_del_lp3);
+  int i = cvt_int((logp - E.xlp30) * E.inv_del_lp3);
   if (i < 0) i = 0;
   if (i > E.N_lp3 - 2) i = E.N_lp3 - 2;
-  int j = cvt_int((ss1 - E.xs3[0]) * E.inv_del_s3);
+  int j = cvt_int((ss1 - E.xs30) * E.inv_del_s3);
   if (j < 0) j = 0;
   if (j > E.N_s3 - 2) j = E.N_s3 - 2;
   const float* T = E.itab + (size_t)t * E.N_lp3 * E.N_s3;
@@ -127,10 +127,10 @@ MRM_HD void eos_issue(const EosDev& E, double d, double& m1, double& m2, double&
   double lgr = log(d);
   double ep = log(eps);
   double lr = lgr;
-  int i_d = cvt_int((lr - E.xlr[0]) * E.inv_del_lr);
+  int i_d = cvt_int((lr - E.xlr0) * E.inv_del_lr);
   i_d = imax_(0, i_d);
   i_d = imin_(E.N_lr - 2, i_d);
-  int i_e = cvt_int((ep - E.xeps[0]) * E.inv_del_eps);
+  int i_e = cvt_int((ep - E.xeps0) * E.inv_del_eps);
   i_e = imax_(0, i_e);
   i_e = imin_(E.N_eps - 2, i_e);
   g.d = d;
@@ -190,7 +190,7 @@ MRM_HD void eos_finish(const EosDev& E, const EosLook<NOUT>& g, double out[5]) {
     // The temperature only decides whether p is valid (cons_to_prim.C:195).
     // exp(tt) is finite and positive for |tt| < 700, so it is evaluated only
     // when a fallback branch uses it or tt lies outside that range.
-    const bool plain = !(lgr < E.xlr[0]) && !(eps > 0.8 * E.eps_max);
+    const bool plain = !(lgr < E.xlr0) && !(eps > 0.8 * E.eps_max);
     if (!WANT_T && plain && tt > -700.0 && tt < 700.0) {
       temp = 1.0;
     } else {
@@ -198,7 +198,7 @@ MRM_HD void eos_finish(const EosDev& E, const EosLook<NOUT>& g, double out[5]) {
     }
   }
   double nel = NOUT > 2 ? r[2] : 0.0;
-  if (lgr < E.xlr[0]) {
+  if (lgr < E.xlr0) {
     pres = c_pres * e;
     temp = c_temp * eps;
     nel = pres / (temp * kb) - d / (eos_mu_n * mh);

@@ -119,6 +119,39 @@ __device__ __forceinline__ float mx_lfac(const float c[ST_NF], float inv_va2max)
 // res[1..3] and its work to res[4]: the caller combines it with the Boris
 // correction without cancellation.  ETA: the sweep's share of J = curl B and
 // eta times the second derivative of B (mhdres_SR.C:422-462), as eta_sweep.
+// the residual of sweep D from the flux differences d1 = F(+1) - F(-1),
+// d2 = F(+2) - F(-2) and (ETA) the sums bs1 = B(+1) + B(-1), bs2 = B(+2) + B(-2)
+template <int D, bool ETA>
+__device__ __forceinline__ void mx_sweep_comb(const float d1[13], const float d2[13], const float bs1[3],
+                                              const float bs2[3], const float c[ST_NF], float w1, float w2,
+                                              float lf, float res[MRM_NV], float lor[3], const MxStageArgs* A,
+                                              float* J) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) res[q] = 0.0f - (w1 * d1[q] + w2 * d2[q]);
+  float dB2 = w1 * d1[11] + w2 * d2[11];
+  float dB3 = w1 * d1[12] + w2 * d2[12];
+  float cb[3];
+  cb[D] = -c[5 + D2] * dB2 - c[5 + D3] * dB3;
+  cb[D2] = c[5 + D] * dB2;
+  cb[D3] = c[5 + D] * dB3;
+  float lrt0 = w1 * d1[8] + w2 * d2[8];
+  float lrt1 = w1 * d1[9] + w2 * d2[9];
+  float lrt2 = w1 * d1[10] + w2 * d2[10];
+  lor[0] = lf * lrt0 + (1.0f - lf) * cb[0];
+  lor[1] = lf * lrt1 + (1.0f - lf) * cb[1];
+  lor[2] = lf * lrt2 + (1.0f - lf) * cb[2];
+  if constexpr (ETA) {
+    constexpr float sgn = D == 1 ? 1.0f : -1.0f;
+    J[D2] += sgn * dB3;
+    J[D3] += -sgn * dB2;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      res[5 + q] += A->eta * (A->ww0[D] * c[5 + q] + A->ww1[D] * bs1[q] + A->ww2[D] * bs2[q]);
+  }
+}
+
 template <int D, bool ETA = false, class GET>
 __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1, float w2, float lf,
                                          float res[MRM_NV], float lor[3], const MxStageArgs* A = nullptr,
@@ -153,28 +186,45 @@ __device__ __forceinline__ void mx_sweep(GET get, const float c[ST_NF], float w1
       for (int q = 0; q < 3; q++) bs2[q] = pb[5 + q] + pa[5 + q];
     }
   }
+  mx_sweep_comb<D, ETA>(d1, d2, bs1, bs2, c, w1, w2, lf, res, lor, A, J);
+}
+
+// the point flux of sweep D packed for sharing through LDS: mx_point_flux's
+// 13 values and, with ETA, B along D (the three B components are then
+// f[13], f[11], f[12] for B[D], B[D2], B[D3])
+template <bool ETA>
+constexpr int mx_nfp() { return ETA ? 14 : 13; }
+template <int D, bool ETA>
+__device__ __forceinline__ void mx_flux_pack(const float var[ST_NF], float f[14]) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  mx_point_flux<D>(var, f, D2, D3);
+  if (ETA) f[13] = var[5 + D];
+}
+
+// sweep D from packed neighbour fluxes: get(s, f) returns value f of the
+// packed flux at offset s - 2 along D
+template <int D, bool ETA, class GET>
+__device__ __forceinline__ void mx_sweep_packed(GET get, const float c[ST_NF], float w1, float w2, float lf,
+                                                float res[MRM_NV], float lor[3], const MxStageArgs* A,
+                                                float* J) {
+  constexpr int D2 = D == 0 ? 1 : 0;
+  constexpr int D3 = D == 2 ? 1 : 2;
+  float d1[13], d2[13], bs1[3], bs2[3];
 #pragma unroll
-  for (int q = 0; q < MRM_NV; q++) res[q] = 0.0f - (w1 * d1[q] + w2 * d2[q]);
-  float dB2 = w1 * d1[11] + w2 * d2[11];
-  float dB3 = w1 * d1[12] + w2 * d2[12];
-  float cb[3];
-  cb[D] = -c[5 + D2] * dB2 - c[5 + D3] * dB3;
-  cb[D2] = c[5 + D] * dB2;
-  cb[D3] = c[5 + D] * dB3;
-  float lrt0 = w1 * d1[8] + w2 * d2[8];
-  float lrt1 = w1 * d1[9] + w2 * d2[9];
-  float lrt2 = w1 * d1[10] + w2 * d2[10];
-  lor[0] = lf * lrt0 + (1.0f - lf) * cb[0];
-  lor[1] = lf * lrt1 + (1.0f - lf) * cb[1];
-  lor[2] = lf * lrt2 + (1.0f - lf) * cb[2];
-  if constexpr (ETA) {
-    constexpr float sgn = D == 1 ? 1.0f : -1.0f;
-    J[D2] += sgn * dB3;
-    J[D3] += -sgn * dB2;
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-      res[5 + q] += A->eta * (A->ww0[D] * c[5 + q] + A->ww1[D] * bs1[q] + A->ww2[D] * bs2[q]);
+  for (int q = 0; q < 13; q++) {
+    d1[q] = get(3, q) - get(1, q);
+    d2[q] = get(4, q) - get(0, q);
   }
+  if constexpr (ETA) {
+    bs1[D] = get(3, 13) + get(1, 13);
+    bs1[D2] = get(3, 11) + get(1, 11);
+    bs1[D3] = get(3, 12) + get(1, 12);
+    bs2[D] = get(4, 13) + get(0, 13);
+    bs2[D2] = get(4, 11) + get(0, 11);
+    bs2[D3] = get(4, 12) + get(0, 12);
+  }
+  mx_sweep_comb<D, ETA>(d1, d2, bs1, bs2, c, w1, w2, lf, res, lor, A, J);
 }
 
 // pressure from the mixed conservative values (ConsToPrim's p and its
@@ -205,10 +255,10 @@ __device__ __forceinline__ void mx_eos_issue(const EosDev& E, float d, float m0,
   // one off at a node boundary interpolates to the same value
   const float lr = __builtin_amdgcn_logf(d) * 0.693147180559945f;
   const float ep = __builtin_amdgcn_logf((float)g.eps) * 0.693147180559945f;
-  int i_d = cvt_int((double)(lr - (float)E.xlr[0]) * E.inv_del_lr);
+  int i_d = cvt_int((double)(lr - (float)E.xlr0) * E.inv_del_lr);
   i_d = imax_(0, i_d);
   i_d = imin_(E.N_lr - 2, i_d);
-  int i_e = cvt_int((double)(ep - (float)E.xeps[0]) * E.inv_del_eps);
+  int i_e = cvt_int((double)(ep - (float)E.xeps0) * E.inv_del_eps);
   i_e = imax_(0, i_e);
   i_e = imin_(E.N_eps - 2, i_e);
   g.i_d = i_d;
@@ -308,6 +358,72 @@ __device__ __forceinline__ void mx_tcheck(float u[MRM_NV], double& e, float rho_
   u[1] *= s;
   u[2] *= s;
   u[3] *= s;
+}
+
+// the rest of a cell's stage after the three sweeps' sums acc (without the
+// Lorentz force) and accL (the Lorentz force): eta |J|^2 and the conduction
+// term, sources, Boris correction, the RK update (FP64 energy) and Tcheck;
+// stores the new state of the cell and returns it in o (rho, M, B) and oe (e)
+template <bool ETA, bool ERES>
+__device__ __forceinline__ void mx_cell_finish(const MxStageArgs& A, const float w2[ST_NF], float acc[MRM_NV],
+                                               const float accL[3], const float J[3], unsigned node, MxState S0,
+                                               MxState Sout, const double* __restrict__ Qtot, float o[MRM_NV],
+                                               double& oe) {
+  const long bs = A.G.bs;
+  if (ETA) acc[4] += A.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
+  if (ERES) acc[4] += (float)ld32(A.eres, node);
+
+  // sources, Boris correction (Add_Sources_Integrate_SR.C:331-392), FP32
+  const float* up = w2;
+  const float Qrad = Qtot ? (float)ld32(Qtot, node) : 0.0f;
+  float dn = A.invRdt * up[4];
+  float efaci = fdiv(dn, fmaxf(dn, fabsf(Qrad)));
+  dn = A.g * up[0];
+  float e_dmpi = dn * up[1];
+  acc[1] += dn;
+  acc[4] += e_dmpi + Qrad * efaci;
+  {
+    // Lorentz force f_L (accL) and its work, Boris correction
+    // (Add_Sources_Integrate_SR.C:355-392).  With f = f_h + f_L - v R_rho
+    // the reference adds f_L, v.f_L, then -bf f_perp and -bf v.f_perp.
+    // Where the Alfven limiter is strong bf -> 1 and those terms cancel
+    // most digits; with lb = 1 - bf = (1+x2)/(1+x2+x4) the same sums are
+    //   R_M = v R_rho + f_par + lb f_perp
+    //   R_e += prL (v.b) + lb v.f_L,perp - bf v.(f_perp - f_L,perp)
+    // (prL b = the part of f_L along b), which FP32 keeps accurate.
+    float rh = up[0], rd = acc[0];
+    float vx = up[1], vy = up[2], vz = up[3];
+    float bx = up[5], by = up[6], bz = up[7];
+    float fx = (acc[1] + accL[0]) - vx * rd, fy = (acc[2] + accL[1]) - vy * rd, fz = (acc[3] + accL[2]) - vz * rd;
+    float bb = bx * bx + by * by + bz * bz;
+    float x2 = bb * frcp(rh) * A.inv_va2max;
+    float x4 = x2 * x2;
+    float sx = 1 + x2 + x4;
+    const float isx = frcp(sx);
+    float bf = x4 * isx, lb = (1 + x2) * isx;
+    float ib = frcp(fmaxf(1e-30f, bb));
+    float pr = (fx * bx + fy * by + fz * bz) * ib;
+    float px = fx - pr * bx, py = fy - pr * by, pz = fz - pr * bz;
+    float prL = (accL[0] * bx + accL[1] * by + accL[2] * bz) * ib;
+    float lx = accL[0] - prL * bx, ly = accL[1] - prL * by, lz = accL[2] - prL * bz;
+    acc[1] = (vx * rd + pr * bx) + lb * px;
+    acc[2] = (vy * rd + pr * by) + lb * py;
+    acc[3] = (vz * rd + pr * bz) + lb * pz;
+    acc[4] += prL * (vx * bx + vy * by + vz * bz) + lb * (vx * lx + vy * ly + vz * lz) -
+              bf * (vx * (px - lx) + vy * (py - ly) + vz * (pz - lz));
+  }
+  // RK update: FP32 for rho, M, B; FP64 for e (Add_Sources_Integrate_SR.C:399)
+  const float wdtf = (float)A.wdt;
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) o[q] = q == 4 ? 0.0f : ld32(S0.f + mx_slot(q) * bs, node) + wdtf * acc[q];
+  oe = ld32(S0.e, node) + A.wdt * (double)acc[4];
+  mx_tcheck(o, oe, A.rho_min, A.eps_min, A.eps_max, A.vmax);
+#pragma unroll
+  for (int q = 0; q < MRM_NV; q++) {
+    if (q == 4) continue;
+    st32(Sout.f + mx_slot(q) * bs, node, o[q]);
+  }
+  st32(Sout.e, node, oe);
 }
 
 // ------------------------------------------------------------------ stage kernel
@@ -434,62 +550,166 @@ k_stage_mx(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0
       for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
 #pragma unroll
       for (int q = 0; q < 3; q++) accL[q] += lor[q];
-      if (ETA) acc[4] += A.eta * (J[0] * J[0] + J[1] * J[1] + J[2] * J[2]);
-      if (ERES) acc[4] += (float)ld32(A.eres, col + (unsigned)k * s2u);
-
-      // sources, Boris correction (Add_Sources_Integrate_SR.C:331-392), FP32
       const unsigned node = col + (unsigned)k * s2u;
-      const float* up = w[2];
-      const float Qrad = Qtot ? (float)ld32(Qtot, node) : 0.0f;
-      float dn = A.invRdt * up[4];
-      float efaci = fdiv(dn, fmaxf(dn, fabsf(Qrad)));
-      dn = A.g * up[0];
-      float e_dmpi = dn * up[1];
-      acc[1] += dn;
-      acc[4] += e_dmpi + Qrad * efaci;
-      {
-        // Lorentz force f_L (accL) and its work, Boris correction
-        // (Add_Sources_Integrate_SR.C:355-392).  With f = f_h + f_L - v R_rho
-        // the reference adds f_L, v.f_L, then -bf f_perp and -bf v.f_perp.
-        // Where the Alfven limiter is strong bf -> 1 and those terms cancel
-        // most digits; with lb = 1 - bf = (1+x2)/(1+x2+x4) the same sums are
-        //   R_M = v R_rho + f_par + lb f_perp
-        //   R_e += prL (v.b) + lb v.f_L,perp - bf v.(f_perp - f_L,perp)
-        // (prL b = the part of f_L along b), which FP32 keeps accurate.
-        float rh = up[0], rd = acc[0];
-        float vx = up[1], vy = up[2], vz = up[3];
-        float bx = up[5], by = up[6], bz = up[7];
-        float fx = (acc[1] + accL[0]) - vx * rd, fy = (acc[2] + accL[1]) - vy * rd, fz = (acc[3] + accL[2]) - vz * rd;
-        float bb = bx * bx + by * by + bz * bz;
-        float x2 = bb * frcp(rh) * A.inv_va2max;
-        float x4 = x2 * x2;
-        float sx = 1 + x2 + x4;
-        const float isx = frcp(sx);
-        float bf = x4 * isx, lb = (1 + x2) * isx;
-        float ib = frcp(fmaxf(1e-30f, bb));
-        float pr = (fx * bx + fy * by + fz * bz) * ib;
-        float px = fx - pr * bx, py = fy - pr * by, pz = fz - pr * bz;
-        float prL = (accL[0] * bx + accL[1] * by + accL[2] * bz) * ib;
-        float lx = accL[0] - prL * bx, ly = accL[1] - prL * by, lz = accL[2] - prL * bz;
-        acc[1] = (vx * rd + pr * bx) + lb * px;
-        acc[2] = (vy * rd + pr * by) + lb * py;
-        acc[3] = (vz * rd + pr * bz) + lb * pz;
-        acc[4] += prL * (vx * bx + vy * by + vz * bz) + lb * (vx * lx + vy * ly + vz * lz) -
-                  bf * (vx * (px - lx) + vy * (py - ly) + vz * (pz - lz));
-      }
-      // RK update: FP32 for rho, M, B; FP64 for e (Add_Sources_Integrate_SR.C:399)
-      const float wdtf = (float)A.wdt;
       float o[MRM_NV];
-#pragma unroll
-      for (int q = 0; q < MRM_NV; q++) o[q] = q == 4 ? 0.0f : ld32(S0.f + mx_slot(q) * bs, node) + wdtf * acc[q];
-      double oe = ld32(S0.e, node) + A.wdt * (double)acc[4];
-      mx_tcheck(o, oe, A.rho_min, A.eps_min, A.eps_max, A.vmax);
-#pragma unroll
-      for (int q = 0; q < MRM_NV; q++) {
-        if (q == 4) continue;
-        st32(Sout.f + mx_slot(q) * bs, node, o[q]);
+      double oe;
+      mx_cell_finish<ETA, ERES>(A, w[2], acc, accL, J, node, S0, Sout, Qtot, o, oe);
+      if (WP) {
+        const float pk = mx_eos_finish(A.E, pg);
+        if (k > kbeg) st32(Pout, pd_node, pk);
+        pd_d = o[0]; pd_m0 = o[1]; pd_m1 = o[2]; pd_m2 = o[3];
+        pd_e = oe;
+        pd_node = node;
       }
-      st32(Sout.e, node, oe);
+    }
+    __syncthreads();
+  }
+  if (WP && valid && kend > kbeg) st32(Pout, pd_node, mx_pres(A.E, pd_d, pd_m0, pd_m1, pd_m2, pd_e));
+}
+
+// ------------------------------------------------------------------ stage kernel, fluxes shared
+// k_stage_mx with each x and y point flux formed once per plane: every thread
+// packs the fluxes of its own point along dims 0 and 1 into LDS, the ring
+// points' fluxes are packed by one thread each (threads 0 .. 4*SX-1 the four
+// dim-1 ring rows, the next 4*SY threads the dim-0 ring columns), and the x
+// and y sweeps take the four neighbours' fluxes from LDS (4.5 point-flux
+// evaluations per plane and wave instead of 12).  Dim-2 neighbours come from
+// the five-plane register window.  Each term is formed with k_stage_mx's
+// expressions.
+template <bool ETA = false, bool ERES = false, bool WP = true>
+__global__ void __launch_bounds__(MX_SX* MX_SY, 2)
+k_stage_mxf(MxStageArgs A, MxState Sin, const float* __restrict__ Pin, MxState S0, MxState Sout,
+            float* __restrict__ Pout, const double* __restrict__ Qtot) {
+  constexpr int NFP = mx_nfp<ETA>();
+  static_assert(4 * MX_SX + 4 * MX_SY <= MX_SX * MX_SY, "one ring point per thread at most");
+  __shared__ float fx[NFP][MX_SY][MX_LX];   // dim-0 fluxes of row ty, columns i0-2 .. i0+SX+1
+  __shared__ float fy[NFP][MX_LY][MX_SX];   // dim-1 fluxes of column tx, rows j0-2 .. j0+SY+1
+  const Geom& G = A.G;
+  const long bs = G.bs;
+  const long nblocks = (long)A.ntx * A.nty * A.nkc;
+  long lb = xcd_remap(blockIdx.x, nblocks);
+  const int tx_t = (int)(lb % A.ntx);
+  lb /= A.ntx;
+  const int ty_t = (int)(lb % A.nty);
+  const int kc = (int)(lb / A.nty);
+  const int t = threadIdx.x;
+  const int tx = t % MX_SX, ty = t / MX_SX;
+  const int i0 = G.lb0 + tx_t * MX_SX, j0 = G.lb1 + ty_t * MX_SY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = (i <= G.le0) && (j <= G.le1);
+  const int kbeg = G.lb2 + kc * MX_KC;
+  const int kend = min(kbeg + MX_KC, G.le2 + 1);
+  int ring = 0;                       // 1: dim-1 ring point, 2: dim-0 ring point
+  int r_row = 0, r_col = 0, r_i = 0, r_j = 0;
+  if (t < 4 * MX_SX) {
+    const int rr_ = t / MX_SX;
+    r_row = rr_ < 2 ? rr_ : MX_SY + rr_;
+    r_col = t % MX_SX;
+    r_i = i0 + r_col;
+    r_j = j0 - 2 + r_row;
+    ring = 1;
+  } else if (t < 4 * MX_SX + 4 * MX_SY) {
+    const int u = t - 4 * MX_SX;
+    r_row = u / 4;
+    const int c4 = u % 4;
+    r_col = c4 < 2 ? c4 : MX_SX + c4;
+    r_i = i0 - 2 + r_col;
+    r_j = j0 + r_row;
+    ring = 2;
+  }
+  const bool ring_ok = ring != 0 && r_i < G.n0 && r_j < G.n1;
+  const bool own_ok = i < G.n0 && j < G.n1;
+  const unsigned col = (unsigned)i + (unsigned)j * (unsigned)G.s1;
+  const unsigned rcol = (unsigned)r_i + (unsigned)r_j * (unsigned)G.s1;
+  const unsigned s2u = (unsigned)G.s2;
+
+  float w[5][ST_NF];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    int kk = kbeg - 2 + q;
+    if (own_ok) mx_load_prim(Sin, Pin, bs, col + (unsigned)kk * s2u, w[q + 1]);
+    else {
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0f;
+    }
+  }
+  float pd_d = 1.0f, pd_m0 = 0.0f, pd_m1 = 0.0f, pd_m2 = 0.0f;
+  double pd_e = 1.0;
+  unsigned pd_node = 0;
+  MxRaw rz, rr;   // own column's plane k + 3, the ring point's plane k + 1
+  if (own_ok) mx_load_raw(Sin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
+  if (ring_ok) mx_load_raw(Sin, Pin, bs, rcol + (unsigned)kbeg * s2u, rr);
+  for (int k = kbeg; k < kend; k++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
+    if (own_ok) mx_prim_of_raw(rz, w[4]);
+    {
+      float o[14];
+      mx_flux_pack<0, ETA>(w[2], o);
+#pragma unroll
+      for (int f = 0; f < NFP; f++) fx[f][ty][tx + 2] = o[f];
+      mx_flux_pack<1, ETA>(w[2], o);
+#pragma unroll
+      for (int f = 0; f < NFP; f++) fy[f][ty + 2][tx] = o[f];
+    }
+    if (ring != 0) {
+      float h[ST_NF], o[14];
+      if (ring_ok) mx_prim_of_raw(rr, h);
+      else {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) h[f] = 1.0f;
+      }
+      if (ring == 1) {
+        mx_flux_pack<1, ETA>(h, o);
+#pragma unroll
+        for (int f = 0; f < NFP; f++) fy[f][r_row][r_col] = o[f];
+      } else {
+        mx_flux_pack<0, ETA>(h, o);
+#pragma unroll
+        for (int f = 0; f < NFP; f++) fx[f][r_row][r_col] = o[f];
+      }
+    }
+    if (k + 1 < kend) {
+      // k + 3 <= kend + 1 <= lend2 + 2 = n2 - 1
+      if (own_ok) mx_load_raw(Sin, Pin, bs, col + (unsigned)(k + 3) * s2u, rz);
+      if (ring_ok) mx_load_raw(Sin, Pin, bs, rcol + (unsigned)(k + 1) * s2u, rr);
+    }
+    __syncthreads();
+    if (valid) {
+      MxEosG pg;
+      if (WP) mx_eos_issue(A.E, pd_d, pd_m0, pd_m1, pd_m2, pd_e, pg);
+      float acc[MRM_NV], res[MRM_NV], accL[3] = {0.0f, 0.0f, 0.0f}, lor[3];
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] = 0.0f;
+      const float lfc = mx_lfac(w[2], A.inv_va2max);
+      float J[3] = {0.0f, 0.0f, 0.0f};
+      auto getx = [&](int s_, int f) { return fx[f][ty][tx + s_]; };
+      mx_sweep_packed<0, ETA>(getx, w[2], A.w1[0], A.w2[0], lfc, res, lor, &A, J);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+#pragma unroll
+      for (int q = 0; q < 3; q++) accL[q] += lor[q];
+      auto gety = [&](int s_, int f) { return fy[f][ty + s_][tx]; };
+      mx_sweep_packed<1, ETA>(gety, w[2], A.w1[1], A.w2[1], lfc, res, lor, &A, J);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+#pragma unroll
+      for (int q = 0; q < 3; q++) accL[q] += lor[q];
+      auto getz = [&](int s_, float o[ST_NF]) {
+#pragma unroll
+        for (int f = 0; f < ST_NF; f++) o[f] = w[s_][f];
+      };
+      mx_sweep<2, ETA>(getz, w[2], A.w1[2], A.w2[2], lfc, res, lor, &A, J);
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) acc[q] += res[q];
+#pragma unroll
+      for (int q = 0; q < 3; q++) accL[q] += lor[q];
+      const unsigned node = col + (unsigned)k * s2u;
+      float o[MRM_NV];
+      double oe;
+      mx_cell_finish<ETA, ERES>(A, w[2], acc, accL, J, node, S0, Sout, Qtot, o, oe);
       if (WP) {
         const float pk = mx_eos_finish(A.E, pg);
         if (k > kbeg) st32(Pout, pd_node, pk);

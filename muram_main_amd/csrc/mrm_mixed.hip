@@ -8,10 +8,35 @@ namespace mixed {
 
 static inline unsigned nblk_(long n) { return (unsigned)((n + MRM_BLOCK - 1) / MRM_BLOCK); }
 
+// stage kernel of the mixed step: 1 fluxes shared through LDS (k_stage_mxf),
+// 0 every sweep's fluxes formed in place (k_stage_mx); MRM_MX_STAGE
+static int stage_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* s = getenv("MRM_MX_STAGE");
+    v = (s && s[0] == '0') ? 0 : 1;
+  }
+  return v;
+}
+
 hipError_t launch_stage(hipStream_t st, unsigned nb, const MxStageArgs& A, MxState in, const float* Pin, MxState s0,
                         MxState out, float* Pout, const double* Q) {
   const bool eta = A.eta > 0.0f, eres = A.eres != nullptr, wp = Pout != nullptr;
   const unsigned nt = MX_SX * MX_SY;
+  if (stage_variant() == 1) {
+    if (wp) {
+      if (eta && eres) k_stage_mxf<true, true, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+      else if (eta) k_stage_mxf<true, false, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+      else if (eres) k_stage_mxf<false, true, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+      else k_stage_mxf<false, false, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    } else {
+      if (eta && eres) k_stage_mxf<true, true, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+      else if (eta) k_stage_mxf<true, false, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+      else if (eres) k_stage_mxf<false, true, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+      else k_stage_mxf<false, false, false><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
+    }
+    return hipGetLastError();
+  }
   if (wp) {
     if (eta && eres) k_stage_mx<true, true, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);
     else if (eta) k_stage_mx<true, false, true><<<nb, nt, 0, st>>>(A, in, Pin, s0, out, Pout, Q);

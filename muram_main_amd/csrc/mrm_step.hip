@@ -481,6 +481,7 @@ extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   E.eps_off = buf[12];
   E.ss_off = buf[13];
   E.eps_max = exp(xeps[N_eps - 1]) - E.eps_off;   // cons_to_prim.C:52, host libm
+  E.xeps0 = xeps[0]; E.xlr0 = xlr[0]; E.xlp30 = xlp3[0]; E.xs30 = xs3[0];
   double* dax = (double*)base;
   E.xeps = dax; E.xlr = dax + N_eps; E.xlp3 = dax + N_eps + N_lr; E.xs3 = dax + N_eps + N_lr + N_lp3;
   E.rxeps = E.xs3 + N_s3;

@@ -50,6 +50,9 @@ struct EosDev {
   int N_eps, N_lr, N_lp3, N_s3;
   double inv_del_eps, inv_del_lr, inv_del_lp3, inv_del_s3;
   double eps_off, ss_off, eps_max;
+  // the axes' first nodes xeps[0], xlr[0], xlp3[0], xs3[0]: kernel arguments,
+  // so that a lookup's index needs no memory read before its table reads
+  double xeps0, xlr0, xlp30, xs30;
   const double* xeps;
   const double* xlr;
   const double* xlp3;

@@ -62,6 +62,7 @@ static bool load_eos(const char* path, HostEos& H) {
   E.eps_off = buf[12]; E.ss_off = buf[13];
   E.xeps = H.ax.data(); E.xlr = E.xeps + Ne; E.xlp3 = E.xlr + Nr; E.xs3 = E.xlp3 + Np;
   E.eps_max = std::exp(E.xeps[Ne - 1]) - E.eps_off;
+  E.xeps0 = E.xeps[0]; E.xlr0 = E.xlr[0]; E.xlp30 = E.xlp3[0]; E.xs30 = E.xs3[0];
   E.rxeps = nullptr; E.rxlr = nullptr;
   E.tab = H.tab.data(); E.itab = H.itab.data();
   return true;
