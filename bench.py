@@ -358,18 +358,25 @@ def local_direction(U, inner, rng):
     return d
 
 
-def adjoint_study(table, tab, steps=50, every=10, ndir=2, hs=(1e-9,)):
+def adjoint_study(table, tab, steps=50, every=10, ndir=2, hs=(1e-9,), precision=None, ref_grad=None):
     """Adjoint of J = sum v_vertical^2 over the interior layer 64 of the 128^3
-    box after `steps` FP64 steps, checkpointed every `every` steps: forward,
+    box after `steps` steps, checkpointed every `every` steps: forward,
     recorded-forward and backward wall times, HBM in use, and central
     differences along random interior directions (dt held fixed), one list of
-    relative differences per finite-difference step h in hs."""
+    relative differences per finite-difference step h in hs.  With several
+    steps, each direction's estimate is also taken at the step where it
+    agrees best with the next smaller one (the plateau between the limiter's
+    kinks at larger steps and rounding at smaller ones).  precision: the
+    context's (FP64 by default; in an FP32-mixed context the forward is the
+    mixed step and the gradient is compared with ref_grad, the FP64
+    context's).  Returns (summary, gradient)."""
     import torch
     from muram_main_amd import abi, config, initial
+    prec = abi.PREC_FP64 if precision is None else precision
     wl = config.workloads()["adjoint128"]
     U = initial.make_state(tab, wl.box)
     free0, total = torch.cuda.mem_get_info()
-    m, r = make_grid(wl, table, U, abi.PREC_FP64)
+    m, r = make_grid(wl, table, U, prec)
     p_bc, s_bc = m.bc()
     m.step(r, config.DT_RAD_OFF, 1)
     dt = 0.5 * r.dt
@@ -411,23 +418,54 @@ def adjoint_study(table, tab, steps=50, every=10, ndir=2, hs=(1e-9,)):
     # the forward is smooth only between limiter switches, which lie within
     # 1e-10 of many points (relative near ties): differences at 1e-6..1e-8
     # straddle switches and disagree among themselves, so the steps are
-    # 1e-9 and below (relative to each cell's magnitudes)
+    # 1e-8 and below (relative to each cell's magnitudes)
     errs = {h: [] for h in hs}
+    plateau = []
     for _ in range(ndir):
         d = local_direction(U, inner, rng)
         ad = float(np.sum(grad[(slice(None),) + inner] * d[(slice(None),) + inner]))
+        fds = []
         for h in hs:
             fd = (J(run(U + h * d)[0]) - J(run(U - h * d)[0])) / (2 * h)
+            fds.append(fd)
             errs[h].append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+        if len(hs) > 1:
+            k = min(range(len(hs) - 1), key=lambda i: abs(fds[i] - fds[i + 1]))
+            plateau.append(abs(fds[k] - ad) / max(abs(fds[k]), abs(ad), 1e-300))
     m.close()
-    return {"workload": wl.name, "steps": steps, "checkpoint_every": every, "dt_s": dt,
-            "forward_s": round(t_plain, 4), "forward_recorded_s": round(t_rec, 4), "backward_s": round(t_back, 4),
-            "backward_over_forward": round(t_back / t_plain, 2),
-            "hbm_in_use_bytes": int(free0 - free1),
-            "fd_step": list(hs) if len(hs) > 1 else hs[0],
-            "fd_rel_diff": ({f"{h:g}": [float(f"{e:.3e}") for e in errs[h]] for h in hs} if len(hs) > 1
-                            else [float(f"{e:.3e}") for e in errs[hs[0]]]),
-            "note": "FP64 adjoint; dt, TVD branch, p_bc/s_bc held at the forward's values"}
+    out = {"workload": wl.name, "steps": steps, "checkpoint_every": every, "dt_s": dt,
+           "forward_s": round(t_plain, 4), "forward_recorded_s": round(t_rec, 4), "backward_s": round(t_back, 4),
+           "backward_over_forward": round(t_back / t_plain, 2),
+           "hbm_in_use_bytes": int(free0 - free1)}
+    if ndir > 0:
+        out["fd_step"] = list(hs) if len(hs) > 1 else hs[0]
+        out["fd_rel_diff"] = ({f"{h:g}": [float(f"{e:.3e}") for e in errs[h]] for h in hs} if len(hs) > 1
+                              else [float(f"{e:.3e}") for e in errs[hs[0]]])
+        if plateau:
+            out["fd_rel_diff_plateau"] = [float(f"{e:.3e}") for e in plateau]
+            out["fd_plateau_max"] = float(f"{max(plateau):.3e}")
+    if ref_grad is not None:
+        # per field: max|a - b| / max|b| and ||a - b|| / ||b|| over the
+        # interior; along random directions: |a.d - b.d| / sum|b * d|
+        fe, f2 = {}, {}
+        for q, name in enumerate(FIELDS):
+            a, b = grad[q][inner], ref_grad[q][inner]
+            den, n2 = float(np.abs(b).max()), float(np.linalg.norm(b))
+            fe[name] = float(f"{(np.abs(a - b).max() / den if den > 0 else 0.0):.3e}")
+            f2[name] = float(f"{(np.linalg.norm(a - b) / n2 if n2 > 0 else 0.0):.3e}")
+        out["grad_rel_diff_vs_fp64_adjoint"] = {"linf": fe, "l2": f2}
+        rng2 = np.random.default_rng(12)
+        dd = []
+        for _ in range(8):
+            d = local_direction(U, inner, rng2)
+            ta = grad[(slice(None),) + inner] * d[(slice(None),) + inner]
+            tb = ref_grad[(slice(None),) + inner] * d[(slice(None),) + inner]
+            dd.append(float(f"{abs(float(ta.sum()) - float(tb.sum())) / float(np.abs(tb).sum()):.3e}"))
+        out["directional_rel_diff_vs_fp64_adjoint"] = dd
+    out["note"] = ("FP64 adjoint; dt, TVD branch, p_bc/s_bc held at the forward's values" if prec == abi.PREC_FP64 else
+                   "FP32-mixed context: mixed forward and mixed checkpoints, segments replayed with the mixed step, "
+                   "FP64 transposes at the widened mixed states; dt, TVD branch, p_bc/s_bc held")
+    return out, grad
 
 
 def main():
@@ -601,12 +639,19 @@ def main():
                                               control=True, positivity=True)
     if args.adjoint_steps > 0:
         t = time.perf_counter()
-        res["adjoint_study"] = adjoint_study(table, tab, steps=args.adjoint_steps, ndir=args.adjoint_dirs)
+        hs = (1e-8, 3e-9, 1e-9, 3e-10)
+        res["adjoint_study"], g64 = adjoint_study(table, tab, steps=args.adjoint_steps, ndir=args.adjoint_dirs,
+                                                  hs=hs)
+        # the same horizon in an FP32-mixed context (timing, HBM, gradient
+        # against the FP64 context's)
+        from muram_main_amd import abi as _abi
+        res["adjoint_study_fp32_mixed"], _ = adjoint_study(table, tab, steps=args.adjoint_steps, ndir=0,
+                                                           precision=_abi.PREC_FP32_MIXED, ref_grad=g64)
+        del g64
         # a shorter horizon; the limiter's near ties (about 1e6 per step at
         # margins below 1e-10) make the forward piecewise smooth at any
         # horizon, so the steps stay small
-        res["adjoint_study_10"] = adjoint_study(table, tab, steps=10, every=5, ndir=args.adjoint_dirs,
-                                                hs=(1e-9, 1e-10))
+        res["adjoint_study_10"], _ = adjoint_study(table, tab, steps=10, every=5, ndir=args.adjoint_dirs, hs=hs)
         log(f"adjoint study in {time.perf_counter() - t:.1f}s")
     if not args.no_cpu:
         t = time.perf_counter()

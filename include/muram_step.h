@@ -297,10 +297,15 @@ int mrm_io_read_psf_kernels(const char* dir, int gx, int gy, float dx_ratio, flo
  * differentiate the step it defines (solver.C:109-365, as mrm_step runs it)
  * for callers that need sensitivities of a scalar objective to the state.
  * mrm_adjoint_forward: nsteps steps as mrm_step, keeping the state every
- * `every` steps and each step's dt and TVD branch (FP64 contexts, va_adjust = 0,
- * interior extents >= 4).  mrm_adjoint_backward: seed = dJ/dU after the steps
+ * `every` steps and each step's dt and TVD branch (va_adjust = 0, interior
+ * extents >= 4).  mrm_adjoint_backward: seed = dJ/dU after the steps
  * (SoA, 8 x bufsize doubles, host); grad receives dJ/dU before them.  Each
- * segment of `every` steps is recomputed from its checkpoint.  dt, the TVD
+ * segment of `every` steps is recomputed from its checkpoint.  In an
+ * FP32-mixed context the forward is the mixed step, the checkpoints hold the
+ * mixed state (36 bytes per cell), the backward replays each segment with
+ * the mixed step and applies the FP64 transposes at the widened mixed states
+ * (the FP64 step's derivative along the mixed trajectory), and leaves the
+ * context's state as the forward left it.  dt, the TVD
  * branch, p_bc/s_bc, Qtot and Tau are held fixed; min/max, clamps and table
  * cells are differentiated along the forward's branch.  With
  * fold_initial_ghosts != 0 the initial state's ghost cells are taken as filled

@@ -88,6 +88,8 @@ struct mrm_ctx {
   bool adj_fill0 = false;
   double* adj_mem = nullptr;   // one allocation for the work buffers below
   size_t adj_bytes = 0;
+  char* adj_mx_mem = nullptr;  // FP32-mixed backward: a segment's mixed states + the current state's copy
+  size_t adj_mx_bytes = 0;
   void* mxmem[3] = {nullptr, nullptr, nullptr};
   float* Pm[2] = {nullptr, nullptr};
   bool qtot_set = false, tau_set = false;
@@ -396,6 +398,7 @@ extern "C" int mrm_destroy(mrm_ctx* c) {
   hipFree(c->d_damp);
   for (double* p : c->adj_ckpt) hipFree(p);
   hipFree(c->adj_mem);
+  hipFree(c->adj_mx_mem);
   if (c->h_red) hipHostFree(c->h_red);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -2205,6 +2208,8 @@ static int stage_mx(mrm_ctx* c, const mrm_run_desc* run, int stage, MxState in, 
   return MRM_OK;
 }
 
+static int step_mixed_body(mrm_ctx* c, const mrm_run_desc* run, double dt_prev, bool split);
+
 // One time step in the FP32-mixed form: the fused step's schedule on mixed
 // state buffers (FP32 rho, M, B, p; FP64 e), with the cmax reduction and dt
 // on the host in FP64.
@@ -2249,6 +2254,20 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   double cfl = 0.0;
   const double dt_prev = run->dt;   // stage 1's conduction constants (mhdres_SR.C:136-145)
   if ((rc = mrm_sync_timestep(c, run, dt_rad, eta_dt_limit(c, run, c->dxmin / cm), &cfl))) return rc;
+  c->last_dt = run->dt;
+  c->last_split = !(cfl <= run->CFL_tvd);
+  return step_mixed_body(c, run, dt_prev, c->last_split);
+}
+
+// the stages, TVDlimit and ghost fills of a mixed step whose dt (run->dt) and
+// TVD split are decided; the current state's pressure must be valid
+static int step_mixed_body(mrm_ctx* c, const mrm_run_desc* run, double dt_prev, bool split) {
+  const Geom& G = c->G;
+  float* Pc = c->Pm[0];
+  float* Pa = c->Pm[1];
+  unsigned long long* cmax = c->d_red + RED_CMAX;
+  int rc;
+  const bool va_adj = c->phys.params[2] > 0.0;
   const int i0 = c->cur, i1 = (c->cur + 1) % 3, i2 = (c->cur + 2) % 3;
   MxState* M = c->M;
   struct { int in; float* pin; int out; float* pout; } st[4] = {
@@ -2296,7 +2315,7 @@ static int step_mixed(mrm_ctx* c, mrm_run_desc* run, double dt_rad) {
   // with Clean_divB its last correction pass finishes the cells
   const bool divb = c->phys.divB[0] > 0.0;
   const MxEpilogue* tvd_epi = divb ? nullptr : &X;
-  if (cfl <= run->CFL_tvd) {
+  if (!split) {
     if ((rc = tvd_mx(c, M[i2], Pa, run->dt, tvd_epi))) return rc;
   } else {
     if ((rc = tvd_mx(c, M[i2], Pa, 0.5 * run->dt, nullptr))) return rc;
@@ -2496,10 +2515,22 @@ static int adj_step_back(mrm_ctx* c, int n, AdjBufs& B) {
   return MRM_OK;
 }
 
+// bytes of one mixed state (7 FP32 fields, padded, and the FP64 energy), as
+// allocated at context creation
+static size_t mx_state_bytes(const mrm_ctx* c) {
+  const size_t bs = (size_t)c->G.bs;
+  return (7 * bs * sizeof(float) + 255) / 256 * 256 + bs * sizeof(double);
+}
+
+// In an FP32-mixed context the forward runs the mixed step and keeps its
+// checkpoints in the mixed format (36 instead of 64 bytes per cell); the
+// backward replays each segment with the mixed step (recorded dt and TVD
+// branch) and applies the FP64 transposes at the widened mixed states: the
+// adjoint of the FP64 step along the mixed trajectory.
 extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps, int every) {
   if (!c || !run || nsteps < 1 || every < 1) return MRM_ERR_ARG;
   if (int rc = drop_settle(c)) return rc;
-  NEED_FP64(c, "mrm_adjoint_forward");
+  const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_adjoint_forward: no EOS table loaded");
   if (run->need_diagnostics) return fail(c, MRM_ERR_UNSUPPORTED, "diagnostics are outside the path");
   if (c->force_sequence || c->phys.params[2] > 0.0 || c->phys.divB[0] > 0.0 || c->phys.params[4] > 0.0 ||
@@ -2514,14 +2545,16 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   c->adj_rec.clear();
   c->adj_every = every;
   const size_t f8b = 8 * (size_t)c->G.bs * sizeof(double);
+  const size_t cb = mx ? mx_state_bytes(c) : f8b;
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0) {
       double* p = nullptr;
-      HIPCHK(c, hipMalloc(&p, f8b));
+      HIPCHK(c, hipMalloc(&p, cb));
       c->adj_ckpt.push_back(p);
-      HIPCHK(c, hipMemcpyAsync(p, c->S[c->cur], f8b, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(p, mx ? c->mxmem[c->cur] : (void*)c->S[c->cur], cb, hipMemcpyDeviceToDevice,
+                               c->stream));
     }
-    int rc = step_fused(c, run, dt_rad);
+    int rc = mx ? step_mixed(c, run, dt_rad) : step_fused(c, run, dt_rad);
     if (rc) return rc;
     c->adj_rec.push_back({c->last_dt, c->last_split});
     run->iteration++;
@@ -2532,10 +2565,40 @@ extern "C" int mrm_adjoint_forward(mrm_ctx* c, mrm_run_desc* run, double dt_rad,
   return MRM_OK;
 }
 
+// the mixed state in buffer src (mx_state_bytes) widened to FP64 U
+static int mx_widen(mrm_ctx* c, const void* src, double* U) {
+  MxState S;
+  S.f = (float*)src;
+  S.e = (double*)((char*)src + (mx_state_bytes(c) - (size_t)c->G.bs * sizeof(double)));
+  HIPCHK(c, mixed::launch_convert(c->stream, false, c->G.bs, U, S));
+  return MRM_OK;
+}
+
+// the mixed step n of the recorded forward from the mixed state src into dst,
+// with the step's dt and TVD branch (the context's current state buffer is
+// the working space)
+static int mx_replay(mrm_ctx* c, int n, const void* src, void* dst) {
+  const size_t mb = mx_state_bytes(c);
+  HIPCHK(c, hipMemcpyAsync(c->mxmem[c->cur], src, mb, hipMemcpyDeviceToDevice, c->stream));
+  unsigned nb = nblk(c->G.bs);
+  if (nb > 65536u) nb = 65536u;
+  // (the cmax slot takes a maximum that nothing reads)
+  HIPCHK(c, mixed::launch_eos_p(c->stream, nb, c->G, c->E, c->M[c->cur], c->Pm[0], c->inv_va2max,
+                                c->d_red + RED_CMAX, nullptr, 0.0, 0.0));
+  mrm_run_desc r = {};
+  r.dt = c->adj_rec[n].dt;
+  int rc = step_mixed_body(c, &r, r.dt, c->adj_rec[n].split);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(dst, c->mxmem[c->cur], mb, hipMemcpyDeviceToDevice, c->stream));
+  return MRM_OK;
+}
+
+static int adjoint_backward_mixed(mrm_ctx* c, const double* seed, double* grad, int fold_initial_ghosts);
+
 extern "C" int mrm_adjoint_backward(mrm_ctx* c, const double* seed, double* grad, int fold_initial_ghosts) {
   if (!c || !seed || !grad) return MRM_ERR_ARG;
-  NEED_FP64(c, "mrm_adjoint_backward");
   if (c->adj_rec.empty()) return fail(c, MRM_ERR_STATE, "mrm_adjoint_backward: no recorded forward");
+  if (c->precision == MRM_PREC_FP32_MIXED) return adjoint_backward_mixed(c, seed, grad, fold_initial_ghosts);
   AdjBufs B;
   int rc = adj_buffers(c, B);
   if (rc) return rc;
@@ -2565,6 +2628,67 @@ extern "C" int mrm_adjoint_backward(mrm_ctx* c, const double* seed, double* grad
   HIPCHK(c, hipMemcpyAsync(grad, B.lamA, f8b, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRM_OK;
+}
+
+// The backward of an FP32-mixed context: the segment's mixed states come from
+// replaying its checkpoint with the mixed step, each is widened to FP64 for
+// the step's recompute and transpose; the context's state is left as the
+// forward left it (its pressure is formed again by the next step).
+static int adjoint_backward_mixed(mrm_ctx* c, const double* seed, double* grad, int fold_initial_ghosts) {
+  AdjBufs B;
+  const int every = c->adj_every;
+  c->adj_every = 1;   // one FP64 segment slot: the widened state of the step being transposed
+  int rc = adj_buffers(c, B);
+  c->adj_every = every;
+  if (rc) return rc;
+  const size_t mb = mx_state_bytes(c), f8b = 8 * (size_t)c->G.bs * sizeof(double);
+  // the mixed segment states and a copy of the context's current state (kept
+  // by the context, like the FP64 work buffers, for the next backward)
+  const size_t need = (size_t)(every + 1) * mb;
+  if (c->adj_mx_bytes < need) {
+    hipFree(c->adj_mx_mem);
+    c->adj_mx_mem = nullptr;
+    c->adj_mx_bytes = 0;
+    HIPCHK(c, hipMalloc(&c->adj_mx_mem, need));
+    c->adj_mx_bytes = need;
+  }
+  char* ms = c->adj_mx_mem;
+  auto slot = [&](int q) { return (void*)(ms + (size_t)q * mb); };
+  const int cur0 = c->cur;
+  HIPCHK(c, hipMemcpyAsync(slot(every), c->mxmem[cur0], mb, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(B.lamA, seed, f8b, hipMemcpyHostToDevice, c->stream));
+  const int N = (int)c->adj_rec.size();
+  const int nseg = (N + every - 1) / every;
+  for (int sg = nseg - 1; sg >= 0 && !rc; sg--) {
+    const int n0 = sg * every, n1 = std::min(N, n0 + every);
+    if (hipMemcpyAsync(slot(0), c->adj_ckpt[sg], mb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+      rc = fail(c, MRM_ERR_HIP, "hipMemcpyAsync checkpoint");
+      break;
+    }
+    for (int n = n0; n < n1 - 1 && !rc; n++) rc = mx_replay(c, n, slot(n - n0), slot(n - n0 + 1));
+    for (int n = n1 - 1; n >= n0 && !rc; n--) {
+      if ((rc = mx_widen(c, slot(n - n0), B.seg))) break;
+      if ((rc = adj_recompute(c, n, B.seg, B, nullptr))) break;
+      rc = adj_step_back(c, n, B);
+    }
+  }
+  if (!rc && fold_initial_ghosts) {
+    // the initial state's ghosts came from mrm_exchange_grid + mrm_set_boundary_conditions
+    if (!(rc = mx_widen(c, c->adj_ckpt[0], B.seg)) && !(rc = bottom_sums(c, 0, B.seg, B.small + 10))) {
+      if (adj::launch_bc(c->stream, c->G, c->E, bc_params(c), B.small + 10, B.seg, B.lamA, c->d_part,
+                         B.small + 16) != hipSuccess ||
+          adj::launch_exchange(c->stream, c->G, B.lamA) != hipSuccess)
+        rc = fail(c, MRM_ERR_HIP, "adjoint initial ghosts");
+    }
+  }
+  if (!rc && hipMemcpyAsync(grad, B.lamA, f8b, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = fail(c, MRM_ERR_HIP, "hipMemcpyAsync gradient");
+  // the forward's final state back in the context
+  c->cur = cur0;
+  (void)hipMemcpyAsync(c->mxmem[cur0], slot(every), mb, hipMemcpyDeviceToDevice, c->stream);
+  c->fast_valid = false;
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = fail(c, MRM_ERR_HIP, "hipStreamSynchronize");
+  return rc;
 }
 
 // ------------------------------------------------------------------ timers

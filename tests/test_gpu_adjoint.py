@@ -23,14 +23,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 class Problem:
-    def __init__(self, gsize, seed, nsteps, every, v_rms=2.0e5, dt_frac=0.5, layer=None):
+    def __init__(self, gsize, seed, nsteps, every, v_rms=2.0e5, dt_frac=0.5, layer=None, precision=abi.PREC_FP64,
+                 dt=None):
         self.table = open(os.path.join(HERE, "golden", "eos_small.dat"), "rb").read()
         tab = eos_table.load_tables(self.table)
         box = initial.BoxSpec(gsize=gsize, dx=(3.2e6,) * 3, seed=seed, v_rms=v_rms)
         self.U = initial.make_state(tab, box)
         self.g = config.grid_desc(box.gsize, box.dx)
         self.phys = config.stratified_phys()
-        self.m = MhdGrid(self.g, self.phys)
+        self.m = MhdGrid(self.g, self.phys, precision)
         self.m.eos_init(self.table)
         self.nsteps, self.every = nsteps, every
         self.layer = 2 + (gsize[0] // 2 if layer is None else layer)
@@ -41,7 +42,7 @@ class Problem:
         self.m.SetBoundaryConditions(r, 0)
         self.p_bc, self.s_bc = self.m.bc()
         self.m.step(r, config.DT_RAD_OFF, 1)
-        self.dt = dt_frac * r.dt
+        self.dt = dt_frac * r.dt if dt is None else dt
         self.inner = (slice(2, -2),) * 3
 
     def run(self, U0, record=False):
