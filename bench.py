@@ -53,6 +53,13 @@ KERNEL_BYTES = {
 # a step, so the bytes per launch are the mean over the stages.
 STAGE_BYTES = {"fp64": (144.0, 208.0, 208.0, 200.0), "fp32": (80.0, 116.0, 116.0, 112.0)}
 KERNEL_BYTES_FUSED = dict(KERNEL_BYTES, tvd_sweep_z=(144.0, "interior"))   # + new p (epilogue)
+# drop-in sequence with the deferred stage: MHD_Residual forms ConsToPrim's
+# EOS fields (read rho, M, e 40; write p, T, ne, rhoi, amb 40 per buffer
+# cell) and Source_Integrate_Tcheck runs the stage kernel from the
+# conservative state (read U 64 + p 8 + U0 64, write U 64 = 200 B; at stage 1
+# U0 is U itself, 136 B)
+KERNEL_BYTES_DROPIN = dict(KERNEL_BYTES, cons_to_prim=(80.0, "buffer"))
+STAGE_BYTES_DROPIN = (136.0, 200.0, 200.0, 200.0)
 # FP32-mixed form: a cell is rho, M, B in FP32 (28 B) + e in FP64 (8 B) = 36 B, p 4 B
 KERNEL_BYTES_MX = {
     "tvd_sweep_x": (76.0, "interior"),    # read U 36 + p 4; write U 36
@@ -598,7 +605,8 @@ def main():
         value = ncell * nseq / wall / 1e6
         drop_in = dict(value=round(value, 2), ms_per_step=round(wall / nseq * 1e3, 3), steps=nseq,
                        hbm_frac_model=round(value * 1e6 * MODEL_BYTES["fp64"] / (HBM_PEAK_GBS * 1e9), 4),
-                       roofline=roofline(kernels, KERNEL_BYTES, ncell, nbuf, "fp64_sequence"),
+                       roofline=roofline(kernels, KERNEL_BYTES_DROPIN, ncell, nbuf, "fp64_sequence",
+                                         STAGE_BYTES_DROPIN),
                        kernels=kernel_summary(kernels, wall),
                        path="sequence: one C-ABI call per reference operation (solver.C:109-365)")
         log(f"drop-in sequence: {drop_in['ms_per_step']} ms/step, {value:.1f} Mcell/s")

@@ -706,20 +706,38 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
     double S0[MRM_NV], S1[MRM_NV], Fm[MRM_NV], F[MRM_NV];
     double qm = 0.0, qv = 0.0, qvm = 0.0;
     int slot = 0;    // ring slot of point s; point s-2 is in slot (s+1) % 3
+    // PF: point s's conserved values and pressure are loaded during point
+    // s-1's work (the last iteration loads point sz-1 again and does not use
+    // it); the sweeps with the epilogue or the diagnostics have no registers
+    // to spare for them
+    constexpr bool PF = !EPI && !DIAG;
+    double un[MRM_NV], pn = 0.0;
+    if (PF) {
+#pragma unroll
+      for (int q = 0; q < MRM_NV; q++) un[q] = U[q * bs + base];
+      pn = Pr[base];
+    }
     for (int s = 0; s < sz; s++) {
       long node = base + s * str;
       double u[MRM_NV];
 #pragma unroll
       for (int q = 0; q < MRM_NV; q++) {
-        u[q] = U[q * bs + node];
+        u[q] = PF ? un[q] : U[q * bs + node];
         ring[slot][q][tl] = u[q];
+      }
+      const double pr = PF ? pn : Pr[node];
+      if (PF) {
+        const long nn = base + (long)min(s + 1, sz - 1) * str;
+#pragma unroll
+        for (int q = 0; q < MRM_NV; q++) un[q] = U[q * bs + nn];
+        pn = Pr[nn];
       }
       const int slot2 = slot == 2 ? 0 : slot + 1;
       slot = slot2;
       double tau = HAS_TAU ? P.tau[node] : 0.0;
       double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
       if (CNT) cnt.key = br_key(D, bs, node);
-      tvd_point(P, u, Pr[node], tau, hfb1, hft1, D == 0, W3, cn);
+      tvd_point(P, u, pr, tau, hfb1, hft1, D == 0, W3, cn);
       if (s >= 2) {
         const int ip = s - 1;   // slope at point s-1
 #pragma unroll

@@ -51,6 +51,9 @@ static mrm_ctx* ctx_of(const GridData& G, const PhysicsData& P) {
   std::copy(P.divB, P.divB + 3, pd.divB);
   mrm_ctx* c = nullptr;
   chk(nullptr, mrm_create(&gd, &pd, MRM_PREC_FP64, &c), "mrm_create");
+  // potential-field top boundary: the PSF kernels from the run directory,
+  // where potential_sd_fftw3.C:77-86 reads them
+  if (P.bnd[1] > 0) chk(c, mrm_potential_kernels_load(c, "."), "mrm_potential_kernels_load");
   g_ctx[&G] = c;
   return c;
 }
@@ -139,10 +142,10 @@ void exchange_grid_acc(GridData& G, const PhysicsData& P, const int) {          
 }
 
 void SetBoundaryConditions(const RunData& R, GridData& G, const PhysicsData& P,    // icbc.H:10-12
-                           const int stage, const int, RTS*) {
+                           const int stage, const int pt_update, RTS*) {
   mrm_ctx* c = ctx_of(G, P);
   mrm_run_desc r = run_of(R);
-  chk(c, mrm_set_boundary_conditions(c, &r, stage), "SetBoundaryConditions");
+  chk(c, mrm_set_boundary_conditions_pt(c, &r, stage, pt_update), "SetBoundaryConditions");
 }
 
 // ACCH.h:49-50 -- the solver's mirror hooks (ACCH.C's, renamed with -D):
