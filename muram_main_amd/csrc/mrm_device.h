@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "mrm_types.h"
+#include "mrm_math.h"
 
 #ifndef MRM_NS
 #define MRM_NS exact
@@ -113,8 +114,9 @@ struct EosLook {
   float t11[NOUT], t10[NOUT], t01[NOUT], t00[NOUT];
 };
 
-// m1..m3 become the velocities and e the internal energy, as in ConsToPrim
-template <int NOUT>
+// m1..m3 become the velocities and e the internal energy, as in ConsToPrim;
+// FLOG: ln rho and ln eps through mrm_math::mlog (false: the library's log)
+template <int NOUT, bool FLOG = true>
 MRM_HD void eos_issue(const EosDev& E, double d, double& m1, double& m2, double& m3, double& e,
                       EosLook<NOUT>& g) {
   double dn = 1.0 / d;
@@ -124,8 +126,8 @@ MRM_HD void eos_issue(const EosDev& E, double d, double& m1, double& m2, double&
   double vv = (m1 * m1 + m2 * m2 + m3 * m3);
   e = e - 0.5 * vv * d;
   double eps = e * dn + E.eps_off;
-  double lgr = log(d);
-  double ep = log(eps);
+  double lgr = FLOG ? mrm_math::mlog(d) : log(d);
+  double ep = FLOG ? mrm_math::mlog(eps) : log(eps);
   double lr = lgr;
   int i_d = cvt_int((lr - E.xlr0) * E.inv_del_lr);
   i_d = imax_(0, i_d);
@@ -227,11 +229,11 @@ MRM_HD void eos_finish(const EosDev& E, const EosLook<NOUT>& g, double out[5]) {
   }
 }
 
-template <int NOUT, bool WANT_T = false>
+template <int NOUT, bool WANT_T = false, bool FLOG = true>
 MRM_HD void cons_to_prim_cell(const EosDev& E, double d, double& m1, double& m2, double& m3, double& e,
                               double out[5]) {
   EosLook<NOUT> g;
-  eos_issue<NOUT>(E, d, m1, m2, m3, e, g);
+  eos_issue<NOUT, FLOG>(E, d, m1, m2, m3, e, g);
   eos_finish<NOUT, WANT_T>(E, g, out);
 }
 
@@ -666,6 +668,9 @@ struct TvdCnt {
 MRM_HD double fmax3_(double a, double b, double c) { return fmax(fmax(a, b), c); }
 
 // per-point quantities (tvdlimit_SR.C:269-385)
+// FLOG: the logarithms through mrm_math::mlog (false: the library's log,
+// for kernels whose registers are at their limit)
+template <bool FLOG = true>
 MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, double tau,
                                           double hfb1, double hft1, bool xsweep, TvdPt& o,
                                           TvdCnt* cn = nullptr) {
@@ -719,11 +724,11 @@ MRM_HD void tvd_point(const TvdParams& P, const double u[MRM_NV], double pres, d
     o.hd = 0.5 * P.idxd1 * smax(0.0, smin(h, P.cmax - o.cm));
   }
   if (P.rho_log) {
-    v0 = log(v0);
-    v4 = log(v4);
+    v0 = FLOG ? mrm_math::mlog(v0) : log(v0);
+    v4 = FLOG ? mrm_math::mlog(v4) : log(v4);
     o.lnr = v0;
   } else {
-    o.lnr = log(v0);
+    o.lnr = FLOG ? mrm_math::mlog(v0) : log(v0);
   }
   o.v[0] = v0; o.v[1] = v1; o.v[2] = v2; o.v[3] = v3; o.v[4] = v4;
 }

@@ -737,7 +737,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
       double tau = HAS_TAU ? P.tau[node] : 0.0;
       double hfb1 = D == 0 ? P.hfb[s] : lhfb, hft1 = D == 0 ? P.hft[s] : lhft;
       if (CNT) cnt.key = br_key(D, bs, node);
-      tvd_point(P, u, pr, tau, hfb1, hft1, D == 0, W3, cn);
+      tvd_point<!EPI>(P, u, pr, tau, hfb1, hft1, D == 0, W3, cn);
       if (s >= 2) {
         const int ip = s - 1;   // slope at point s-1
 #pragma unroll
