@@ -348,6 +348,109 @@ def precision_study(wl, table, tab, nsteps, control=True, positivity=False):
     return out
 
 
+def storage_floor_study(table, tab, nsteps=100, gsize=(256, 64, 64)):
+    """The box512_v3 distributions on a 256 x 64 x 64 crop, nsteps steps: the
+    FP32-mixed form, the FP64 form from the FP32-rounded state (control_init)
+    and the FP64 form with rho, M and B rounded to FP32 after every step
+    (control_each: FP32 storage with FP64 arithmetic), each against the FP64
+    form; relative L2 per field over the lower and upper halves.  control_each
+    is the floor that any FP32-storage scheme meets on this configuration;
+    control_each_stage rounds after every RK stage and after TCheck as well,
+    where the FP32-mixed form stores its state (the reference's stage loop
+    through the drop-in operations, FP64 arithmetic throughout)."""
+    from muram_main_amd import abi, config, initial
+    from muram_main_amd.mhd import MhdGrid
+    wl0 = config.workloads()["box512_v3"]
+    box = initial.BoxSpec(gsize=gsize, dx=wl0.box.dx, v_rms=wl0.box.v_rms, seed=wl0.box.seed)
+    U = initial.make_state(tab, box)
+    g = config.grid_desc(box.gsize, box.dx)
+
+    def rnd(V):
+        V = V.copy()
+        for v in (0, 1, 2, 3, 5, 6, 7):
+            V[v] = V[v].astype(np.float32).astype(np.float64)
+        return V
+
+    def run(U0, prec, each=False):
+        m = MhdGrid(g, wl0.phys, prec)
+        m.eos_init(table)
+        r = config.run_desc()
+        m.set_state(U0)
+        m.exchange_grid()
+        m.SetBoundaryConditions(r, 0)
+        for _ in range(nsteps):
+            m.step(r, config.DT_RAD_OFF, 1)
+            if each:
+                m.set_state(rnd(m.get_state()))
+        out = m.get_state()
+        m.close()
+        return out
+
+    def run_staged(U0, round_stages):
+        """the reference's stage loop (solver.C:109-365) through the drop-in
+        operations, with rho, M and B rounded to FP32 after each stage's
+        update and after TCheck (round_stages), i.e. where the FP32-mixed
+        form stores its state"""
+        m = MhdGrid(g, wl0.phys, abi.PREC_FP64)
+        m.eos_init(table)
+        r = config.run_desc()
+        m.set_state(U0)
+        m.exchange_grid()
+        m.SetBoundaryConditions(r, 0)
+        for _ in range(nsteps):
+            m.SaveCons()
+            cfl = 0.0
+            for stage in range(1, 5):
+                m.ConsToPrim()
+                if stage == 1:
+                    m.Adjust_Valf_Max(r)
+                dt_mhd = m.MHD_Residual(r)
+                if stage == 1:
+                    cfl = m.Sync_timestep(r, config.DT_RAD_OFF, dt_mhd)
+                m.Source_Integrate_Tcheck(r, stage)
+                if round_stages:
+                    m.set_state(rnd(m.get_state()))
+                m.exchange_grid()
+                m.SetBoundaryConditions(r, stage)
+            if cfl <= r.CFL_tvd:
+                m.TVDlimit(r, r.dt)
+            else:
+                m.TVDlimit(r, 0.5 * r.dt)
+                m.exchange_grid()
+                m.TVDlimit(r, 0.5 * r.dt)
+            m.TCheck(r)
+            if round_stages:
+                m.set_state(rnd(m.get_state()))
+            m.exchange_grid()
+            m.SetBoundaryConditions(r, 5)
+            r.iteration += 1
+            r.globiter += 1
+            r.time += r.dt
+        out = m.get_state()
+        m.close()
+        return out
+
+    t = time.perf_counter()
+    ref = run(U, abi.PREC_FP64)
+    res = {"mixed": run(U, abi.PREC_FP32_MIXED), "control_init": run(rnd(U), abi.PREC_FP64),
+           "control_each": run(rnd(U), abi.PREC_FP64, each=True),
+           "control_each_stage": run_staged(rnd(U), True)}
+    # the Python stage loop itself, unrounded, against mrm_step (a consistency check)
+    seq = run_staged(U, False)
+    n0 = gsize[0]
+    inner = (slice(2, -2),) * 2
+    sl = {"lower_half": (slice(None),) + inner + (slice(2, 2 + n0 // 2),),
+          "upper_half": (slice(None),) + inner + (slice(2 + n0 // 2, 2 + n0),)}
+    out = {"workload": f"box512_v3 distributions on {gsize[0]}x{gsize[1]}x{gsize[2]}", "steps": nsteps,
+           "stage_loop_vs_mrm_step_max_l2": max(e["l2"] for e in _errs(seq, ref, (slice(None),) + inner +
+                                                                        (slice(2, 2 + n0),)).values())}
+    for h, s_ in sl.items():
+        out[h] = {name: {f: e["l2"] for f, e in _errs(X, ref, s_).items()} for name, X in res.items()}
+        out[h + "_max_l2"] = {name: max(v.values()) for name, v in out[h].items()}
+    log(f"storage floor study in {time.perf_counter() - t:.1f}s")
+    return out
+
+
 def local_direction(U, inner, rng):
     """random interior direction scaled by each cell's own magnitudes: rho and
     e relative, momentum as rho times the rms velocity, B by its rms"""
@@ -642,6 +745,9 @@ def main():
     if args.error_steps > 0 and "fp32" in res_forms:
         res["precision_study"] = precision_study(config.workloads()[args.error_workload], table, tab,
                                                  args.error_steps)
+        # the FP32-storage floor of the same distributions (a crop, where the
+        # every-step storage rounding of the FP64 form is affordable)
+        res["precision_study"]["storage_floor_crop"] = storage_floor_study(table, tab, args.error_steps)
     if args.corona_steps > 0 and "fp32" in res_forms:
         res["corona_study"] = precision_study(config.workloads()["corona"], table, tab, args.corona_steps,
                                               control=True, positivity=True)

@@ -994,14 +994,17 @@ __device__ __forceinline__ void mx_tvd_point(const MxTvdParams& P, const float u
     float h = P.vhyp * fsqrt(o.vsqr);
     o.hd = 0.5f * P.idxd1 * fsmax(0.0f, fsmin(h, P.cmax - o.cm));
   }
+  // the logarithms within half an ulp (mlogf): the slopes of ln rho and
+  // ln eps are differences of values of 10 to 45, so a few ulps of error in
+  // each log are a large error of the slope
   float v4;
   if (P.rho_log) {
-    v0 = logf(v0);
-    v4 = logf((float)eps);
+    v0 = mrm_math::mlogf(v0);
+    v4 = mrm_math::mlogf((float)eps);
     o.lnr = v0;
   } else {
     v4 = (float)eps;
-    o.lnr = logf(v0);
+    o.lnr = mrm_math::mlogf(v0);
   }
   o.v[0] = v0; o.v[1] = v1; o.v[2] = v2; o.v[3] = v3; o.v[4] = v4;
 }

@@ -1,9 +1,11 @@
-"""CPU check of the FP64 log the step's hot loops use
-(muram_main_amd/csrc/mrm_math.h): compiled for the host with g++ and compared
-with the C library (glibc) over 2e6 arguments per sweep -- near 1, the
-density and specific-energy ranges of the EOS lookup, every normal double,
-denormals -- to within one ulp, plus the special values (0, negative, NaN,
-inf)."""
+"""CPU check of the logarithms the step's hot loops use
+(muram_main_amd/csrc/mrm_math.h), compiled for the host with g++: the FP64
+mlog against the C library (glibc) over 2e6 arguments per sweep -- near 1,
+the density and specific-energy ranges of the EOS lookup, every normal
+double, denormals -- to within one ulp; the FP32 mlogf against the exact
+logarithm (glibc's FP64 log) to within one FP32 ulp everywhere and 0.6 ulp
+over the density and specific-energy ranges (correct rounding is 0.5); the
+special values (0, negative, NaN, inf) of both."""
 import os
 import subprocess
 
@@ -21,5 +23,10 @@ def test_log_within_one_ulp(tmp_path):
     for fn, name, n, worst, at in sweeps:
         print(fn, name, n, "max ulp", worst, "at", at)
         assert float(worst) <= 1.0, (fn, name, worst, at)
+    sweeps = [l.split() for l in out if l.startswith("logf ")]
+    assert len(sweeps) == 4
+    for fn, name, n, worst, at in sweeps:
+        print(fn, name, n, "max ulp", worst, "at", at)
+        assert float(worst) <= (0.6 if name in ("density", "specific_energy") else 1.0), (fn, name, worst, at)
     special = [l for l in out if l.startswith("special")][0]
     assert special.split()[1] == "0"
