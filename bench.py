@@ -609,9 +609,9 @@ def main():
                     help="param_spitzer (Spitzer heat conduction); default: the workload's (0)")
     ap.add_argument("--divb", action="store_true",
                     help="Clean_divB on (divB_switch 1, at most 5 passes, error target 0.2)")
-    ap.add_argument("--stage-zwin", default=None, choices=["0", "1", "2", "3"],
-                    help="stage kernel: 3 two wave groups (default), 2 fluxes shared through LDS in one group, "
-                         "1 register window, 0 re-read")
+    ap.add_argument("--stage-zwin", default=None, choices=["0", "1", "2", "3", "4"],
+                    help="stage kernel: 4 three wave groups, the EOS in its own (default), 3 two wave groups, "
+                         "2 fluxes shared through LDS in one group, 1 register window, 0 re-read")
     ap.add_argument("--fma", default=None, choices=["0", "1"])
     ap.add_argument("--calibrate", action="store_true",
                     help="run a 512 MiB device copy first (reference kernel for PMC byte counters)")

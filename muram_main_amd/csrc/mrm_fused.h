@@ -427,8 +427,10 @@ k_stage_f(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
 // Each iteration has two barriers; the handover buffers are double
 // buffered by iteration parity.  The sums are formed in k_stage_f's order
 // ((0 + R_x) + R_y) + R_z, so the result is bit for bit k_stage_f's.
-template <bool ETA = false, bool ERES = false>
-__global__ void __launch_bounds__(2 * ST_SX * ST_SY, 1)
+// EG: a third group of four waves (threads 512 .. 767) forms the EOS
+// pressure instead of the xy group, so the block runs three waves per SIMD.
+template <bool ETA = false, bool ERES = false, bool EG = false>
+__global__ void __launch_bounds__((EG ? 3 : 2) * ST_SX * ST_SY, 1)
 k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict__ Pin, const double* __restrict__ U0,
           double* __restrict__ Uout, double* __restrict__ Pout, const double* __restrict__ Qtot) {
   constexpr int NFP = nfp_<ETA>();
@@ -447,8 +449,9 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
   lb /= A.ntx;
   const int ty_t = (int)(lb % A.nty);
   const int kc = (int)(lb / A.nty);
-  const bool zg = threadIdx.x >= NT;
-  const int t = zg ? threadIdx.x - NT : threadIdx.x;
+  const int grp = threadIdx.x / NT;   // 0: xy group, 1: z group, 2: EOS group (EG)
+  const bool zg = grp == 1;
+  const int t = threadIdx.x - grp * NT;
   const int tx = t % ST_SX, ty = t / ST_SX;
   const int i0 = G.lb0 + tx_t * ST_SX, j0 = G.lb1 + ty_t * ST_SY;
   const int i = i0 + tx, j = j0 + ty;
@@ -459,7 +462,26 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
   const bool own_ok = i < G.n0 && j < G.n1;
   const unsigned col = (unsigned)i + (unsigned)j * (unsigned)G.s1;
   const unsigned s2u = (unsigned)G.s2;
-  if (!zg) {
+  if (EG && grp == 2) {
+    // ---- EOS group: the pressure of plane kbeg + m - 2's new state
+    for (int m = 0; m <= NK + 1; m++) {
+      const bool eos = A.write_p && valid && m >= 2;
+      EosLook<2> pg;
+      if (eos) {
+        const int sl = (m - 1) & 1;
+        double d = aout[sl][0][t], m1 = aout[sl][1][t], m2 = aout[sl][2][t], m3 = aout[sl][3][t],
+               e = aout[sl][4][t];
+        eos_issue<2>(A.E, d, m1, m2, m3, e, pg);
+      }
+      __syncthreads();
+      if (eos) {
+        double pr[5];
+        eos_finish<2>(A.E, pg, pr);
+        st32(Pout, col + (unsigned)(kbeg + m - 2) * s2u, pr[0]);
+      }
+      __syncthreads();
+    }
+  } else if (!zg) {
     // ---- xy group: the ring point of this thread as in k_stage_f
     int ring = 0;
     int r_row = 0, r_col = 0, r_i = 0, r_j = 0;
@@ -522,7 +544,7 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
         }
       }
       // the pressure of plane kbeg + m - 2's new state (handed over in iteration m - 1)
-      const bool eos = A.write_p && valid && m >= 2;
+      const bool eos = !EG && A.write_p && valid && m >= 2;
       EosLook<2> pg;
       if (eos) {
         const int sl = (m - 1) & 1;
@@ -573,7 +595,7 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
         for (int f = 0; f < ST_NF; f++) w[q + 1][f] = 1.0;
       }
     }
-    if (own_ok) load_raw(Uin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
+    if (!EG && own_ok) load_raw(Uin, Pin, bs, col + (unsigned)(kbeg + 2) * s2u, rz);
     auto getz = [&](int s, double o[9]) {
 #pragma unroll
       for (int f = 0; f < ST_NF; f++) o[f] = w[s][f];
@@ -588,9 +610,16 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
 #pragma unroll
           for (int f = 0; f < ST_NF; f++) w[q][f] = w[q + 1][f];
         if (own_ok) {
-          prim_of_raw(rz, w[4]);     // plane k + 2
-          // k + 3 <= kend + 1 <= lend2 + 2 = n2 - 1: inside the buffer
-          if (k + 1 < kend) load_raw(Uin, Pin, bs, col + (unsigned)(k + 3) * s2u, rz);
+          if (EG) {
+            // three groups: no load ahead (the registers of rz), plane k + 2
+            // loaded here; k + 2 <= lend2 + 2 = n2 - 1
+            load_raw(Uin, Pin, bs, col + (unsigned)(k + 2) * s2u, rz);
+            prim_of_raw(rz, w[4]);
+          } else {
+            prim_of_raw(rz, w[4]);     // plane k + 2
+            // k + 3 <= kend + 1 <= lend2 + 2 = n2 - 1: inside the buffer
+            if (k + 1 < kend) load_raw(Uin, Pin, bs, col + (unsigned)(k + 3) * s2u, rz);
+          }
         }
         // without resistivity the dim-2 terms do not depend on the xy
         // group's sums: form them while that group packs its fluxes (with

@@ -16,13 +16,22 @@ inline void launch_stage_z(hipStream_t st, unsigned nb, const StageArgs& A, cons
   else k_stage<ZWIN><<<nb, ST_SX * ST_SY, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
 }
 
-// variant 3: k_stage_f's work over two wave groups (k_stage_w); 2: fluxes
-// shared through LDS (k_stage_f); 1: register window of
-// primitives (k_stage<true>); 0: dim-2 neighbours re-read (k_stage<false>)
+// variant 4: k_stage_w with the EOS pressure in a third wave group (three
+// waves per SIMD; the two-group kernel with resistivity); 3: k_stage_f's
+// work over two wave groups (k_stage_w); 2: fluxes shared through LDS
+// (k_stage_f); 1: register window of primitives (k_stage<true>); 0: dim-2
+// neighbours re-read (k_stage<false>)
 inline hipError_t launch_stage(hipStream_t st, unsigned nb, int variant, const StageArgs& A, const double* Uin,
                                const double* Pin, const double* U0, double* Uout, double* Pout,
                                const double* Q) {
-  if (variant == 3) {
+  // with resistivity the three-group kernel would not fit three waves per
+  // SIMD without spilling: the two-group kernel runs instead
+  if (variant == 4 && A.RP.eta > 0.0) variant = 3;
+  if (variant == 4) {
+    const unsigned nt = 3 * ST_SX * ST_SY;
+    if (A.eres != nullptr) k_stage_w<false, true, true><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+    else k_stage_w<false, false, true><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);
+  } else if (variant == 3) {
     const bool eta = A.RP.eta > 0.0, eres = A.eres != nullptr;
     const unsigned nt = 2 * ST_SX * ST_SY;
     if (eta && eres) k_stage_w<true, true><<<nb, nt, 0, st>>>(A, Uin, Pin, U0, Uout, Pout, Q);

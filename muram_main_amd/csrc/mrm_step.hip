@@ -71,8 +71,8 @@ struct mrm_ctx {
   bool force_sequence = false;
   bool tvd_x_march = false;                      // dim-0 TVD always by the marching kernel
   size_t lds_max = 64 * 1024;                    // LDS bytes one block may use (device attribute)
-  int stage_variant = 3;                         // stage kernel: 3 two wave groups, 2 shared fluxes,
-                                                 // 1 register window, 0 re-read
+  int stage_variant = 4;                         // stage kernel: 4 three wave groups, 3 two, 2 shared
+                                                 // fluxes, 1 register window, 0 re-read
   bool fma = false;                              // fused step uses the contracted build (MRM_FMA=1)
   double last_dt = 0.0;                          // dt and TVD split of the last fused step
   bool last_split = false;
@@ -353,7 +353,7 @@ extern "C" int mrm_create(const mrm_grid_desc* grid, const mrm_phys_desc* phys, 
   const char* path = getenv("MRM_STEP_PATH");
   c->force_sequence = path && strcmp(path, "sequence") == 0;
   const char* zw = getenv("MRM_STAGE_ZWIN");
-  c->stage_variant = (zw && zw[0] >= '0' && zw[0] <= '3') ? zw[0] - '0' : 3;
+  c->stage_variant = (zw && zw[0] >= '0' && zw[0] <= '4') ? zw[0] - '0' : 4;
   const char* df = getenv("MRM_DEFER");
   c->defer = !(df && df[0] == '0');
   const char* fm = getenv("MRM_FMA");

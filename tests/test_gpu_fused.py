@@ -66,14 +66,30 @@ def compare(o, ro, m, r):
     assert r.time == pytest.approx(ro.time, rel=TOL)
 
 
-@pytest.mark.parametrize("zwin,fma", [("0", "0"), ("1", "0"), ("2", "0"), ("3", "0"), ("0", "1"), ("1", "1"), ("2", "1"),
-                                      ("3", "1")])
+@pytest.mark.parametrize("zwin,fma", [("0", "0"), ("1", "0"), ("2", "0"), ("3", "0"), ("4", "0"), ("0", "1"), ("1", "1"),
+                                      ("2", "1"), ("3", "1"), ("4", "1")])
 def test_fused_steps_match_oracle(table, zwin, fma):
     print(f"case: fused 40x36x28 zwin={zwin} fma={fma}", flush=True)
     o, ro, m, r = setup_pair(table, (40, 36, 28), seed=31, zwin=zwin, fma=fma)
     o.step(ro, 1e30, 3)
     m.step(r, 1e30, 3)
     compare(o, ro, m, r)
+
+
+@pytest.mark.parametrize("fma", ["0", "1"])
+def test_stage_wave_groups_bit_identical(table, fma):
+    """the three-group stage kernel (the EOS pressure in its own wave group)
+    and the two-group one perform the same operations: equal states and
+    pressures bit for bit after three steps"""
+    print(f"case: fused 40x36x28, stage kernel with three vs two wave groups, fma={fma}", flush=True)
+    out = []
+    for zwin in ("3", "4"):
+        o, ro, m, r = setup_pair(table, (40, 36, 28), seed=33, zwin=zwin, fma=fma)
+        m.step(r, 1e30, 3)
+        out.append((m.get_state(), m.get_field(abi.PRES), r.dt))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
 
 
 def test_fused_split_tvd(table):
