@@ -435,7 +435,10 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
           double* __restrict__ Uout, double* __restrict__ Pout, const double* __restrict__ Qtot) {
   constexpr int NFP = nfp_<ETA>();
   constexpr int NT = ST_SX * ST_SY;
-  constexpr int NA = ETA ? MRM_NV + 3 : MRM_NV;   // x + y sums, and J with ETA
+  // xy group -> z group: the x + y sums, the point's Alfven limiter factor
+  // (the same value the z group would form from the same primitives), and J
+  // with ETA
+  constexpr int NA = ETA ? MRM_NV + 4 : MRM_NV + 1;
   static_assert(4 * ST_SX + 4 * ST_SY <= NT, "one ring point per thread at most");
   __shared__ double fx[NFP][ST_SY][ST_LX];   // dim-0 fluxes of row ty, columns i0-2 .. i0+SX+1
   __shared__ double fy[NFP][ST_LY][ST_SX];   // dim-1 fluxes of column tx, rows j0-2 .. j0+SY+1
@@ -570,9 +573,10 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
         const int sl = m & 1;
 #pragma unroll
         for (int q = 0; q < MRM_NV; q++) axy[sl][q][t] = acc[q];
+        axy[sl][MRM_NV][t] = lfc;
         if (ETA) {
 #pragma unroll
-          for (int q = 0; q < 3; q++) axy[sl][(MRM_NV + q) % NA][t] = J[q];
+          for (int q = 0; q < 3; q++) axy[sl][(MRM_NV + 1 + q) % NA][t] = J[q];
         }
       }
       if (eos) {
@@ -625,7 +629,8 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
         // group's sums: form them while that group packs its fluxes (with
         // it, J passes through the three sweeps in order, so they wait)
         if (!ETA && valid) {
-          const double lfc = lfac_of(w[2], A.RP.inv_va2max);
+          // the xy group handed this plane's sums over in the previous iteration
+          const double lfc = axy[(n - 1) & 1][MRM_NV][t];
           sweep_residual_lr<2, false>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res, &A.RP, nullptr);
         }
       }
@@ -638,8 +643,8 @@ k_stage_w(StageArgs A, const double* __restrict__ Uin, const double* __restrict_
         double J[3] = {0.0, 0.0, 0.0};
         if (ETA) {
 #pragma unroll
-          for (int q = 0; q < 3; q++) J[q] = axy[sl][(MRM_NV + q) % NA][t];
-          const double lfc = lfac_of(w[2], A.RP.inv_va2max);
+          for (int q = 0; q < 3; q++) J[q] = axy[sl][(MRM_NV + 1 + q) % NA][t];
+          const double lfc = axy[sl][MRM_NV][t];
           sweep_residual_lr<2, ETA>(getz, w[2], A.RP.w1[2], A.RP.w2[2], lfc, res, &A.RP, J);
         }
 #pragma unroll
