@@ -468,6 +468,18 @@ def local_direction(U, inner, rng):
     return d
 
 
+def device_free_bytes():
+    """free device memory of the current GPU from the HIP runtime the library
+    runs on (hipMemGetInfo)"""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    rc = hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total))
+    if rc != 0:
+        raise RuntimeError(f"hipMemGetInfo failed ({rc})")
+    return free.value
+
+
 def adjoint_study(table, tab, steps=50, every=10, ndir=2, hs=(1e-9,), precision=None, ref_grad=None):
     """Adjoint of J = sum v_vertical^2 over the interior layer 64 of the 128^3
     box after `steps` steps, checkpointed every `every` steps: forward,
@@ -480,12 +492,11 @@ def adjoint_study(table, tab, steps=50, every=10, ndir=2, hs=(1e-9,), precision=
     context's (FP64 by default; in an FP32-mixed context the forward is the
     mixed step and the gradient is compared with ref_grad, the FP64
     context's).  Returns (summary, gradient)."""
-    import torch
     from muram_main_amd import abi, config, initial
     prec = abi.PREC_FP64 if precision is None else precision
     wl = config.workloads()["adjoint128"]
     U = initial.make_state(tab, wl.box)
-    free0, total = torch.cuda.mem_get_info()
+    free0 = device_free_bytes()
     m, r = make_grid(wl, table, U, prec)
     p_bc, s_bc = m.bc()
     m.step(r, config.DT_RAD_OFF, 1)
@@ -523,7 +534,7 @@ def adjoint_study(table, tab, steps=50, every=10, ndir=2, hs=(1e-9,), precision=
     t0 = time.perf_counter()
     grad = m.adjoint_backward(seed, fold_initial_ghosts=True)
     t_back = time.perf_counter() - t0
-    free1, _ = torch.cuda.mem_get_info()
+    free1 = device_free_bytes()
     rng = np.random.default_rng(11)
     # the forward is smooth only between limiter switches, which lie within
     # 1e-10 of many points (relative near ties): differences at 1e-6..1e-8
