@@ -54,11 +54,12 @@ KERNEL_BYTES = {
 STAGE_BYTES = {"fp64": (144.0, 208.0, 208.0, 200.0), "fp32": (80.0, 116.0, 116.0, 112.0)}
 KERNEL_BYTES_FUSED = dict(KERNEL_BYTES, tvd_sweep_z=(144.0, "interior"))   # + new p (epilogue)
 # drop-in sequence with the deferred stage: MHD_Residual forms ConsToPrim's
-# EOS fields (read rho, M, e 40; write p, T, ne, rhoi, amb 40 per buffer
-# cell) and Source_Integrate_Tcheck runs the stage kernel from the
-# conservative state (read U 64 + p 8 + U0 64, write U 64 = 200 B; at stage 1
-# U0 is U itself, 136 B)
-KERNEL_BYTES_DROPIN = dict(KERNEL_BYTES, cons_to_prim=(80.0, "buffer"))
+# pressure and the speeds for dt (read U 64, write p 8 per buffer cell; the
+# other EOS outputs wait for a reader) and Source_Integrate_Tcheck runs the
+# stage kernel from the conservative state (read U 64 + p 8 + U0 64, write
+# U 64 = 200 B; at stage 1 U0 is U itself, 136 B); TCheck runs as the
+# epilogue of TVDlimit's dim-2 sweep (136 B)
+KERNEL_BYTES_DROPIN = dict(KERNEL_BYTES, cons_to_prim=(72.0, "buffer"))
 STAGE_BYTES_DROPIN = (136.0, 200.0, 200.0, 200.0)
 # FP32-mixed form: a cell is rho, M, B in FP32 (28 B) + e in FP64 (8 B) = 36 B, p 4 B
 KERNEL_BYTES_MX = {

@@ -974,11 +974,14 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_p(Geom G, EosDev E, const dou
   }
 }
 
-// ConsToPrim's five EOS outputs (p, T, ne, rhoi, amb) of the conservative
-// state U written to F without converting U, and the characteristic speed of
-// the points MHD_Residual's sweeps visit (k_eos_p's set) into cmax_out: what
-// the host must see of a ConsToPrim + MHD_Residual pair before the stage
-// kernel does the rest (the drop-in path's deferred stage)
+// ConsToPrim's EOS outputs of the conservative state U written to F without
+// converting U (the drop-in path's deferred stage).  PRES: the pressure alone
+// and the characteristic speed of the points MHD_Residual's sweeps visit
+// (k_eos_p's set) into cmax_out -- what the stage kernel and the host's dt
+// need of a ConsToPrim + MHD_Residual pair; the pressure is ConsToPrim's, bit
+// for bit (the temperature only decides its validity).  !PRES: the other
+// four outputs (T, ne, rhoi, amb), formed when a caller reads them.
+template <bool PRES>
 __global__ void __launch_bounds__(MRM_BLOCK) k_eos_fields(Geom G, EosDev E, const double* __restrict__ U,
                                                           double* __restrict__ F, double inv_va2max,
                                                           unsigned long long* cmax_out) {
@@ -990,15 +993,22 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_fields(Geom G, EosDev E, cons
 #pragma unroll
     for (int q = 0; q < MRM_NV; q++) u[q] = U[q * bs + n];
     double m1 = u[1], m2 = u[2], m3 = u[3], ei = u[4], pr[5];
-    cons_to_prim_cell<5>(E, u[0], m1, m2, m3, ei, pr);
+    if (PRES) {
+      cons_to_prim_cell<2>(E, u[0], m1, m2, m3, ei, pr);
+      F[MRM_PRES * bs + n] = pr[0];
+      bool inner, vis;
+      cmax_cell_class(G, n, inner, vis);
+      if (vis) cm[0] = smax(cm[0], point_cmax_cons(u, pr[0], inv_va2max));
+    } else {
+      cons_to_prim_cell<5>(E, u[0], m1, m2, m3, ei, pr);
 #pragma unroll
-    for (int q = 0; q < 5; q++) F[q * bs + n] = pr[q];
-    bool inner, vis;
-    cmax_cell_class(G, n, inner, vis);
-    if (vis) cm[0] = smax(cm[0], point_cmax_cons(u, pr[0], inv_va2max));
+      for (int q = 1; q < 5; q++) F[q * bs + n] = pr[q];
+    }
   }
-  block_reduce<OpMax, 1>(cm, lds);
-  if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
+  if (PRES) {
+    block_reduce<OpMax, 1>(cm, lds);
+    if (threadIdx.x == 0) atomic_max_nonneg(cmax_out, cm[0]);
+  }
 }
 
 // every ghost cell (outside the interior) of the conservative state src

@@ -663,7 +663,8 @@ __device__ __forceinline__ unsigned long long br_key(int d, long bs, long node) 
   return ((unsigned long long)d * (unsigned long long)bs + (unsigned long long)node) << 3;
 }
 
-template <int D, bool HAS_TAU, bool EPI, bool VALF = false, bool DIAG = false, bool CNT = false>
+// TCK: TCheck alone after each update (X's limits; no pressure, no cmax)
+template <int D, bool HAS_TAU, bool EPI, bool VALF = false, bool DIAG = false, bool CNT = false, bool TCK = false>
 __global__ void __launch_bounds__(MRM_BLOCK) __attribute__((amdgpu_waves_per_eu(VALF ? 2 : 1)))
 k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restrict__ Pr, TvdEpilogue X,
             DiagArgs DA) {
@@ -766,18 +767,23 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
             DA.dg[DG_T7 * bs + nd] += (uu[4] - dg[3]) * DA.idt;
             DA.dg[DG_QVIS * bs + nd] += DA.dt_fac * (0.5 * (qv + qvm) * iq);
           }
+          if (TCK) tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
           if (EPI) {
             tcheck_vals(uu, X.rho_min, X.eps_min, X.eps_max, X.vmax);
-            const double pn = ghost_pres_(X.E, uu);
-            if (VALF) {
-              // v and e_int as ConsToPrim forms them (cons_to_prim.C:110-118)
-              const double dn = 1.0 / uu[0];
-              const double v1 = uu[1] * dn, v2 = uu[2] * dn, v3 = uu[3] * dn;
-              const double ei = uu[4] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3) * uu[0];
-              valf_lds_cell(vacc + threadIdx.x, MRM_BLOCK, uu[0], v1, v2, v3, ei, pn, X.vlim95, X.elim95);
+            // without X.Pnew the epilogue is TCheck alone (the drop-in
+            // TVDlimit's last sweep carried out by TCheck)
+            if (X.Pnew) {
+              const double pn = ghost_pres_(X.E, uu);
+              if (VALF) {
+                // v and e_int as ConsToPrim forms them (cons_to_prim.C:110-118)
+                const double dn = 1.0 / uu[0];
+                const double v1 = uu[1] * dn, v2 = uu[2] * dn, v3 = uu[3] * dn;
+                const double ei = uu[4] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3) * uu[0];
+                valf_lds_cell(vacc + threadIdx.x, MRM_BLOCK, uu[0], v1, v2, v3, ei, pn, X.vlim95, X.elim95);
+              }
+              X.Pnew[nd] = pn;
+              if (!(X.skip_top_cmax && lineX == G.le0)) cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
             }
-            X.Pnew[nd] = pn;
-            if (!(X.skip_top_cmax && lineX == G.le0)) cm[0] = smax(cm[0], point_cmax_cons(uu, pn, X.inv_va2max));
           }
 #pragma unroll
           for (int k = 0; k < MRM_NV; k++) U[k * bs + nd] = uu[k];
@@ -793,7 +799,7 @@ k_tvd_sweep(Geom G, TvdParams P, double* __restrict__ U, const double* __restric
   if (CNT) tvd_cnt_flush(cnt, P.bcount);
   if (EPI) {
     block_reduce<OpMax, 1>(cm, lds);
-    if (threadIdx.x == 0) atomic_max_nonneg(X.cmax, cm[0]);
+    if (threadIdx.x == 0 && X.Pnew) atomic_max_nonneg(X.cmax, cm[0]);
     if (VALF) {
       __syncthreads();
       ValfAcc va = valf_lds_load(vacc + threadIdx.x, MRM_BLOCK);

@@ -93,6 +93,12 @@ inline hipError_t launch_tvd_sweep(hipStream_t st, int d, bool tau, const TvdEpi
                                    const Geom& G, const TvdParams& P, double* U, const double* Pr) {
   if (P.bcount) return launch_tvd_sweep_cnt(st, d, tau, epi, nb, G, P, U, Pr);
   TvdEpilogue X{};
+  if (d == 2 && epi && !epi->Pnew) {
+    // TCheck alone as the epilogue
+    if (tau) k_tvd_sweep<2, true, false, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, DiagArgs{});
+    else k_tvd_sweep<2, false, false, false, false, false, true><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, *epi, DiagArgs{});
+    return hipGetLastError();
+  }
   if (d == 0) {
     if (tau) k_tvd_sweep<0, true, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});
     else k_tvd_sweep<0, false, false><<<nb, MRM_BLOCK, 0, st>>>(G, P, U, Pr, X, DiagArgs{});

@@ -60,6 +60,15 @@ struct mrm_ctx {
   // Source_Integrate_Tcheck then runs), a SaveCons recorded (U0 = U, no copy)
   bool defer = true;
   bool c2p_def = false, res_def = false, u0_alias = false;
+  // the state buffer (index into S) whose ConsToPrim outputs T, ne, rhoi and
+  // amb are still to be formed (a recorded MHD_Residual forms the pressure
+  // alone); -1: the fields hold the last ConsToPrim's outputs
+  int eosf_pend = -1;
+  // a drop-in TVDlimit's dim-2 sweep, recorded: carried out with TCheck as
+  // its epilogue when TCheck is the next call, plain before any other
+  bool tvd2_def = false;
+  double tvd2_dt = 0.0;
+  bool tvd2_periodic = false;
   double def_inv_va2max = 0.0;                   // inv_va2max of the recorded MHD_Residual
   double* dR = nullptr;
   double* dP2 = nullptr;                         // second pressure buffer of the fused step
@@ -425,6 +434,7 @@ extern "C" int mrm_synchronize(mrm_ctx* c) {
 // forward tables transposed to [i_eps][i_rho] and divided by del_eps*del_lr in
 // double before rounding to float (:170-175); inverse tables likewise (:195-196).
 static int drop_settle(mrm_ctx* c);
+static int drop_settle_k(mrm_ctx* c, const double* w);
 
 extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   if (!c || !bytes) return MRM_ERR_ARG;
@@ -507,7 +517,7 @@ static int drop_settle(mrm_ctx* c);
 // device: rho, M, B rounded to FP32, e kept in FP64.
 extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layout) {
   if (!c || !host) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, state_ptr(c, which))) return rc;
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_set_state: FP32-mixed contexts hold U only");
   double* d = mx ? c->dR : state_ptr(c, which);
@@ -533,7 +543,7 @@ extern "C" int mrm_set_state(mrm_ctx* c, int which, const double* host, int layo
 
 extern "C" int mrm_get_state(mrm_ctx* c, int which, double* host, int layout) {
   if (!c || !host) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, nullptr)) return rc;
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   if (mx && which != MRM_U) return fail(c, MRM_ERR_UNSUPPORTED, "mrm_get_state: FP32-mixed contexts hold U only");
   double* d = mx ? c->dR : state_ptr(c, which);
@@ -563,7 +573,7 @@ static int diag_ensure(mrm_ctx* c);
 
 extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
   if (!c || !host || f < 0 || f >= MRM_TVAR1 + DG_N) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = (f >= MRM_TEMP && f <= MRM_AMB) ? drop_settle(c) : drop_settle_k(c, nullptr)) return rc;
   const size_t bs = c->G.bs;
   if (f >= MRM_TVAR1) {
     int rc = diag_ensure(c);
@@ -583,7 +593,7 @@ extern "C" int mrm_set_field(mrm_ctx* c, int f, const double* host) {
 
 extern "C" int mrm_get_field(mrm_ctx* c, int f, double* host) {
   if (!c || !host || f < 0 || f >= MRM_TVAR1 + DG_N) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = (f >= MRM_TEMP && f <= MRM_AMB) ? drop_settle(c) : drop_settle_k(c, nullptr)) return rc;
   const size_t bs = c->G.bs;
   if (f >= MRM_TVAR1) {
     // diagnostics never collected read as zero
@@ -614,8 +624,12 @@ extern "C" int mrm_get_limits(const mrm_ctx* c, double lim[4]) {
   lim[0] = c->inv_va2max; lim[1] = c->v_lim; lim[2] = c->e_lim; lim[3] = c->va_max_old;
   return MRM_OK;
 }
+static int tvd2_settle(mrm_ctx* c);
+
 extern "C" int mrm_set_limits(mrm_ctx* c, const double lim[4]) {
   if (!c || !lim) return MRM_ERR_ARG;
+  // a recorded TVD sweep runs with the limits it was recorded under
+  if (int rc = tvd2_settle(c)) return rc;
   c->inv_va2max = lim[0]; c->v_lim = lim[1]; c->e_lim = lim[2]; c->va_max_old = lim[3];
   c->fast_valid = false;
   return MRM_OK;
@@ -643,7 +657,7 @@ extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
   // recorded: the next MHD_Residual and Source_Integrate_Tcheck read the
   // conservative state themselves, anything else runs it first
   if (c->c2p_def || c->res_def) {
-    int rc = drop_settle(c);
+    int rc = drop_settle_k(c, nullptr);
     if (rc) return rc;
   }
   if (c->defer) {
@@ -673,7 +687,7 @@ extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double s
     valf_constants(c);
     return MRM_OK;
   }
-  if ((rc = drop_settle(c))) return rc;
+  if ((rc = drop_settle_k(c, nullptr))) return rc;
   // slots RED_CMAX..RED_CNT_E only: a fused step's Clean_divB maxima may
   // still wait in RED_DIVB0/1 for their host update
   HIPCHK(c, hipMemsetAsync(c->d_red, 0, (RED_CNT_E + 1) * sizeof(unsigned long long), c->stream));
@@ -923,6 +937,8 @@ static int c2p_launch(mrm_ctx* c) {
   TScope ts(c, T_C2P);
   k_cons_to_prim<5><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF);
   LAUNCHCHK(c, "k_cons_to_prim");
+  // all five outputs of the newest ConsToPrim: nothing stays pending
+  c->eosf_pend = -1;
   return MRM_OK;
 }
 
@@ -930,8 +946,48 @@ static int c2p_launch(mrm_ctx* c) {
 // called: SaveCons's copy, ConsToPrim, MHD_Residual's residual.  Every entry
 // point that reads or changes the state outside the deferred stage calls
 // this first.
+// the pending ConsToPrim outputs T, ne, rhoi, amb formed from their state
+static int eosf_settle(mrm_ctx* c) {
+  if (c->eosf_pend < 0) return MRM_OK;
+  const double* U = c->S[c->eosf_pend];
+  c->eosf_pend = -1;
+  unsigned nb = nblk(c->G.bs);
+  if (nb > 65536u) nb = 65536u;
+  TScope ts(c, T_C2P);
+  k_eos_fields<false><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, U, c->dF, 0.0, c->d_red + RED_SCRATCH);
+  LAUNCHCHK(c, "k_eos_fields");
+  return MRM_OK;
+}
+
+// drop_settle for an operation that reads no pending EOS field and writes
+// only the state buffer w (or none): the pending fields stay pending unless
+// their state is w
+static int drop_settle_k(mrm_ctx* c, const double* w);
+
 static int drop_settle(mrm_ctx* c) {
+  if (int rc = eosf_settle(c)) return rc;
+  return drop_settle_k(c, nullptr);
+}
+
+static int tvd_buf(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, const TvdEpilogue* epi,
+                   bool contracted, int only_d, bool y_interior);
+
+// the recorded dim-2 sweep of a drop-in TVDlimit carried out plain
+static int tvd2_settle(mrm_ctx* c) {
+  if (!c->tvd2_def) return MRM_OK;
+  c->tvd2_def = false;
+  return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, c->tvd2_dt, nullptr, false, 2, c->tvd2_periodic);
+}
+
+static int drop_settle_k(mrm_ctx* c, const double* w) {
   int rc;
+  if ((rc = tvd2_settle(c))) return rc;
+  if (c->eosf_pend >= 0) {
+    // the pending state is about to change: the operation writes it, or
+    // SaveCons's recorded copy goes into it
+    const double* P = c->S[c->eosf_pend];
+    if (((w && w == P) || (c->u0_alias && c->dU0 == P)) && (rc = eosf_settle(c))) return rc;
+  }
   if (c->u0_alias) {
     TScope ts(c, T_SAVECONS);
     k_save_cons<<<nblk(ninterior(c)), MRM_BLOCK, 0, c->stream>>>(c->G, c->dU, c->dU0);
@@ -972,9 +1028,12 @@ extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_
     if (nb > 65536u) nb = 65536u;
     {
       TScope ts(c, T_C2P);
-      k_eos_fields<<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF, c->inv_va2max, c->d_red + RED_CMAX);
+      k_eos_fields<true><<<nb, MRM_BLOCK, 0, c->stream>>>(c->G, c->E, c->dU, c->dF, c->inv_va2max,
+                                                          c->d_red + RED_CMAX);
     }
     LAUNCHCHK(c, "k_eos_fields");
+    // T, ne, rhoi, amb of this state are formed when a caller reads them
+    c->eosf_pend = c->cur;
     c->res_def = true;
     c->def_inv_va2max = c->inv_va2max;
     if (!dt_cfl) return MRM_OK;
@@ -983,7 +1042,7 @@ extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_
     *dt_cfl = eta_dt_limit(c, run, c->dxmin / cmax);   // mhdres_SR.C:795-819
     return MRM_OK;
   }
-  if ((rc = drop_settle(c))) return rc;
+  if ((rc = drop_settle_k(c, nullptr))) return rc;
   rc = launch_residual(c, run);
   if (rc || !dt_cfl) return rc;
   double cmax;
@@ -1183,6 +1242,9 @@ static int srcint_deferred(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   int rc;
   if (stage == 1 && (rc = damping_update(c, c->dU, false))) return rc;
   const int spare = 3 - c->cur - c->c0;
+  // the stage kernel writes the spare buffer (the pending EOS fields' state
+  // is the input, or an older state superseded by it)
+  if (c->eosf_pend == spare && (rc = eosf_settle(c))) return rc;
   double* Uin = c->dU;
   double* Uout = c->S[spare];
   const double* U0 = c->u0_alias ? c->dU : c->dU0;
@@ -1210,7 +1272,7 @@ extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, 
   NEED_FP64(c, "mrm_source_integrate_tcheck");
   c->fast_valid = false;
   if (c->res_def && !run->need_diagnostics) return srcint_deferred(c, run, stage);
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, c->dU)) return rc;
   // the damping profile of the step, from the primitive state of stage 1
   if (stage == 1) {
     int rc = damping_update(c, c->dU, true);
@@ -1245,7 +1307,7 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_save_cons");
   c->fast_valid = false;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, c->defer ? nullptr : c->dU0)) return rc;
   if (c->defer) {
     c->u0_alias = true;   // copied only if U changes in place before a deferred stage
   } else {
@@ -1262,11 +1324,29 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
 
 extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
   if (!c) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
   NEED_FP64(c, "mrm_tcheck");
-  c->fast_valid = false;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
+  if (c->tvd2_def && !(run && run->need_diagnostics)) {
+    // the recorded dim-2 sweep with TCheck as its epilogue: the clamp of
+    // Add_Sources_Integrate_SR.C:447-465 applied to each cell right after
+    // its last TVD update (every interior cell is updated by that sweep)
+    c->tvd2_def = false;
+    if (int rc = drop_settle_k(c, c->dU)) return rc;
+    c->fast_valid = false;
+    TvdEpilogue X{};
+    X.E = c->E;
+    X.rho_min = c->phys.tchk[1];
+    X.eps_min = c->phys.tchk[0];
+    X.eps_max = eps_max;
+    X.vmax = vmax;
+    X.Pnew = nullptr;
+    X.cmax = nullptr;
+    X.valf = nullptr;
+    return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, c->tvd2_dt, &X, false, 2, c->tvd2_periodic);
+  }
+  if (int rc = drop_settle_k(c, c->dU)) return rc;
+  c->fast_valid = false;
   DiagArgs DA;
   int rc = diag_args(c, run, DA);
   if (rc) return rc;
@@ -1301,7 +1381,7 @@ static int exchange_arrays(mrm_ctx* c, double* base, int na);
 
 extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, c->dU)) return rc;
   c->fast_valid = false;
   if (c->precision == MRM_PREC_FP32_MIXED) return exchange_mx(c, c->M[c->cur], nullptr);
   int rc = exchange_buf(c, c->dU, nullptr);
@@ -1435,7 +1515,7 @@ static int clean_divb_mx(mrm_ctx* c, MxState M, const MxEpilogue* epi) {
 
 extern "C" int mrm_clean_divB(mrm_ctx* c, const mrm_run_desc* run, double err[2]) {
   if (!c) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, c->dU)) return rc;
   NEED_FP64(c, "mrm_clean_divB");
   if (c->phys.divB[1] < 1.0) return fail(c, MRM_ERR_ARG, "mrm_clean_divB: divB_itmax < 1");
   c->fast_valid = false;
@@ -1596,7 +1676,7 @@ extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, 
 extern "C" int mrm_set_boundary_conditions_pt(mrm_ctx* c, const mrm_run_desc* run, int stage, int pt_update) {
   (void)stage;
   if (!c) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, c->dU)) return rc;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
   if (int rc = diag_track(c, run)) return rc;
@@ -1838,6 +1918,7 @@ static int tvd_buf_diag(mrm_ctx* c, double* U, const double* Pr, double dt_tvd, 
 // sweeps are not counted)
 extern "C" int mrm_branch_count(mrm_ctx* c, int on, double tie) {
   if (!c || !(tie >= 0.0)) return MRM_ERR_ARG;
+  if (int rc = drop_settle_k(c, nullptr)) return rc;   // a recorded sweep counts under the old setting
   c->btie = tie;
   if (on) {
     NEED_FP64(c, "mrm_branch_count");
@@ -1851,6 +1932,7 @@ extern "C" int mrm_branch_count(mrm_ctx* c, int on, double tie) {
 
 extern "C" int mrm_get_branch_counts(mrm_ctx* c, unsigned long long out[MRM_BR_NSLOT]) {
   if (!c || !out) return MRM_ERR_ARG;
+  if (int rc = drop_settle_k(c, nullptr)) return rc;
   if (!c->d_bcount) {
     memset(out, 0, MRM_BR_NSLOT * sizeof(unsigned long long));
     return MRM_OK;
@@ -1863,7 +1945,7 @@ extern "C" int mrm_get_branch_counts(mrm_ctx* c, unsigned long long out[MRM_BR_N
 
 extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) {
   if (!c || !run) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  if (int rc = drop_settle_k(c, c->dU)) return rc;
   NEED_FP64(c, "mrm_tvdlimit");
   c->fast_valid = false;
   DiagArgs DA;
@@ -1873,7 +1955,14 @@ extern "C" int mrm_tvdlimit(mrm_ctx* c, const mrm_run_desc* run, double dt_tvd) 
   const double* Pr = c->dF + MRM_PRES * c->G.bs;
   bool periodic = false;
   if (!c->tau_set && (rc = ghosts_periodic(c, c->dU, Pr, &periodic))) return rc;
-  return tvd_buf(c, c->dU, Pr, dt_tvd, nullptr, false, -1, periodic);
+  if (!c->defer) return tvd_buf(c, c->dU, Pr, dt_tvd, nullptr, false, -1, periodic);
+  // the dim-0 and dim-1 sweeps now, the dim-2 sweep recorded
+  if ((rc = tvd_buf(c, c->dU, Pr, dt_tvd, nullptr, false, 0, periodic))) return rc;
+  if ((rc = tvd_buf(c, c->dU, Pr, dt_tvd, nullptr, false, 1, periodic))) return rc;
+  c->tvd2_def = true;
+  c->tvd2_dt = dt_tvd;
+  c->tvd2_periodic = periodic;
+  return MRM_OK;
 }
 
 // ------------------------------------------------------------------ full step

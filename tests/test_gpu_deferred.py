@@ -11,7 +11,11 @@ operation as it is called).
   and after exchange and boundary, within 1e-12 of the reference's;
 * reads in the middle: U after a recorded ConsToPrim is the primitive state,
   the EOS fields and Res after a recorded MHD_Residual are ConsToPrim's and
-  the residual, U0 after a recorded SaveCons is U -- each bit for bit the
+  the residual, U0 after a recorded SaveCons is U; ConsToPrim's five EOS
+  outputs read after the deferred stage, after exchange and boundary, and
+  after stage 2 (a recorded MHD_Residual forms the pressure alone, the
+  other four are formed from the recorded state when read), and a
+  temperature written by the caller in between -- each bit for bit the
   calls with MRM_DEFER=0;
 * three whole steps on the drop-in sequence (mrm_step with MRM_STEP_PATH=
   sequence), deferred and not, plain, with va_adjust 1 (whose stage 1 runs
@@ -107,12 +111,25 @@ def _ops(table, gold, defer, read):
         got["f"] = np.stack([m.get_field(f) for f in range(5)])
     m.Sync_timestep(r, 1e30, dt)
     m.Source_Integrate_Tcheck(r, 1)
+    if read == "fields_srcint":
+        got["f"] = np.stack([m.get_field(f) for f in range(5)])
     m.exchange_grid()
     m.SetBoundaryConditions(r, 1)
+    if read == "fields_bc":
+        got["f"] = np.stack([m.get_field(f) for f in range(5)])
+    if read in ("fields_stage2", "temp_set"):
+        # stage 2 up to Source_Integrate_Tcheck, then the fields
+        m.ConsToPrim()
+        m.Adjust_Valf_Max(r)
+        m.MHD_Residual(r)
+        m.Source_Integrate_Tcheck(r, 2)
+        if read == "temp_set":
+            m.set_field(abi.TEMP, np.full(m.get_field(abi.PRES).shape, 1234.5))
+        got["f"] = np.stack([m.get_field(f) for f in range(5)])
     return got, m.get_state(), m.get_state(abi.U0), r.dt
 
 
-@pytest.mark.parametrize("read", ["u0", "prim", "res"])
+@pytest.mark.parametrize("read", ["u0", "prim", "res", "fields_srcint", "fields_bc", "fields_stage2", "temp_set"])
 def test_reads_carry_out_recorded_operations(gold, table, read):
     print(f"case: read {read} in the middle of a deferred stage", flush=True)
     g1, U1, U01, dt1 = _ops(table, gold, "1", read)
