@@ -15,8 +15,9 @@ operation as it is called).
   outputs read after the deferred stage, after exchange and boundary, and
   after stage 2 (a recorded MHD_Residual forms the pressure alone, the
   other four are formed from the recorded state when read), and a
-  temperature written by the caller in between -- each bit for bit the
-  calls with MRM_DEFER=0;
+  temperature written by the caller in between; TVDlimit's recorded dim-2
+  sweep run by TCheck, or plain before a read or a change of the limits,
+  whole and split -- each bit for bit the calls with MRM_DEFER=0;
 * three whole steps on the drop-in sequence (mrm_step with MRM_STEP_PATH=
   sequence), deferred and not, plain, with va_adjust 1 (whose stage 1 runs
   ConsToPrim for Adjust_Valf_Max's statistics) and with resistivity, against
@@ -140,6 +141,51 @@ def test_reads_carry_out_recorded_operations(gold, table, read):
     # the read made the stage run as called: the end state is bit for bit
     assert np.array_equal(U1, U0)
     assert np.array_equal(U01, U00)
+
+
+def _tvd_ops(table, gold, defer, read, split):
+    """a whole stage by the drop-in calls, then TVDlimit (whole, or two
+    halves with an exchange between) and TCheck, with one read at the point
+    `read`; returns what was read and the state at the end"""
+    m = make(table, defer)
+    r = start(m, gold["base_in"])
+    m.SaveCons()
+    m.ConsToPrim()
+    m.Adjust_Valf_Max(r)
+    dt = m.MHD_Residual(r)
+    m.Sync_timestep(r, 1e30, dt)
+    m.Source_Integrate_Tcheck(r, 1)
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 1)
+    got = {}
+    if split:
+        m.TVDlimit(r, 0.5 * r.dt)
+        m.exchange_grid()
+        m.TVDlimit(r, 0.5 * r.dt)
+    else:
+        m.TVDlimit(r, r.dt)
+    if read == "u":
+        got["u"] = m.get_state()
+    if read == "limits":
+        m.set_limits(m.limits())
+    m.TCheck(r)
+    m.exchange_grid()
+    m.SetBoundaryConditions(r, 5)
+    return got, m.get_state()
+
+
+@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("read", ["none", "u", "limits"])
+def test_recorded_tvd_sweep(gold, table, read, split):
+    """TVDlimit's dim-2 sweep recorded: run by the following TCheck with the
+    clamp as its epilogue, or plain before a read or a change of the limits;
+    bit for bit the calls with MRM_DEFER=0"""
+    print(f"case: recorded dim-2 TVD sweep, read {read}, split {split}", flush=True)
+    g1, U1 = _tvd_ops(table, gold, "1", read, split)
+    g0, U0 = _tvd_ops(table, gold, "0", read, split)
+    for k in g0:
+        assert np.array_equal(g1[k], g0[k]), k
+    assert np.array_equal(U1, U0)
 
 
 @pytest.mark.parametrize("defer", ["1", "0"])
