@@ -2421,7 +2421,10 @@ static int step_mixed_body(mrm_ctx* c, const mrm_run_desc* run, double dt_prev, 
 
 extern "C" int mrm_step(mrm_ctx* c, mrm_run_desc* run, double dt_rad, int nsteps) {
   if (!c || !run || nsteps < 0) return MRM_ERR_ARG;
-  if (int rc = drop_settle(c)) return rc;
+  // the fused step rotates every state buffer; the per-operation sequence's
+  // calls settle what they touch themselves (ConsToPrim's pending outputs
+  // stay pending, as between the reference's own calls)
+  if (int rc = c->force_sequence ? drop_settle_k(c, nullptr) : drop_settle(c)) return rc;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_step: no EOS table loaded");
   const bool mx = c->precision == MRM_PREC_FP32_MIXED;
   const int want_diag = run->need_diagnostics;
