@@ -21,7 +21,8 @@ operation as it is called).
 * three whole steps on the drop-in sequence (mrm_step with MRM_STEP_PATH=
   sequence), deferred and not, plain, with va_adjust 1 (whose stage 1 runs
   ConsToPrim for Adjust_Valf_Max's statistics) and with resistivity, against
-  the oracle: decisive limiter choices equal, fields within 1e-12.
+  the oracle: decisive limiter choices equal, fields within 1e-12, and the
+  last ConsToPrim's five EOS outputs within 1e-12.
 """
 import os
 
@@ -205,3 +206,9 @@ def test_sequence_steps_match_oracle(table, case, defer):
         m.step(r, config.DT_RAD_OFF, 1)
         print(f"step {s + 1}: dt gpu {r.dt:.15e} oracle {ro.dt:.15e}", flush=True)
     check_pair(m, r, o, ro)
+    # the last ConsToPrim's outputs (stage 4's input), formed on this read
+    # when the stage was deferred
+    for f in (abi.PRES, abi.TEMP, abi.NE, abi.RHOI, abi.AMB):
+        e = relerr(m.get_field(f), o.field(f))
+        print("EOS field", f, "rel err", e, flush=True)
+        assert e <= TOL, f
