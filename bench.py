@@ -740,7 +740,9 @@ def main():
                    "path": args.path, "va_adjust": wl.phys.params[config.I_PARAM_VA_ADJUST],
                    "divb": bool(wl.phys.divB[0] > 0), "eta": wl.phys.params[config.I_PARAM_ETA],
                    "spitzer": wl.phys.params[config.I_PARAM_SPITZER],
-                   "stage_kernel": os.environ.get("MRM_STAGE_ZWIN", "2"),
+                   "stage_kernel": {"4": "three wave groups (k_stage_w, EOS group)", "3": "two wave groups (k_stage_w)",
+                                    "2": "one group, shared fluxes (k_stage_f)", "1": "register window (k_stage)",
+                                    "0": "re-read (k_stage)"}.get(os.environ.get("MRM_STAGE_ZWIN", "4"), "?"),
                    "fma": os.environ.get("MRM_FMA", "0")},
         "hbm_frac_model": R["hbm_frac_model"],
         "roofline": R["roofline"],
