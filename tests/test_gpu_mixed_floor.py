@@ -13,8 +13,8 @@ field over the interior's lower half (rho >~ 1e-11 g/cm^3) and upper half.
 Bar: the mixed form's largest per-field error within 2 times the floor's,
 in each half.  Measured: lower half 1.21e-5 against 1.02e-5 (1.2x), upper
 half 1.08e-2 against 7.1e-3 (1.5x).  With the FP32 library logarithm in the
-TVD limiter's ln rho and ln eps (a few ulps of error, biased low by 0.6 ulp
-on average) the lower half was 2.8e-5 (2.75x: M1).
+TVD limiter's ln rho and ln eps (up to 2.3 ulps of error, biased low by 0.35
+to 0.65 ulp on average) the lower half was 2.8e-5 (2.75x: M1).
 """
 import numpy as np
 import pytest
