@@ -1018,12 +1018,14 @@ __global__ void __launch_bounds__(MRM_BLOCK) k_eos_fields(Geom G, EosDev E, cons
 // leaves.  The ghost width is 2 in every dimension.  Index space: the dim-0
 // ghost layers over all (j, k), then the dim-1 ghost rows over all (i, k),
 // then the dim-2 ghost planes over all (i, j); edges and corners are written
-// more than once, with the same values.
+// more than once, with the same values.  DIM0: the dim-0 ghost layers alone.
+template <bool DIM0>
 __global__ void __launch_bounds__(MRM_BLOCK) k_ghosts_prim(Geom G, const double* __restrict__ src,
                                                            double* __restrict__ dst) {
   const long na = 4L * G.n1 * G.n2, nb = 4L * G.n0 * G.n2, nc = 4L * G.n0 * G.n1;
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
+  if (DIM0 && t >= na) return;
   if (t < na) {
     const int l = (int)(t % 4);
     const long r = t / 4;

@@ -69,6 +69,14 @@ struct mrm_ctx {
   bool tvd2_def = false;
   double tvd2_dt = 0.0;
   bool tvd2_periodic = false;
+  // the ghost cells of a deferred Source_Integrate_Tcheck, recorded: those of
+  // gp_src in ConsToPrim's primitive form, into gp_dst.  1: every ghost
+  // cell; 2: the dim-0 ghost layers alone (an exchange has since written
+  // every ghost cell at interior i).  SetBoundaryConditions after an
+  // exchange writes the rest itself
+  int gp_pend = 0;
+  const double* gp_src = nullptr;
+  double* gp_dst = nullptr;
   double def_inv_va2max = 0.0;                   // inv_va2max of the recorded MHD_Residual
   double* dR = nullptr;
   double* dP2 = nullptr;                         // second pressure buffer of the fused step
@@ -434,7 +442,8 @@ extern "C" int mrm_synchronize(mrm_ctx* c) {
 // forward tables transposed to [i_eps][i_rho] and divided by del_eps*del_lr in
 // double before rounding to float (:170-175); inverse tables likewise (:195-196).
 static int drop_settle(mrm_ctx* c);
-static int drop_settle_k(mrm_ctx* c, const double* w);
+static int drop_settle_k(mrm_ctx* c, const double* w, bool gp_keep = false);
+static int gp_settle(mrm_ctx* c);
 
 extern "C" int mrm_eos_load(mrm_ctx* c, const void* bytes, size_t nbytes) {
   if (!c || !bytes) return MRM_ERR_ARG;
@@ -652,6 +661,7 @@ static int c2p_launch(mrm_ctx* c);
 extern "C" int mrm_cons_to_prim(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_cons_to_prim");
+  if (int rc = gp_settle(c)) return rc;
   c->fast_valid = false;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_cons_to_prim: no EOS table loaded");
   // recorded: the next MHD_Residual and Source_Integrate_Tcheck read the
@@ -678,6 +688,7 @@ static void valf_constants(mrm_ctx* c);
 extern "C" int mrm_adjust_valf_max(mrm_ctx* c, const mrm_run_desc* run, double stats[5]) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_adjust_valf_max");
+  if (int rc = gp_settle(c)) return rc;
   c->fast_valid = false;
   int rc = diag_track(c, run);
   if (rc) return rc;
@@ -961,8 +972,25 @@ static int eosf_settle(mrm_ctx* c) {
 
 // drop_settle for an operation that reads no pending EOS field and writes
 // only the state buffer w (or none): the pending fields stay pending unless
-// their state is w
-static int drop_settle_k(mrm_ctx* c, const double* w);
+// their state is w.  gp_keep: the caller takes recorded ghost cells over
+// (exchange, SetBoundaryConditions), unless another record is pending
+static int drop_settle_k(mrm_ctx* c, const double* w, bool gp_keep);
+
+// the recorded ghost cells of a deferred Source_Integrate_Tcheck written
+static int gp_settle(mrm_ctx* c) {
+  if (!c->gp_pend) return MRM_OK;
+  const Geom& G = c->G;
+  const long na = 4L * G.n1 * G.n2;
+  if (c->gp_pend == 2) {
+    k_ghosts_prim<true><<<nblk(na), MRM_BLOCK, 0, c->stream>>>(G, c->gp_src, c->gp_dst);
+  } else {
+    const long ng = na + 4L * ((long)G.n0 * G.n2 + (long)G.n0 * G.n1);
+    k_ghosts_prim<false><<<nblk(ng), MRM_BLOCK, 0, c->stream>>>(G, c->gp_src, c->gp_dst);
+  }
+  c->gp_pend = 0;
+  LAUNCHCHK(c, "k_ghosts_prim");
+  return MRM_OK;
+}
 
 static int drop_settle(mrm_ctx* c) {
   if (int rc = eosf_settle(c)) return rc;
@@ -979,8 +1007,11 @@ static int tvd2_settle(mrm_ctx* c) {
   return tvd_buf(c, c->dU, c->dF + MRM_PRES * c->G.bs, c->tvd2_dt, nullptr, false, 2, c->tvd2_periodic);
 }
 
-static int drop_settle_k(mrm_ctx* c, const double* w) {
+static int drop_settle_k(mrm_ctx* c, const double* w, bool gp_keep) {
   int rc;
+  if (c->gp_pend && !(gp_keep && !c->c2p_def && !c->res_def && !c->u0_alias && !c->tvd2_def) &&
+      (rc = gp_settle(c)))
+    return rc;
   if ((rc = tvd2_settle(c))) return rc;
   if (c->eosf_pend >= 0) {
     // the pending state is about to change: the operation writes it, or
@@ -1019,6 +1050,7 @@ extern "C" int mrm_mhd_residual(mrm_ctx* c, const mrm_run_desc* run, double* dt_
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_mhd_residual");
   int rc;
+  if ((rc = gp_settle(c))) return rc;
   if (res_deferrable(c, run)) {
     // the EOS outputs of the recorded ConsToPrim and cmax; the residual is
     // formed by the stage kernel of the next Source_Integrate_Tcheck
@@ -1235,9 +1267,11 @@ static int stage_launch(mrm_ctx* c, const mrm_run_desc* run, int stage, const do
 // Source_Integrate_Tcheck after a recorded ConsToPrim and MHD_Residual: the
 // stage kernel forms the residual from the conservative state and the
 // recorded EOS pressure and applies sources, Boris correction, RK update and
-// Tcheck, into the spare state buffer, whose ghost cells then take the
+// Tcheck, into the spare state buffer, whose ghost cells are to take the
 // input's in ConsToPrim's primitive form (what ConsToPrim and the in-place
-// update leave there); the buffers change roles, and Res := 0 is recorded
+// update leave there: recorded, since the exchange and SetBoundaryConditions
+// that follow in the stage loop overwrite every one of them); the buffers
+// change roles, and Res := 0 is recorded
 static int srcint_deferred(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   int rc;
   if (stage == 1 && (rc = damping_update(c, c->dU, false))) return rc;
@@ -1251,9 +1285,9 @@ static int srcint_deferred(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   if ((rc = stage_launch(c, run, stage, Uin, c->dF + MRM_PRES * c->G.bs, U0, Uout, nullptr, nullptr,
                          c->def_inv_va2max)))
     return rc;
-  const long ng = 4L * ((long)c->G.n1 * c->G.n2 + (long)c->G.n0 * c->G.n2 + (long)c->G.n0 * c->G.n1);
-  k_ghosts_prim<<<nblk(ng), MRM_BLOCK, 0, c->stream>>>(c->G, Uin, Uout);
-  LAUNCHCHK(c, "k_ghosts_prim");
+  c->gp_pend = 1;
+  c->gp_src = Uin;
+  c->gp_dst = Uout;
   if (c->u0_alias) {
     c->c0 = c->cur;     // the input state is U0
     c->u0_alias = false;
@@ -1270,6 +1304,7 @@ static int srcint_deferred(mrm_ctx* c, const mrm_run_desc* run, int stage) {
 extern "C" int mrm_source_integrate_tcheck(mrm_ctx* c, const mrm_run_desc* run, int stage) {
   if (!c || !run || stage < 1 || stage > 4) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_source_integrate_tcheck");
+  if (int rc = gp_settle(c)) return rc;
   c->fast_valid = false;
   if (c->res_def && !run->need_diagnostics) return srcint_deferred(c, run, stage);
   if (int rc = drop_settle_k(c, c->dU)) return rc;
@@ -1325,6 +1360,7 @@ extern "C" int mrm_save_cons(mrm_ctx* c) {
 extern "C" int mrm_tcheck(mrm_ctx* c, const mrm_run_desc* run) {
   if (!c) return MRM_ERR_ARG;
   NEED_FP64(c, "mrm_tcheck");
+  if (int rc = gp_settle(c)) return rc;
   double vmax = (c->v_lim > 0.0) ? c->v_lim : c->phys.tchk[3];
   double eps_max = (c->e_lim > 0.0) ? c->e_lim : c->phys.tchk[2];
   if (c->tvd2_def && !(run && run->need_diagnostics)) {
@@ -1381,11 +1417,14 @@ static int exchange_arrays(mrm_ctx* c, double* base, int na);
 
 extern "C" int mrm_exchange_grid(mrm_ctx* c) {
   if (!c) return MRM_ERR_ARG;
-  if (int rc = drop_settle_k(c, c->dU)) return rc;
+  if (int rc = drop_settle_k(c, c->dU, true)) return rc;
   c->fast_valid = false;
   if (c->precision == MRM_PREC_FP32_MIXED) return exchange_mx(c, c->M[c->cur], nullptr);
   int rc = exchange_buf(c, c->dU, nullptr);
-  if (rc || !c->d_spt) return rc;
+  if (rc) return rc;
+  // every ghost cell at interior i written: the dim-0 ghost layers remain
+  if (c->gp_pend == 1) c->gp_pend = 2;
+  if (!c->d_spt) return rc;
   return exchange_arrays(c, c->dF + MRM_SFLX * c->G.bs, 1);
 }
 
@@ -1676,13 +1715,17 @@ extern "C" int mrm_set_boundary_conditions(mrm_ctx* c, const mrm_run_desc* run, 
 extern "C" int mrm_set_boundary_conditions_pt(mrm_ctx* c, const mrm_run_desc* run, int stage, int pt_update) {
   (void)stage;
   if (!c) return MRM_ERR_ARG;
-  if (int rc = drop_settle_k(c, c->dU)) return rc;
+  // after an exchange the recorded ghost cells are the dim-0 layers, which
+  // the boundaries write in every column from interior cells alone
+  if (int rc = drop_settle_k(c, c->dU, c->gp_pend == 2)) return rc;
   if (!c->eos_ok) return fail(c, MRM_ERR_STATE, "mrm_set_boundary_conditions: no EOS table loaded");
   c->fast_valid = false;
   if (int rc = diag_track(c, run)) return rc;
   if (c->precision == MRM_PREC_FP32_MIXED) return bc_mx(c, c->M[c->cur], nullptr, nullptr);
   int rc = bc_buf(c, c->dU, nullptr, nullptr, pt_update);
-  if (rc || !c->d_spt) return rc;
+  if (rc) return rc;
+  c->gp_pend = 0;
+  if (!c->d_spt) return rc;
   const long nyz = (long)c->G.n1 * c->G.n2;
   k_spitzer_bc<<<nblk(nyz), MRM_BLOCK, 0, c->stream>>>(c->G, c->phys.bnd[3] > 0.0, c->dF + MRM_SFLX * c->G.bs);
   LAUNCHCHK(c, "k_spitzer_bc");
@@ -2690,6 +2733,7 @@ static int adjoint_backward_mixed(mrm_ctx* c, const double* seed, double* grad, 
 extern "C" int mrm_adjoint_backward(mrm_ctx* c, const double* seed, double* grad, int fold_initial_ghosts) {
   if (!c || !seed || !grad) return MRM_ERR_ARG;
   if (c->adj_rec.empty()) return fail(c, MRM_ERR_STATE, "mrm_adjoint_backward: no recorded forward");
+  if (int rc = gp_settle(c)) return rc;
   if (c->precision == MRM_PREC_FP32_MIXED) return adjoint_backward_mixed(c, seed, grad, fold_initial_ghosts);
   AdjBufs B;
   int rc = adj_buffers(c, B);

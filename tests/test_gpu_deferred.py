@@ -15,7 +15,11 @@ operation as it is called).
   outputs read after the deferred stage, after exchange and boundary, and
   after stage 2 (a recorded MHD_Residual forms the pressure alone, the
   other four are formed from the recorded state when read), and a
-  temperature written by the caller in between; TVDlimit's recorded dim-2
+  temperature written by the caller in between; the stage output's ghost
+  cells (recorded: the exchange and boundaries overwrite them) read right
+  after the stage, after the exchange alone, with the boundaries before the
+  exchange and after a ConsToPrim with no boundaries before it; TVDlimit's
+  recorded dim-2
   sweep run by TCheck, or plain before a read or a change of the limits,
   whole and split -- each bit for bit the calls with MRM_DEFER=0;
 * three whole steps on the drop-in sequence (mrm_step with MRM_STEP_PATH=
@@ -115,8 +119,23 @@ def _ops(table, gold, defer, read):
     m.Source_Integrate_Tcheck(r, 1)
     if read == "fields_srcint":
         got["f"] = np.stack([m.get_field(f) for f in range(5)])
+    # the ghost cells of the stage's output: recorded, read with every ghost
+    # cell pending (u_srcint), with the dim-0 layers pending (u_exch), taken
+    # over by boundaries before the exchange (bc_first), the dim-0 layers
+    # converted by a ConsToPrim with no boundaries in between (skip_bc: the
+    # primitive state; the EOS of such twice-converted ghost cells is NaN)
+    if read == "u_srcint":
+        got["u"] = m.get_state()
+    if read == "bc_first":
+        m.SetBoundaryConditions(r, 1)
     m.exchange_grid()
-    m.SetBoundaryConditions(r, 1)
+    if read == "u_exch":
+        got["u"] = m.get_state()
+    if read == "skip_bc":
+        m.ConsToPrim()
+        got["u"] = m.get_state()
+    else:
+        m.SetBoundaryConditions(r, 1)
     if read == "fields_bc":
         got["f"] = np.stack([m.get_field(f) for f in range(5)])
     if read in ("fields_stage2", "temp_set"):
@@ -131,7 +150,8 @@ def _ops(table, gold, defer, read):
     return got, m.get_state(), m.get_state(abi.U0), r.dt
 
 
-@pytest.mark.parametrize("read", ["u0", "prim", "res", "fields_srcint", "fields_bc", "fields_stage2", "temp_set"])
+@pytest.mark.parametrize("read", ["u0", "prim", "res", "fields_srcint", "fields_bc", "fields_stage2", "temp_set",
+                                  "u_srcint", "u_exch", "bc_first", "skip_bc"])
 def test_reads_carry_out_recorded_operations(gold, table, read):
     print(f"case: read {read} in the middle of a deferred stage", flush=True)
     g1, U1, U01, dt1 = _ops(table, gold, "1", read)
