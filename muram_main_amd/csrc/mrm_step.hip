@@ -953,10 +953,6 @@ static int c2p_launch(mrm_ctx* c) {
   return MRM_OK;
 }
 
-// the recorded drop-in operations carried out, in the order they were
-// called: SaveCons's copy, ConsToPrim, MHD_Residual's residual.  Every entry
-// point that reads or changes the state outside the deferred stage calls
-// this first.
 // the pending ConsToPrim outputs T, ne, rhoi, amb formed from their state
 static int eosf_settle(mrm_ctx* c) {
   if (c->eosf_pend < 0) return MRM_OK;
@@ -992,6 +988,11 @@ static int gp_settle(mrm_ctx* c) {
   return MRM_OK;
 }
 
+// every recorded drop-in operation carried out: the pending EOS outputs,
+// the stage output's ghost cells, TVDlimit's last sweep, then in the order
+// they were called SaveCons's copy, ConsToPrim and MHD_Residual's residual.
+// Every entry point that reads or changes the state outside the deferred
+// stage calls this, or drop_settle_k for what it touches, first.
 static int drop_settle(mrm_ctx* c) {
   if (int rc = eosf_settle(c)) return rc;
   return drop_settle_k(c, nullptr);
